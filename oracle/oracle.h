@@ -1,0 +1,169 @@
+/*
+ * oracle.h -- CPU restatement (FP64, C99 + OpenMP) of the reference
+ * Chebyshev 1/r FMM path (ScalFMM FChebSymKernel + FInterpMatrixKernelR over
+ * the group tree).
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker used by tests/,
+ * __graft_entry__.smoke() and the cpu_baseline leg of bench.py.  The product
+ * (scalfmm_amd/) never links, loads or calls it.
+ *
+ * Every function cites the reference file:line it restates (paths relative
+ * to the reference's Src/ directory).
+ */
+#ifndef SCALFMM_ORACLE_H
+#define SCALFMM_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ORC_OK 0
+#define ORC_ERR_ARG 1          /* invalid argument (order, height, n<0, NULL) */
+#define ORC_ERR_OUTSIDE_BOX 2  /* a particle lies outside the simulation box */
+#define ORC_ERR_NUMERIC 3      /* SVD failed to converge / rank overflow */
+#define ORC_ERR_NOMEM 4
+
+#define ORC_MAX_ORDER 14
+#define ORC_MIN_ORDER 2
+
+/* ---- indexing (Containers/FTreeCoordinate.hpp:91-142) ------------------ */
+int64_t orc_morton_encode(int x, int y, int z);
+void orc_morton_decode(int64_t key, int32_t out_xyz[3]);
+
+/* Leaf coordinates and Morton keys of n particles
+ * (Containers/FCoordinateComputer.hpp:15-24,42-54; GroupTree/Core/FGroupTree.hpp:199-211).
+ * coords: 3*n int32 (x,y,z per particle) or NULL; keys: n int64. */
+int orc_particle_keys(int64_t n, const double* x, const double* y, const double* z,
+                      const double center[3], double width, int height,
+                      int32_t* coords, int64_t* keys);
+
+/* P2P neighbour keys of a leaf, reference order and positions
+ * (FTreeCoordinate.hpp:189-214).  Returns the count (<=26). */
+int orc_neighbor_keys(int x, int y, int z, int height, int64_t keys[26], int32_t pos[26]);
+
+/* M2L interaction keys of a cell at `level`, reference order and positions
+ * (FTreeCoordinate.hpp:219-264).  Returns the count (<=189). */
+int orc_interaction_keys(int x, int y, int z, int level, int64_t keys[216], int32_t pos[216]);
+
+/* ---- Chebyshev tools (Kernels/Chebyshev/FChebRoots.hpp, FChebTensor.hpp) */
+int orc_cheb_roots(int order, double* roots);          /* hard-coded reference table */
+double orc_cheb_T(unsigned n, double x);                 /* cos(n acos x), clamped */
+double orc_cheb_U(unsigned n, double x);                 /* n sin(n acos x)/sqrt(1-x^2) */
+
+/* Opaque Chebyshev operator set: roots, 1-D child/parent interpolators,
+ * 343 symmetry permutations and the 16 compressed M2L operators. */
+typedef struct orc_cheb orc_cheb;
+int orc_cheb_create(int order, double epsilon, orc_cheb** out);
+void orc_cheb_destroy(orc_cheb* c);
+int orc_cheb_order(const orc_cheb* c);
+/* rank of the operator stored at canonical index idx (0 when absent) */
+int orc_cheb_rank(const orc_cheb* c, int idx);
+/* K[idx] = [U (nnodes x rank) | V (nnodes x rank)], column-major, unweighted
+ * (FChebSymM2LHandler.hpp:165-189,258-261).  out: 2*rank*nnodes doubles. */
+int orc_cheb_get_K(const orc_cheb* c, int idx, double* out);
+/* pindices[343] and pvectors[343][nnodes] (FChebSymM2LHandler.hpp:329-338) */
+void orc_cheb_get_symmetries(const orc_cheb* c, uint32_t* pindices, uint32_t* pvectors);
+/* child/parent 1-D interpolators: 8 children x 3 axes x order^2,
+ * S[n*order+m] = S(x_child_m, xbar_n) (FChebInterpolator.hpp:300-338,466-485) */
+void orc_cheb_get_child_interp(const orc_cheb* c, double* out);
+/* Apply the M2L operator of relative position idx (0..342, far field only)
+ * to one multipole (nnodes) and ADD scale*result into a local (nnodes),
+ * exactly as FChebSymKernel::M2L does for one source (FChebSymKernel.hpp:215-325). */
+int orc_cheb_apply_m2l(const orc_cheb* c, int idx, double scale, const double* M, double* L);
+
+/* Unweighted dense kernel block of relative position idx at reference cell
+ * width 2: A[m*nnodes+n] = K(Y_m, X_n), X = target roots at the origin,
+ * Y = source roots at 2*(i,j,k) (Tests/Utils/testChebSymmetries.cpp:78-98). */
+int orc_cheb_dense_block(int order, int i, int j, int k, double* out);
+
+/* ---- single-leaf / single-cell operators on explicit data -------------- */
+/* P2M of one leaf: M += anterpolation of charges (FChebInterpolator.hpp:625-762) */
+void orc_p2m(const orc_cheb* c, const double center[3], double width, int64_t n,
+             const double* x, const double* y, const double* z, const double* q, double* M);
+/* L2P of one leaf: pot/forces += (FChebInterpolator.hpp:1220-1417) */
+void orc_l2p(const orc_cheb* c, const double center[3], double width, const double* L, int64_t n,
+             const double* x, const double* y, const double* z, const double* q,
+             double* pot, double* fx, double* fy, double* fz);
+/* M2M / L2L for one child (FChebInterpolator.hpp:560-611) */
+void orc_m2m(const orc_cheb* c, int child, const double* Mchild, double* Mparent);
+void orc_l2l(const orc_cheb* c, int child, const double* Lparent, double* Lchild);
+
+/* ---- tree (GroupTree/Core/FGroupTree.hpp:178-360) ------------------------ */
+typedef struct orc_tree orc_tree;
+int orc_tree_build(int64_t n, const double* x, const double* y, const double* z,
+                   const double center[3], double width, int height, orc_tree** out);
+void orc_tree_destroy(orc_tree* t);
+/* sorted particle keys and original indices (ties broken by input index) */
+void orc_tree_sorted(const orc_tree* t, int64_t* keys, int64_t* perm);
+int64_t orc_tree_num_cells(const orc_tree* t, int level);
+void orc_tree_cell_keys(const orc_tree* t, int level, int64_t* keys);
+/* leaf particle ranges in sorted order: start[nleaf], count[nleaf] */
+void orc_tree_leaf_ranges(const orc_tree* t, int64_t* start, int64_t* count);
+/* P2P list of every leaf in CSR form (reference order):
+ * offsets[nleaf+1]; nbr[] = neighbour leaf index; pos[] = 0..26 */
+int64_t orc_tree_p2p_list_size(const orc_tree* t);
+void orc_tree_p2p_list(const orc_tree* t, int64_t* offsets, int64_t* nbr, int32_t* pos);
+/* M2L list of every cell of `level` (2..height-1) in CSR form:
+ * offsets[ncell+1]; src[] = source cell index in the level; pos[] = 0..342 */
+int64_t orc_tree_m2l_list_size(const orc_tree* t, int level);
+void orc_tree_m2l_list(const orc_tree* t, int level, int64_t* offsets, int64_t* src, int32_t* pos);
+
+/* ---- evaluation ----------------------------------------------------------- */
+#define ORC_P2P 1
+#define ORC_P2M 2
+#define ORC_M2M 4
+#define ORC_M2L 8
+#define ORC_L2L 16
+#define ORC_L2P 32
+#define ORC_ALL 63
+
+typedef struct {
+    double t_tree, t_p2m, t_m2m, t_m2l, t_l2l, t_l2p, t_p2p, t_total;
+    int64_t n_m2l, n_p2p_pairs;
+} orc_stats;
+
+/* Full FMM (build tree + passes selected by ops) with results in INPUT order.
+ * Potentials/forces are overwritten (zeroed first).  nthreads<=0: OpenMP default. */
+int orc_fmm(int64_t n, const double* x, const double* y, const double* z, const double* q,
+            const double center[3], double width, int height, int order, double epsilon,
+            unsigned ops, int nthreads,
+            double* pot, double* fx, double* fy, double* fz, orc_stats* stats);
+
+/* Same, with a prebuilt operator set (its order is used). */
+int orc_fmm_with(const orc_cheb* c, int64_t n, const double* x, const double* y, const double* z,
+                 const double* q, const double center[3], double width, int height,
+                 unsigned ops, int nthreads,
+                 double* pot, double* fx, double* fy, double* fz, orc_stats* stats);
+
+/* Expansions of the last level-by-level run for inspection: the tree and
+ * multipole/local arrays ([ncell][nnodes] per level) of an FMM run. */
+typedef struct orc_run orc_run;
+int orc_run_create(const orc_cheb* c, int64_t n, const double* x, const double* y, const double* z,
+                   const double* q, const double center[3], double width, int height,
+                   unsigned ops, orc_run** out);
+void orc_run_destroy(orc_run* r);
+const orc_tree* orc_run_tree(const orc_run* r);
+void orc_run_expansions(const orc_run* r, int level, double* M, double* L);
+
+/* O(N^2) direct sum (Kernels/P2P/FP2PR.hpp:14-42 conventions):
+ * pot_i = sum_{j!=i} q_j/r_ij ; f_i = q_i grad_i(pot_i) = -q_i sum q_j (x_i-x_j)/r^3.
+ * Targets: all particles when targets==NULL (nt ignored), else the listed indices;
+ * outputs have nt (or n) entries. */
+void orc_direct(int64_t n, const double* x, const double* y, const double* z, const double* q,
+                int64_t nt, const int64_t* targets, int nthreads,
+                double* pot, double* fx, double* fy, double* fz);
+
+/* Dense FP64 helpers exported for their own tests. */
+/* Householder QR of A (m x n, col-major, m>=n): Q (m x n) and R (n x n upper). */
+int orc_qr(int m, int n, const double* A, double* Q, double* R);
+/* SVD of a square matrix A (n x n col-major): U (n x n), S (n, descending), VT (n x n). */
+int orc_svd(int n, const double* A, double* U, double* S, double* VT);
+/* Rank rule of FSvd::getRank (Utils/FSvd.hpp:51-62). Returns -1 if none. */
+int orc_svd_rank(const double* S, int n, double eps);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,103 @@
+"""Oracle direct sum and whole FMM against the reference's own data.
+
+Pins:
+  - direct sum vs Data/UTest/unitCubeRef20kDouble.bfma and DirectDouble.bfma
+    (FP64 direct results stored by the reference, force = +q grad phi);
+  - whole FMM (order 7, height 5, the 20k file) vs the errors the reference
+    publishes for FChebSymKernel in Data/ResultCompareKernels.txt:28-35;
+  - the accuracy gates of UTests/FUKernelTester.hpp:148-170 (order 6, height 4,
+    the 5k file): potential < 9e-3, forces < 9e-2, energy < 9e-3.
+"""
+import numpy as np
+import pytest
+
+# Data/ResultCompareKernels.txt:31-35 (FChebSymKernel, ORDER 7, height 5)
+PUBLISHED = {
+    "energy": 3.86893e-10,
+    "pot": (1.13529e-08, 0.000215561, 5.60288e-08),
+    "fx": (1.66035e-06, 0.0123179, 7.70383e-07),
+    "fy": (1.77675e-06, 0.0123182, 3.1319e-06),
+    "fz": (1.79622e-06, 0.0127138, 1.53567e-06),
+}
+
+
+@pytest.mark.parametrize("name", ["fixture5k", "fixture20k"])
+def test_direct_vs_fixture(oracle, request, name):
+    d = request.getfixturevalue(name)["data"]
+    pot, fx, fy, fz = oracle.direct(d[:, 0], d[:, 1], d[:, 2], d[:, 3])
+    for k, got in zip(range(4, 8), (pot, fx, fy, fz)):
+        acc = oracle.accuracy(d[:, k], got)
+        assert acc["rel_l2"] < 1e-13 and acc["rel_inf"] < 1e-12
+
+
+def test_direct_targets_subset(oracle, fixture5k):
+    d = fixture5k["data"]
+    t = np.array([0, 17, 4999, 2500])
+    pot, fx, fy, fz = oracle.direct(d[:, 0], d[:, 1], d[:, 2], d[:, 3], targets=t)
+    assert np.allclose(pot, d[t, 4], rtol=1e-13)
+    assert np.allclose(fz, d[t, 7], rtol=1e-12)
+
+
+def test_fmm_published_errors(oracle, fixture20k):
+    """The oracle's FMM errors match the reference's published run to < 1%."""
+    f = fixture20k
+    d = f["data"]
+    pot, fx, fy, fz, st = oracle.fmm(d[:, 0], d[:, 1], d[:, 2], d[:, 3], f["center"], f["width"], 5, 7)
+    for name, col, got in (("pot", 4, pot), ("fx", 5, fx), ("fy", 6, fy), ("fz", 7, fz)):
+        acc = oracle.accuracy(d[:, col], got)
+        l2, rms, inf = PUBLISHED[name]
+        assert abs(acc["rel_l2"] / l2 - 1) < 0.01, (name, acc)
+        assert abs(acc["rms"] / rms - 1) < 0.01, (name, acc)
+        assert abs(acc["rel_inf"] / inf - 1) < 0.02, (name, acc)
+    e_direct = np.dot(d[:, 4], d[:, 3])
+    e_fmm = np.dot(pot, d[:, 3])
+    assert abs(abs(e_fmm - e_direct) / e_direct / PUBLISHED["energy"] - 1) < 0.02
+    assert st.n_m2l > 0 and st.n_p2p_pairs > 0
+
+
+def test_fmm_unit_test_gates(oracle, fixture5k):
+    """UTests/FUKernelTester.hpp:148-170 with utestChebyshev.cpp:69-80 (order 6, height 4)."""
+    d = fixture5k["data"]
+    pot, fx, fy, fz, _ = oracle.fmm(d[:, 0], d[:, 1], d[:, 2], d[:, 3], fixture5k["center"], fixture5k["width"], 4, 6)
+    assert oracle.accuracy(d[:, 4], pot)["rel_l2"] < 9e-3
+    assert oracle.accuracy(d[:, 4], pot)["rms"] < 9e-3
+    for col, f in ((5, fx), (6, fy), (7, fz)):
+        assert oracle.accuracy(d[:, col], f)["rel_l2"] < 9e-2
+    e_direct = np.dot(d[:, 4], d[:, 3])
+    assert abs(np.dot(pot, d[:, 3]) - e_direct) / e_direct < 9e-3
+
+
+def test_fmm_near_plus_far_is_everything(oracle):
+    """P2P-only plus far-field-only equals the full run (passes are additive)."""
+    rng = np.random.default_rng(4)
+    n = 3000
+    x, y, z = rng.random((3, n))
+    q = rng.uniform(-1, 1, n)
+    c = oracle.Cheb(5)
+    full = oracle.fmm(x, y, z, q, [0.5] * 3, 1.0, 4, 5, cheb=c)
+    near = oracle.fmm(x, y, z, q, [0.5] * 3, 1.0, 4, 5, ops=oracle.P2P, cheb=c)
+    far = oracle.fmm(x, y, z, q, [0.5] * 3, 1.0, 4, 5, ops=oracle.ALL & ~oracle.P2P, cheb=c)
+    for a, b, cc in zip(full[:4], near[:4], far[:4]):
+        assert np.allclose(a, b + cc, rtol=1e-13, atol=1e-13 * np.abs(a).max())
+
+
+def test_fmm_rejects_outside_particles(oracle):
+    x = np.array([0.1, 0.2, 1.5]); y = np.array([0.1, 0.2, 0.3]); z = np.array([0.1, 0.2, 0.3])
+    with pytest.raises(RuntimeError):
+        oracle.fmm(x, y, z, np.ones(3), [0.5] * 3, 1.0, 3, 5)
+
+
+def test_fmm_order_sweep_accuracy(oracle):
+    """Error falls with the order on a uniform cube (sanity of every order)."""
+    rng = np.random.default_rng(9)
+    n = 4000
+    x, y, z = rng.random((3, n))
+    q = rng.uniform(-1, 1, n)
+    ref = oracle.direct(x, y, z, q)
+    errs = []
+    for p in (4, 5, 6, 7, 8):
+        pot = oracle.fmm(x, y, z, q, [0.5] * 3, 1.0, 4, p)[0]
+        errs.append(oracle.accuracy(ref[0], pot)["rel_l2"])
+    # orders 4-5 carry the early-stopped ACA operators (see test_oracle_cheb)
+    assert max(errs[:2]) < 2e-3
+    assert errs[2] > errs[3] > errs[4] and errs[4] < 1e-6
