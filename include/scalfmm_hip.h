@@ -1,0 +1,172 @@
+/*
+ * scalfmm_hip.h -- C-ABI of the MI355X Chebyshev 1/r FMM engine.
+ *
+ * The engine replaces the reference's Chebyshev 1/r path: the kernel class
+ * FChebSymKernel<FReal,Cell,Container,FInterpMatrixKernelR<FReal>,ORDER,NVALS>
+ * (Src/Kernels/Chebyshev/FChebSymKernel.hpp:45-488) driven through the plugin
+ * interface FAbstractKernels (Src/Components/FAbstractKernels.hpp:23-210) over
+ * the blocked octree FGroupTree (Src/GroupTree/Core/FGroupTree.hpp:178-360)
+ * by FGroupTaskAlgorithm / FGroupSeqAlgorithm.  Instead of one virtual call per
+ * cell, a context owns the device tree and runs whole levels per launch.
+ *
+ * Conventions
+ *   - every entry point returns an int status (HFMM_OK = 0); no exception
+ *     crosses the ABI; hfmm_last_error() gives the message of the last failure;
+ *   - the context owns all device memory; the caller owns every host array;
+ *   - one context per host thread; a context runs on one HIP stream of its
+ *     device; calls on one context are not re-entrant;
+ *   - particle inputs and results are in the caller's (input) order.
+ *
+ * Reference file:line citations are relative to the reference's source tree.
+ */
+#ifndef SCALFMM_HIP_H
+#define SCALFMM_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes --------------------------------------------------------- */
+#define HFMM_OK 0
+#define HFMM_ERR_ARG 1         /* invalid argument (order, height, NULL, size) */
+#define HFMM_ERR_OUTSIDE_BOX 2 /* a particle lies outside the box: the reference
+                                  aborts in FCoordinateComputer.hpp:18 (FAssertLF) */
+#define HFMM_ERR_NUMERIC 3     /* operator compression failed (the reference throws
+                                  std::runtime_error, FChebSymM2LHandler.hpp:156-160) */
+#define HFMM_ERR_NOMEM 4       /* host or device allocation failed */
+#define HFMM_ERR_DEVICE 5      /* a HIP runtime call failed */
+#define HFMM_ERR_STATE 6       /* call out of order (e.g. evaluate before build) */
+
+/* ---- operation bits: the values of FFmmOperations (Src/Core/FCoreCommon.hpp:17-29) */
+#define HFMM_P2P (1u << 0)
+#define HFMM_P2M (1u << 1)
+#define HFMM_M2M (1u << 2)
+#define HFMM_M2L (1u << 3)
+#define HFMM_L2L (1u << 4)
+#define HFMM_L2P (1u << 5)
+#define HFMM_NEAR_FIELD HFMM_P2P
+#define HFMM_FAR_FIELD (HFMM_P2M | HFMM_M2M | HFMM_M2L | HFMM_L2L | HFMM_L2P)
+#define HFMM_NEAR_AND_FAR (HFMM_NEAR_FIELD | HFMM_FAR_FIELD)
+
+/* ---- precision modes ------------------------------------------------------ */
+#define HFMM_PRECISION_FP64 0 /* every kernel in FP64 (the reference's FReal=double) */
+
+typedef struct hfmm_ctx hfmm_ctx;
+
+/* Kernel parameters: the arguments of the FChebSymKernel constructor
+ * (FChebSymKernel.hpp:110-140: treeHeight, boxWidth, boxCenter, epsilon) plus
+ * the template parameter ORDER and the device to run on. */
+typedef struct hfmm_params {
+    int32_t tree_height;   /* 3..20 (levels 0..height-1; leaves at height-1) */
+    int32_t order;         /* Chebyshev interpolation order ORDER: 5, 6, 7 or 8 */
+    double box_width;      /* width of the cubic simulation box */
+    double box_center[3];  /* centre of the box */
+    double epsilon;        /* M2L compression accuracy; <= 0 selects 10^-order */
+    int32_t precision;     /* HFMM_PRECISION_* */
+    int32_t device;        /* HIP device ordinal */
+} hfmm_params;
+
+/* Creates a context: builds the Chebyshev roots, child/parent interpolators,
+ * the 343 symmetry permutations and the 16 compressed M2L operators
+ * (replaces the FChebSymKernel constructor, FChebSymKernel.hpp:110-126 and
+ * SymmetryHandler, FChebSymM2LHandler.hpp:321-344). */
+int hfmm_create(const hfmm_params* params, hfmm_ctx** out);
+int hfmm_destroy(hfmm_ctx* ctx);
+/* Message of the last failure on this context (never NULL). */
+const char* hfmm_last_error(const hfmm_ctx* ctx);
+
+/* Copies n particles (host SoA, one charge per particle) to the device.
+ * Replaces filling the FP2PParticleContainer handed to the FGroupTree
+ * constructor (Tests/GroupTree/testBlockedChebyshev.cpp:98-114). */
+int hfmm_set_particles(hfmm_ctx* ctx, int64_t n, const double* x, const double* y, const double* z,
+                       const double* q);
+/* Same with arrays already resident in device memory of the context's device;
+ * the context reads them during hfmm_build_tree and hfmm_evaluate, so they must
+ * stay valid until the next hfmm_set_particles*. */
+int hfmm_set_particles_device(hfmm_ctx* ctx, int64_t n, const double* x, const double* y, const double* z,
+                              const double* q);
+
+/* Builds the octree on the device: leaf coordinates and Morton keys
+ * (FCoordinateComputer.hpp:42-54, FTreeCoordinate.hpp:91-116), the sort by key
+ * (FGroupTree.hpp:215-219), leaves and parent levels (FGroupTree.hpp:224-360),
+ * the P2P neighbour lists (FTreeCoordinate.hpp:189-214) and the M2L
+ * interaction lists (FTreeCoordinate.hpp:219-264). */
+int hfmm_build_tree(hfmm_ctx* ctx);
+
+/* Runs the passes selected by ops (HFMM_* bits) over the built tree; replaces
+ * FAbstractAlgorithm::execute(operationsToProceed) (Src/Core/FCoreCommon.hpp:73-78)
+ * with FChebSymKernel's P2M/M2M/M2L/L2L/L2P/P2P (FChebSymKernel.hpp:181-486).
+ * Potentials and forces are reset, then accumulated by the selected passes. */
+int hfmm_evaluate(hfmm_ctx* ctx, uint32_t ops);
+
+/* Waits for the context's stream. */
+int hfmm_synchronize(hfmm_ctx* ctx);
+
+/* Copies potentials and forces (input order) to host arrays of n doubles.
+ * force = q * grad(potential), the sign convention of the reference's P2P and
+ * L2P (FP2P.hpp:139-162, FChebInterpolator.hpp:1405-1410). Any pointer may be NULL. */
+int hfmm_get_results(hfmm_ctx* ctx, double* pot, double* fx, double* fy, double* fz);
+/* Device pointers (owned by the context, input order) of the same results. */
+int hfmm_get_results_device(hfmm_ctx* ctx, const double** pot, const double** fx, const double** fy,
+                            const double** fz);
+
+/* ---- inspection (used by the parity tests) -------------------------------- */
+typedef struct hfmm_tree_info {
+    int64_t n_particles;
+    int32_t height;
+    int32_t order;
+    int64_t n_cells[21];       /* per level; n_cells[height-1] = leaves */
+    int64_t p2p_entries;       /* total P2P list entries over all leaves */
+    int64_t m2l_entries[21];   /* total M2L list entries per level */
+} hfmm_tree_info;
+int hfmm_get_tree_info(hfmm_ctx* ctx, hfmm_tree_info* info);
+
+/* Sorted particle keys and the original index of each sorted particle. */
+int hfmm_get_sorted_keys(hfmm_ctx* ctx, int64_t* keys, int64_t* perm);
+/* Morton keys of the cells of one level, ascending. */
+int hfmm_get_cell_keys(hfmm_ctx* ctx, int32_t level, int64_t* keys);
+/* Leaf particle ranges in sorted order. */
+int hfmm_get_leaf_ranges(hfmm_ctx* ctx, int64_t* start, int64_t* count);
+/* P2P list in CSR form and reference order: offsets[n_leaves+1], nbr[] = leaf
+ * index, pos[] = 0..26 (the neighborPositions of FAbstractKernels::P2P). */
+int hfmm_get_p2p_list(hfmm_ctx* ctx, int64_t* offsets, int64_t* nbr, int32_t* pos);
+/* M2L list of a level (2..height-1) in CSR form and reference order:
+ * offsets[n_cells+1], src[] = cell index in the level, pos[] = 0..342. */
+int hfmm_get_m2l_list(hfmm_ctx* ctx, int32_t level, int64_t* offsets, int64_t* src, int32_t* pos);
+/* Compressed operator of canonical position idx (FChebSymM2LHandler.hpp:165-189):
+ * *rank, and K = [U | V] column-major nnodes x rank each (2*rank*nnodes doubles,
+ * K may be NULL to query the rank). */
+int hfmm_get_operator(hfmm_ctx* ctx, int32_t idx, int32_t* rank, double* K);
+/* Multipole and local expansions of a level after hfmm_evaluate:
+ * [n_cells][order^3] each, either pointer may be NULL. */
+int hfmm_get_expansions(hfmm_ctx* ctx, int32_t level, double* M, double* L);
+
+/* ---- timing ------------------------------------------------------------- */
+/* When enabled, every kernel launch is bracketed by HIP events on the
+ * context's stream; hfmm_get_kernel_times() synchronizes and reports the sum of
+ * event durations and the launch count per kernel name. */
+typedef struct hfmm_kernel_time {
+    char name[48];
+    double total_ms;
+    int64_t launches;
+} hfmm_kernel_time;
+int hfmm_set_profiling(hfmm_ctx* ctx, int32_t enable);
+int hfmm_get_kernel_times(hfmm_ctx* ctx, hfmm_kernel_time* out, int32_t max, int32_t* count);
+int hfmm_reset_kernel_times(hfmm_ctx* ctx);
+
+/* ---- diagnostics ------------------------------------------------------------ */
+/* Runs one v_mfma_f64_16x16x4f64 on A[i][k] = 1 + 4i + k, B[k][j] = 0.5 + 16k + j
+ * and writes D = A B (16 x 16, row-major) as read back through the operand
+ * layout the M2L kernel assumes; no counterpart in the reference (a check of
+ * the CDNA4 matrix-core lane layout). */
+int hfmm_selftest_mfma_f64(hfmm_ctx* ctx, double* out);
+
+/* Library version string. */
+const char* hfmm_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,96 @@
+"""ctypes declarations of the C-ABI in include/scalfmm_hip.h.
+
+The product path has no CPU fallback: when the HIP library is missing,
+`lib()` raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "_lib", "libscalfmm_hip.so")
+
+OK, ERR_ARG, ERR_OUTSIDE_BOX, ERR_NUMERIC, ERR_NOMEM, ERR_DEVICE, ERR_STATE = range(7)
+
+# FFmmOperations (Src/Core/FCoreCommon.hpp:17-29)
+P2P, P2M, M2M, M2L, L2L, L2P = 1, 2, 4, 8, 16, 32
+NEAR_FIELD = P2P
+FAR_FIELD = P2M | M2M | M2L | L2L | L2P
+NEAR_AND_FAR = NEAR_FIELD | FAR_FIELD
+
+PRECISION_FP64 = 0
+
+# every symbol the header declares
+EXPORTS = (
+    "hfmm_create", "hfmm_destroy", "hfmm_last_error", "hfmm_set_particles", "hfmm_set_particles_device",
+    "hfmm_build_tree", "hfmm_evaluate", "hfmm_synchronize", "hfmm_get_results", "hfmm_get_results_device",
+    "hfmm_get_tree_info", "hfmm_get_sorted_keys", "hfmm_get_cell_keys", "hfmm_get_leaf_ranges",
+    "hfmm_get_p2p_list", "hfmm_get_m2l_list", "hfmm_get_operator", "hfmm_get_expansions",
+    "hfmm_set_profiling", "hfmm_get_kernel_times", "hfmm_reset_kernel_times", "hfmm_selftest_mfma_f64",
+    "hfmm_version",
+)
+
+
+class Params(C.Structure):
+    _fields_ = [("tree_height", C.c_int32), ("order", C.c_int32), ("box_width", C.c_double),
+                ("box_center", C.c_double * 3), ("epsilon", C.c_double), ("precision", C.c_int32),
+                ("device", C.c_int32)]
+
+
+class TreeInfo(C.Structure):
+    _fields_ = [("n_particles", C.c_int64), ("height", C.c_int32), ("order", C.c_int32),
+                ("n_cells", C.c_int64 * 21), ("p2p_entries", C.c_int64), ("m2l_entries", C.c_int64 * 21)]
+
+
+class KernelTime(C.Structure):
+    _fields_ = [("name", C.c_char * 48), ("total_ms", C.c_double), ("launches", C.c_int64)]
+
+
+_lib = None
+
+
+class HipLibraryMissing(RuntimeError):
+    pass
+
+
+def lib(path: str | None = None):
+    """Loads the HIP engine (no fallback)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    p = path or LIB_PATH
+    if not os.path.exists(p):
+        raise HipLibraryMissing(f"HIP engine not built: {p} is missing (run __graft_entry__.build())")
+    L = C.CDLL(p)
+    vp, i32, i64, d = C.c_void_p, C.c_int32, C.c_int64, C.c_double
+    dp = C.POINTER(C.c_double)
+    L.hfmm_create.argtypes = [C.POINTER(Params), C.POINTER(vp)]
+    L.hfmm_destroy.argtypes = [vp]
+    L.hfmm_last_error.restype = C.c_char_p
+    L.hfmm_last_error.argtypes = [vp]
+    L.hfmm_set_particles.argtypes = [vp, i64, dp, dp, dp, dp]
+    L.hfmm_set_particles_device.argtypes = [vp, i64, vp, vp, vp, vp]
+    L.hfmm_build_tree.argtypes = [vp]
+    L.hfmm_evaluate.argtypes = [vp, C.c_uint32]
+    L.hfmm_synchronize.argtypes = [vp]
+    L.hfmm_get_results.argtypes = [vp, dp, dp, dp, dp]
+    L.hfmm_get_results_device.argtypes = [vp, C.POINTER(vp), C.POINTER(vp), C.POINTER(vp), C.POINTER(vp)]
+    L.hfmm_get_tree_info.argtypes = [vp, C.POINTER(TreeInfo)]
+    L.hfmm_get_sorted_keys.argtypes = [vp, C.POINTER(i64), C.POINTER(i64)]
+    L.hfmm_get_cell_keys.argtypes = [vp, i32, C.POINTER(i64)]
+    L.hfmm_get_leaf_ranges.argtypes = [vp, C.POINTER(i64), C.POINTER(i64)]
+    L.hfmm_get_p2p_list.argtypes = [vp, C.POINTER(i64), C.POINTER(i64), C.POINTER(i32)]
+    L.hfmm_get_m2l_list.argtypes = [vp, i32, C.POINTER(i64), C.POINTER(i64), C.POINTER(i32)]
+    L.hfmm_get_operator.argtypes = [vp, i32, C.POINTER(i32), dp]
+    L.hfmm_get_expansions.argtypes = [vp, i32, dp, dp]
+    L.hfmm_set_profiling.argtypes = [vp, i32]
+    L.hfmm_get_kernel_times.argtypes = [vp, C.POINTER(KernelTime), i32, C.POINTER(i32)]
+    L.hfmm_reset_kernel_times.argtypes = [vp]
+    L.hfmm_selftest_mfma_f64.argtypes = [vp, dp]
+    L.hfmm_version.restype = C.c_char_p
+    for name in EXPORTS:
+        getattr(L, name).restype = getattr(L, name).restype if name in ("hfmm_last_error", "hfmm_version") \
+            else C.c_int
+    _lib = L
+    return L

@@ -1,0 +1,623 @@
+// fmm_device.hip -- FP64 FMM passes of the Chebyshev 1/r kernel on CDNA4.
+//
+//   P2M  per leaf: M_n += sum_p q_p Sx(xi_p, n_x) Sy(.., n_y) Sz(.., n_z)
+//        (Src/Kernels/Chebyshev/FChebInterpolator.hpp:625-762)
+//   M2M  per parent: three 1-D contractions per child (FChebInterpolator.hpp:560-584)
+//   M2L  per level, on MFMA: for tiles of 16 targets of one parity and each
+//        of the 189 far-field slots, C = V_c^T P(M_src) then L += U_c P^-1 C
+//        (Src/Kernels/Chebyshev/FChebSymKernel.hpp:210-328)
+//   L2L  per parent: transposed contractions (FChebInterpolator.hpp:587-611)
+//   L2P  per leaf: potential and q * gradient (FChebInterpolator.hpp:1220-1417)
+//   P2P  per target particle over the 27-leaf neighbourhood
+//        (Src/Kernels/P2P/FP2P.hpp:103-269 conventions)
+#include <hip/hip_runtime.h>
+
+#include "hfmm_internal.hpp"
+
+namespace hfmm {
+namespace {
+
+typedef double d4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ uint32_t compact3(uint64_t x) {
+    x &= 0x1249249249249249ull;
+    x = (x ^ (x >> 2)) & 0x10c30c30c30c30c3ull;
+    x = (x ^ (x >> 4)) & 0x100f00f00f00f00full;
+    x = (x ^ (x >> 8)) & 0x1f0000ff0000ffull;
+    x = (x ^ (x >> 16)) & 0x1f00000000ffffull;
+    x = (x ^ (x >> 32)) & 0x1fffffull;
+    return static_cast<uint32_t>(x);
+}
+
+struct Box {
+    double corner[3];
+    double leafw;
+};
+
+// Leaf centre (getLeafCellCenter, FAbstractChebKernel.hpp:53-58) and the
+// affine map bounds a = c - w/2, b = c + w/2 (FInterpMapping.hpp:30-37).
+__device__ __forceinline__ void leaf_bounds(const Box& bx, uint64_t key, double a[3], double b[3]) {
+    const int co[3] = {static_cast<int>(compact3(key >> 2)), static_cast<int>(compact3(key >> 1)),
+                       static_cast<int>(compact3(key))};
+    for (int d = 0; d < 3; ++d) {
+        const double c = bx.corner[d] + (static_cast<double>(co[d]) + .5) * bx.leafw;
+        a[d] = c - bx.leafw / 2.;
+        b[d] = c + bx.leafw / 2.;
+    }
+}
+
+// S(xi, root_j) = 1/p + 2/p sum_{o>=1} T_o(xi) T_o(root_j); dS = d/dxi S.
+template <int P, bool GRAD>
+__device__ __forceinline__ void interp_weights(double xi, const double* __restrict__ sT, double* S, double* dS) {
+    double T[P], U[P];
+    T[0] = 1.;
+    if (P > 1) T[1] = xi;
+    U[0] = 1.;
+    if (P > 1) U[1] = 2. * xi;
+#pragma unroll
+    for (int j = 2; j < P; ++j) {
+        T[j] = 2. * xi * T[j - 1] - T[j - 2];
+        if (GRAD) U[j] = 2. * xi * U[j - 1] - U[j - 2];
+    }
+#pragma unroll
+    for (int j = 0; j < P; ++j) {
+        double s = 1. / P, ds = 0.;
+#pragma unroll
+        for (int o = 1; o < P; ++o) {
+            const double t = sT[o * P + j];
+            s += 2. / P * T[o] * t;
+            if (GRAD) ds += 2. / P * (o * U[o - 1]) * t;
+        }
+        S[j] = s;
+        if (GRAD) dS[j] = ds;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// P2M: one wave per leaf, 64 particles per round staged in LDS.
+template <int P>
+__global__ __launch_bounds__(64) void k_p2m(int nleaf, const uint64_t* __restrict__ lkeys,
+                                           const int32_t* __restrict__ lstart, const double* __restrict__ sp,
+                                           int64_t n, Box bx, const double* __restrict__ Troots,
+                                           double* __restrict__ M, int LD) {
+    constexpr int NN = P * P * P;
+    constexpr int PER = (NN + 63) / 64;
+    __shared__ double sS[64][3 * P];
+    __shared__ double sQ[64];
+    __shared__ double sT[P * P];
+    const int leaf = blockIdx.x;
+    const int lane = threadIdx.x;
+    for (int i = lane; i < P * P; i += 64) sT[i] = Troots[i];
+    double a[3], b[3];
+    leaf_bounds(bx, lkeys[leaf], a, b);
+    double acc[PER];
+#pragma unroll
+    for (int t = 0; t < PER; ++t) acc[t] = 0.;
+    const int s0 = lstart[leaf], s1 = lstart[leaf + 1];
+    __syncthreads();
+    for (int base = s0; base < s1; base += 64) {
+        const int cnt = min(64, s1 - base);
+        if (lane < cnt) {
+            const int64_t e = base + lane;
+            const double pos[3] = {sp[e], sp[n + e], sp[2 * n + e]};
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                const double xi = (2. * pos[d] - b[d] - a[d]) / (b[d] - a[d]);
+                interp_weights<P, false>(xi, sT, &sS[lane][d * P], nullptr);
+            }
+            sQ[lane] = sp[3 * n + e];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int t = 0; t < PER; ++t) {
+            const int idx = lane + 64 * t;
+            if (idx < NN) {
+                const int i = idx % P, j = (idx / P) % P, k = idx / (P * P);
+                double s = 0.;
+                for (int m = 0; m < cnt; ++m) s += sQ[m] * sS[m][P + j] * sS[m][2 * P + k] * sS[m][i];
+                acc[t] += s;
+            }
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int t = 0; t < PER; ++t) {
+        const int idx = lane + 64 * t;
+        if (idx < NN) M[static_cast<int64_t>(leaf) * LD + idx] = acc[t];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// M2M: one wave per parent; child[m] -> parent[n] += Sx[nx][mx] Sy[ny][my] Sz[nz][mz] child[m].
+template <int P>
+__global__ __launch_bounds__(64) void k_m2m(const int32_t* __restrict__ cstart, const uint64_t* __restrict__ ckeys,
+                                           const double* __restrict__ interp, const double* __restrict__ Mc,
+                                           double* __restrict__ Mp, int LD) {
+    constexpr int NN = P * P * P;
+    constexpr int PER = (NN + 63) / 64;
+    __shared__ double sI[8 * 3 * P * P];
+    __shared__ double b0[NN], b1[NN];
+    const int parent = blockIdx.x, lane = threadIdx.x;
+    for (int i = lane; i < 8 * 3 * P * P; i += 64) sI[i] = interp[i];
+    double acc[PER];
+#pragma unroll
+    for (int t = 0; t < PER; ++t) acc[t] = 0.;
+    const int c0 = cstart[parent], c1 = cstart[parent + 1];
+    for (int ch = c0; ch < c1; ++ch) {
+        const int cidx = static_cast<int>(ckeys[ch] & 7u);
+        const double* Sx = sI + (cidx * 3 + 0) * P * P;
+        const double* Sy = sI + (cidx * 3 + 1) * P * P;
+        const double* Sz = sI + (cidx * 3 + 2) * P * P;
+        for (int i = lane; i < NN; i += 64) b0[i] = Mc[static_cast<int64_t>(ch) * LD + i];
+        __syncthreads();
+        for (int idx = lane; idx < NN; idx += 64) {  // x
+            const int nx = idx % P, rest = idx / P;
+            double s = 0.;
+#pragma unroll
+            for (int m = 0; m < P; ++m) s += Sx[nx * P + m] * b0[rest * P + m];
+            b1[idx] = s;
+        }
+        __syncthreads();
+        for (int idx = lane; idx < NN; idx += 64) {  // y
+            const int i = idx % P, ny = (idx / P) % P, k = idx / (P * P);
+            double s = 0.;
+#pragma unroll
+            for (int m = 0; m < P; ++m) s += Sy[ny * P + m] * b1[k * P * P + m * P + i];
+            b0[idx] = s;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int t = 0; t < PER; ++t) {  // z
+            const int idx = lane + 64 * t;
+            if (idx < NN) {
+                const int i = idx % P, j = (idx / P) % P, nz = idx / (P * P);
+                double s = 0.;
+#pragma unroll
+                for (int m = 0; m < P; ++m) s += Sz[nz * P + m] * b0[m * P * P + j * P + i];
+                acc[t] += s;
+            }
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int t = 0; t < PER; ++t) {
+        const int idx = lane + 64 * t;
+        if (idx < NN) Mp[static_cast<int64_t>(parent) * LD + idx] = acc[t];
+    }
+}
+
+// L2L: one wave per parent; child[m] += sum_n Sx[nx][mx] Sy[ny][my] Sz[nz][mz] parent[n].
+template <int P>
+__global__ __launch_bounds__(64) void k_l2l(const int32_t* __restrict__ cstart, const uint64_t* __restrict__ ckeys,
+                                           const double* __restrict__ interp, const double* __restrict__ Lp,
+                                           double* __restrict__ Lc, int LD) {
+    constexpr int NN = P * P * P;
+    __shared__ double sI[8 * 3 * P * P];
+    __shared__ double par[NN], b1[NN], b2[NN];
+    const int parent = blockIdx.x, lane = threadIdx.x;
+    for (int i = lane; i < 8 * 3 * P * P; i += 64) sI[i] = interp[i];
+    for (int i = lane; i < NN; i += 64) par[i] = Lp[static_cast<int64_t>(parent) * LD + i];
+    __syncthreads();
+    const int c0 = cstart[parent], c1 = cstart[parent + 1];
+    for (int ch = c0; ch < c1; ++ch) {
+        const int cidx = static_cast<int>(ckeys[ch] & 7u);
+        const double* Sx = sI + (cidx * 3 + 0) * P * P;
+        const double* Sy = sI + (cidx * 3 + 1) * P * P;
+        const double* Sz = sI + (cidx * 3 + 2) * P * P;
+        for (int idx = lane; idx < NN; idx += 64) {  // x
+            const int mx = idx % P, rest = idx / P;
+            double s = 0.;
+#pragma unroll
+            for (int nn = 0; nn < P; ++nn) s += Sx[nn * P + mx] * par[rest * P + nn];
+            b1[idx] = s;
+        }
+        __syncthreads();
+        for (int idx = lane; idx < NN; idx += 64) {  // y
+            const int i = idx % P, my = (idx / P) % P, k = idx / (P * P);
+            double s = 0.;
+#pragma unroll
+            for (int nn = 0; nn < P; ++nn) s += Sy[nn * P + my] * b1[k * P * P + nn * P + i];
+            b2[idx] = s;
+        }
+        __syncthreads();
+        for (int idx = lane; idx < NN; idx += 64) {  // z
+            const int i = idx % P, j = (idx / P) % P, mz = idx / (P * P);
+            double s = 0.;
+#pragma unroll
+            for (int nn = 0; nn < P; ++nn) s += Sz[nn * P + mz] * b2[nn * P * P + j * P + i];
+            Lc[static_cast<int64_t>(ch) * LD + idx] += s;
+        }
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// L2P: one wave per leaf, one lane per particle; adds into the sorted results.
+template <int P>
+__global__ __launch_bounds__(64) void k_l2p(int nleaf, const uint64_t* __restrict__ lkeys,
+                                           const int32_t* __restrict__ lstart, const double* __restrict__ sp,
+                                           int64_t n, Box bx, const double* __restrict__ Troots,
+                                           const double* __restrict__ L, int LD, double* __restrict__ res) {
+    constexpr int NN = P * P * P;
+    __shared__ double sL[NN];
+    __shared__ double sT[P * P];
+    const int leaf = blockIdx.x, lane = threadIdx.x;
+    for (int i = lane; i < P * P; i += 64) sT[i] = Troots[i];
+    for (int i = lane; i < NN; i += 64) sL[i] = L[static_cast<int64_t>(leaf) * LD + i];
+    double a[3], b[3];
+    leaf_bounds(bx, lkeys[leaf], a, b);
+    __syncthreads();
+    const int s0 = lstart[leaf], s1 = lstart[leaf + 1];
+    for (int e = s0 + lane; e < s1; e += 64) {
+        const double pos[3] = {sp[e], sp[n + e], sp[2 * n + e]};
+        double S[3][P], dS[3][P], jac[3];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            const double xi = (2. * pos[d] - b[d] - a[d]) / (b[d] - a[d]);
+            interp_weights<P, true>(xi, sT, S[d], dS[d]);
+            jac[d] = 2. / (b[d] - a[d]);
+        }
+        double ph = 0., gx = 0., gy = 0., gz = 0.;
+        for (int k = 0; k < P; ++k) {
+#pragma unroll
+            for (int j = 0; j < P; ++j) {
+                double s0v = 0., s1v = 0.;
+#pragma unroll
+                for (int i = 0; i < P; ++i) {
+                    const double l = sL[(k * P + j) * P + i];
+                    s0v += l * S[0][i];
+                    s1v += l * dS[0][i];
+                }
+                ph += s0v * S[1][j] * S[2][k];
+                gx += s1v * S[1][j] * S[2][k];
+                gy += s0v * dS[1][j] * S[2][k];
+                gz += s0v * S[1][j] * dS[2][k];
+            }
+        }
+        const double q = sp[3 * n + e];
+        res[e] += ph;
+        res[n + e] += gx * jac[0] * q;
+        res[2 * n + e] += gy * jac[1] * q;
+        res[3 * n + e] += gz * jac[2] * q;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// P2P: one thread per target particle, sources over its own leaf and the 26
+// neighbour slots; writes (initialises) the sorted results.
+__global__ __launch_bounds__(256) void k_p2p_f64(int64_t n, const int32_t* __restrict__ pleaf,
+                                                const int32_t* __restrict__ lstart, const int32_t* __restrict__ slots,
+                                                const double* __restrict__ sp, double* __restrict__ res) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int li = pleaf[i];
+    const double tx = sp[i], ty = sp[n + i], tz = sp[2 * n + i], tq = sp[3 * n + i];
+    double pot = 0., fx = 0., fy = 0., fz = 0.;
+    for (int s = -1; s < kNbrSlots; ++s) {
+        const int nb = s < 0 ? li : slots[static_cast<int64_t>(li) * kNbrSlots + s];
+        if (nb < 0) continue;
+        const int j0 = lstart[nb], j1 = lstart[nb + 1];
+        for (int j = j0; j < j1; ++j) {
+            if (j == i) continue;
+            const double dx = tx - sp[j], dy = ty - sp[n + j], dz = tz - sp[2 * n + j];
+            const double inv = 1.0 / sqrt(dx * dx + dy * dy + dz * dz);
+            const double inv3 = inv * inv * inv;
+            const double qs = sp[3 * n + j];
+            pot += inv * qs;
+            fx += -inv3 * dx * qs;
+            fy += -inv3 * dy * qs;
+            fz += -inv3 * dz * qs;
+        }
+    }
+    res[i] = pot;
+    res[n + i] = fx * tq;
+    res[2 * n + i] = fy * tq;
+    res[3 * n + i] = fz * tq;
+}
+
+// ---------------------------------------------------------------------------
+// M2L on FP64 MFMA (v_mfma_f64_16x16x4f64).  A workgroup of 4 waves handles a
+// tile of 16 targets of one parity at one level.  For every far-field slot j
+// (relative position pos, canonical operator c, node permutation pv):
+//   GEMM1  C (kpad x 16) = V_c[pv[n]][r]^T * M_src[n]    K = n split over the 4 waves
+//   GEMM2  acc (nn x 16) += U_c[pv[n]][r] * C            rows split over the 4 waves
+// The permutations are applied as row gathers of the operators, the partial C
+// tiles meet in LDS (double buffered: one barrier per slot), and every target
+// belongs to exactly one tile, so L is written without atomics.
+// MFMA 16x16x4 f64 operand layout: A[i][k] in lane i + 16k, B[k][j] in lane
+// j + 16k, D[4*(l/16)+r][l%16] in register r of lane l.
+constexpr int kM2LWaves = 4;
+constexpr int kMaxRankTiles = 4;
+
+template <int P>
+__global__ __launch_bounds__(256) void k_m2l_f64(const int32_t* __restrict__ slots, const int32_t* __restrict__ order,
+                                                const int32_t* __restrict__ pcount, const int16_t* __restrict__ slotpos,
+                                                const uint32_t* __restrict__ pidx, const uint16_t* __restrict__ pvec,
+                                                const int32_t* __restrict__ rank, const int64_t* __restrict__ opoff,
+                                                const double* __restrict__ ops, const double* __restrict__ Mlev,
+                                                double* __restrict__ Llev, double scale, int LD, int kpadmax) {
+    constexpr int NN = P * P * P;
+    constexpr int W = kM2LWaves;
+    constexpr int NT = (NN + 15) / 16;
+    constexpr int NTW = (NT + W - 1) / W;
+    constexpr int KS = (NN + 3) / 4;
+    constexpr int KSW = (KS + W - 1) / W;
+    extern __shared__ double smem[];  // [2][W][kpadmax][16]
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int col = lane & 15, kq = lane >> 4;
+
+    int e = -1, first = 0, cnt = 0;
+    {
+        int tacc = 0, cacc = 0;
+        for (int q = 0; q < 8; ++q) {
+            const int c = pcount[q];
+            const int t = (c + 15) / 16;
+            if (e < 0 && static_cast<int>(blockIdx.x) < tacc + t) {
+                e = q;
+                first = cacc + (static_cast<int>(blockIdx.x) - tacc) * 16;
+                cnt = min(16, c - (static_cast<int>(blockIdx.x) - tacc) * 16);
+            }
+            tacc += t;
+            cacc += c;
+        }
+    }
+    if (e < 0) return;  // uniform over the workgroup
+    const int tgt = col < cnt ? order[first + col] : -1;
+
+    d4 acc[NTW];
+#pragma unroll
+    for (int t = 0; t < NTW; ++t) acc[t] = d4{0., 0., 0., 0.};
+    int buf = 0;
+    const int ks0 = wave * KSW;
+    const int ks1 = min(KS, ks0 + KSW);
+    for (int j = 0; j < kFarSlots; ++j) {
+        const int src = tgt >= 0 ? slots[static_cast<int64_t>(tgt) * kFarSlots + j] : -1;
+        if (__ballot(src >= 0) == 0) continue;  // same answer in every wave
+        const int pos = slotpos[e * kFarSlots + j];
+        const int cidx = static_cast<int>(pidx[pos]);
+        const int k = rank[cidx];
+        const int kpad = (k + 15) & ~15;
+        const int kt = kpad >> 4;
+        const double* U = ops + opoff[cidx];
+        const double* V = U + static_cast<size_t>(NN) * kpad;
+        const uint16_t* pv = pvec + static_cast<size_t>(pos) * NN;
+        double* Cw = smem + static_cast<size_t>(buf * W + wave) * kpadmax * 16;
+        // GEMM1 (partial over this wave's share of the nodes)
+        d4 c1[kMaxRankTiles];
+#pragma unroll
+        for (int r = 0; r < kMaxRankTiles; ++r) c1[r] = d4{0., 0., 0., 0.};
+        for (int ks = ks0; ks < ks1; ++ks) {
+            const int nidx = 4 * ks + kq;
+            double bv = 0.;
+            int vrow = -1;
+            if (nidx < NN) {
+                vrow = static_cast<int>(pv[nidx]) * kpad;
+                if (src >= 0) bv = Mlev[static_cast<int64_t>(src) * LD + nidx];
+            }
+#pragma unroll
+            for (int r = 0; r < kMaxRankTiles; ++r) {
+                if (r < kt) {
+                    const double av = vrow >= 0 ? V[vrow + 16 * r + col] : 0.;
+                    c1[r] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, c1[r], 0, 0, 0);
+                }
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < kMaxRankTiles; ++r)
+            if (r < kt)
+#pragma unroll
+                for (int i = 0; i < 4; ++i) Cw[(16 * r + 4 * kq + i) * 16 + col] = c1[r][i];
+        __syncthreads();
+        // GEMM2 (this wave's row tiles; B = sum of the W partial C tiles)
+        const double* Cb = smem + static_cast<size_t>(buf * W) * kpadmax * 16;
+        int urow[NTW];
+#pragma unroll
+        for (int t = 0; t < NTW; ++t) {
+            const int row = 16 * (wave + W * t) + col;
+            urow[t] = row < NN ? static_cast<int>(pv[row]) * kpad : -1;
+        }
+        const int kk_end = (k + 3) >> 2;
+        for (int kk = 0; kk < kk_end; ++kk) {
+            const int r = 4 * kk + kq;
+            double bv = 0.;
+#pragma unroll
+            for (int w = 0; w < W; ++w) bv += Cb[(static_cast<size_t>(w) * kpadmax + r) * 16 + col];
+#pragma unroll
+            for (int t = 0; t < NTW; ++t) {
+                if (wave + W * t < NT) {
+                    const double av = urow[t] >= 0 ? U[urow[t] + r] : 0.;
+                    acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc[t], 0, 0, 0);
+                }
+            }
+        }
+        buf ^= 1;
+    }
+    if (tgt < 0) return;
+#pragma unroll
+    for (int t = 0; t < NTW; ++t) {
+        const int nt = wave + W * t;
+        if (nt < NT) {
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int row = 16 * nt + 4 * kq + i;
+                if (row < NN) Llev[static_cast<int64_t>(tgt) * LD + row] = scale * acc[t][i];
+            }
+        }
+    }
+}
+
+__global__ void k_mfma_probe(double* out) {
+    const int l = threadIdx.x;
+    const double a = 1.0 + (l % 16) * 4 + (l / 16);    // A[l%16][l/16]
+    const double b = 0.5 + (l / 16) * 16 + (l % 16);    // B[l/16][l%16]
+    d4 c = {0., 0., 0., 0.};
+    c = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    for (int i = 0; i < 4; ++i) out[(4 * (l / 16) + i) * 16 + (l % 16)] = c[i];
+}
+
+__global__ __launch_bounds__(256) void k_scatter(int64_t n, const int32_t* __restrict__ perm,
+                                                const double* __restrict__ rs, double* __restrict__ r) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int64_t o = perm[i];
+    r[o] = rs[i];
+    r[n + o] = rs[n + i];
+    r[2 * n + o] = rs[2 * n + i];
+    r[3 * n + o] = rs[3 * n + i];
+}
+
+inline unsigned blocks256(int64_t n) { return static_cast<unsigned>((n + 255) / 256); }
+
+Box make_box(const Context* c) {
+    Box b;
+    const double half = c->P.box_width / 2;
+    for (int d = 0; d < 3; ++d) b.corner[d] = c->P.box_center[d] + (-half);
+    b.leafw = c->P.box_width / static_cast<double>(1 << (c->height() - 1));
+    return b;
+}
+
+template <int P>
+int p2m_impl(Context* c) {
+    const int L = c->height() - 1;
+    const int64_t nl = c->ncell[L];
+    prof_begin(c, "p2m");
+    hipLaunchKernelGGL(k_p2m<P>, dim3(static_cast<unsigned>(nl)), dim3(64), 0, c->stream, static_cast<int>(nl),
+                       c->d_cellkey.as<uint64_t>() + c->lvl_off[L], c->d_leafstart.as<int32_t>(),
+                       c->d_sorted.as<double>(), c->n, make_box(c), c->d_Troots.as<double>(),
+                       c->d_M.as<double>() + static_cast<size_t>(c->lvl_off[L]) * c->LD, c->LD);
+    prof_end(c);
+    return check_launch(c, "p2m");
+}
+
+template <int P>
+int m2m_impl(Context* c, int l) {
+    const int64_t np = c->ncell[l];
+    prof_begin(c, "m2m");
+    hipLaunchKernelGGL(k_m2m<P>, dim3(static_cast<unsigned>(np)), dim3(64), 0, c->stream,
+                       c->d_childstart.as<int32_t>() + c->cs_off[l], c->d_cellkey.as<uint64_t>() + c->lvl_off[l + 1],
+                       c->d_child.as<double>(), c->d_M.as<double>() + static_cast<size_t>(c->lvl_off[l + 1]) * c->LD,
+                       c->d_M.as<double>() + static_cast<size_t>(c->lvl_off[l]) * c->LD, c->LD);
+    prof_end(c);
+    return check_launch(c, "m2m");
+}
+
+template <int P>
+int l2l_impl(Context* c, int l) {
+    const int64_t np = c->ncell[l];
+    prof_begin(c, "l2l");
+    hipLaunchKernelGGL(k_l2l<P>, dim3(static_cast<unsigned>(np)), dim3(64), 0, c->stream,
+                       c->d_childstart.as<int32_t>() + c->cs_off[l], c->d_cellkey.as<uint64_t>() + c->lvl_off[l + 1],
+                       c->d_child.as<double>(), c->d_L.as<double>() + static_cast<size_t>(c->lvl_off[l]) * c->LD,
+                       c->d_L.as<double>() + static_cast<size_t>(c->lvl_off[l + 1]) * c->LD, c->LD);
+    prof_end(c);
+    return check_launch(c, "l2l");
+}
+
+template <int P>
+int l2p_impl(Context* c) {
+    const int L = c->height() - 1;
+    const int64_t nl = c->ncell[L];
+    prof_begin(c, "l2p");
+    hipLaunchKernelGGL(k_l2p<P>, dim3(static_cast<unsigned>(nl)), dim3(64), 0, c->stream, static_cast<int>(nl),
+                       c->d_cellkey.as<uint64_t>() + c->lvl_off[L], c->d_leafstart.as<int32_t>(),
+                       c->d_sorted.as<double>(), c->n, make_box(c), c->d_Troots.as<double>(),
+                       c->d_L.as<double>() + static_cast<size_t>(c->lvl_off[L]) * c->LD, c->LD,
+                       c->d_res_sorted.as<double>());
+    prof_end(c);
+    return check_launch(c, "l2p");
+}
+
+template <int P>
+int m2l_impl(Context* c, int l) {
+    const int64_t nc = c->ncell[l];
+    const unsigned tiles = static_cast<unsigned>((nc + 15) / 16 + 8);
+    const size_t shmem = sizeof(double) * 2 * kM2LWaves * static_cast<size_t>(c->kpad_max) * 16;
+    // 2^l cells per axis: scale factor 2 / (width / 2^l) (FInterpMatrixKernel.hpp:124-133)
+    const double scale = 2. / (c->P.box_width / static_cast<double>(int64_t(1) << l));
+    prof_begin(c, "m2l");
+    hipLaunchKernelGGL(k_m2l_f64<P>, dim3(tiles), dim3(64 * kM2LWaves), shmem, c->stream,
+                       c->d_m2l.as<int32_t>() + static_cast<size_t>(c->m2l_off[l]) * kFarSlots,
+                       c->d_order.as<int32_t>() + c->lvl_off[l], c->d_pcount.as<int32_t>() + 8 * l,
+                       c->d_slotpos.as<int16_t>(), c->d_pidx.as<uint32_t>(), c->d_pvec.as<uint16_t>(),
+                       c->d_rank.as<int32_t>(), c->d_opoff.as<int64_t>(), c->d_ops.as<double>(),
+                       c->d_M.as<double>() + static_cast<size_t>(c->lvl_off[l]) * c->LD,
+                       c->d_L.as<double>() + static_cast<size_t>(c->lvl_off[l]) * c->LD, scale, c->LD, c->kpad_max);
+    prof_end(c);
+    return check_launch(c, "m2l");
+}
+
+#define HFMM_DISPATCH(P_, CALL)                                                   \
+    switch (P_) {                                                                 \
+        case 5: return CALL(5);                                                   \
+        case 6: return CALL(6);                                                   \
+        case 7: return CALL(7);                                                   \
+        case 8: return CALL(8);                                                   \
+        default: return fail(c, HFMM_ERR_ARG, "order not compiled (5..8)");      \
+    }
+
+}  // namespace
+
+int run_p2m(Context* c) {
+#define CALL(P) p2m_impl<P>(c)
+    HFMM_DISPATCH(c->order(), CALL)
+#undef CALL
+}
+int run_m2m(Context* c, int l) {
+#define CALL(P) m2m_impl<P>(c, l)
+    HFMM_DISPATCH(c->order(), CALL)
+#undef CALL
+}
+int run_l2l(Context* c, int l) {
+#define CALL(P) l2l_impl<P>(c, l)
+    HFMM_DISPATCH(c->order(), CALL)
+#undef CALL
+}
+int run_l2p(Context* c) {
+#define CALL(P) l2p_impl<P>(c)
+    HFMM_DISPATCH(c->order(), CALL)
+#undef CALL
+}
+int run_m2l(Context* c, int l) {
+    if (c->kpad_max > 16 * kMaxRankTiles) return fail(c, HFMM_ERR_ARG, "operator rank above 64 is not supported");
+#define CALL(P) m2l_impl<P>(c, l)
+    HFMM_DISPATCH(c->order(), CALL)
+#undef CALL
+}
+
+int run_p2p(Context* c) {
+    prof_begin(c, "p2p");
+    hipLaunchKernelGGL(k_p2p_f64, dim3(blocks256(c->n)), dim3(256), 0, c->stream, c->n, c->d_pleaf.as<int32_t>(),
+                       c->d_leafstart.as<int32_t>(), c->d_p2p.as<int32_t>(), c->d_sorted.as<double>(),
+                       c->d_res_sorted.as<double>());
+    prof_end(c);
+    return check_launch(c, "p2p");
+}
+
+int zero_results(Context* c) {
+    HFMM_HIP(c, hipMemsetAsync(c->d_res_sorted.ptr, 0, sizeof(double) * 4 * c->n, c->stream));
+    return HFMM_OK;
+}
+
+int scatter_results(Context* c) {
+    prof_begin(c, "scatter");
+    hipLaunchKernelGGL(k_scatter, dim3(blocks256(c->n)), dim3(256), 0, c->stream, c->n, c->d_perm.as<int32_t>(),
+                       c->d_res_sorted.as<double>(), c->d_res.as<double>());
+    prof_end(c);
+    return check_launch(c, "scatter");
+}
+
+int selftest_mfma_f64(Context* c, double* out) {
+    DevBuf b;
+    HFMM_TRY(ensure(c, b, sizeof(double) * 256));
+    hipLaunchKernelGGL(k_mfma_probe, dim3(1), dim3(64), 0, c->stream, b.as<double>());
+    int st = check_launch(c, "mfma_probe");
+    if (st == HFMM_OK) {
+        if (hipMemcpyAsync(out, b.ptr, sizeof(double) * 256, hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+            hipStreamSynchronize(c->stream) != hipSuccess)
+            st = fail(c, HFMM_ERR_DEVICE, "probe copy failed");
+    }
+    release(b);
+    return st;
+}
+
+}  // namespace hfmm

@@ -1,0 +1,523 @@
+// hfmm_api.cpp -- the extern "C" entry points of include/scalfmm_hip.h.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "hfmm_internal.hpp"
+
+namespace hfmm {
+
+int fail(Context* c, int code, const std::string& msg) {
+    if (c) c->err = msg;
+    return code;
+}
+
+int ensure(Context* c, DevBuf& b, size_t bytes) {
+    if (bytes == 0) bytes = 8;
+    if (b.bytes >= bytes) return HFMM_OK;
+    if (b.ptr) {
+        hipError_t e = hipStreamSynchronize(c->stream);
+        if (e != hipSuccess) return fail(c, HFMM_ERR_DEVICE, std::string("sync before realloc: ") + hipGetErrorString(e));
+        (void)hipFree(b.ptr);
+        b.ptr = nullptr;
+        b.bytes = 0;
+    }
+    hipError_t e = hipMalloc(&b.ptr, bytes);
+    if (e != hipSuccess) {
+        b.ptr = nullptr;
+        return fail(c, HFMM_ERR_NOMEM, std::string("hipMalloc(") + std::to_string(bytes) + "): " + hipGetErrorString(e));
+    }
+    b.bytes = bytes;
+    return HFMM_OK;
+}
+
+void release(DevBuf& b) {
+    if (b.ptr) (void)hipFree(b.ptr);
+    b.ptr = nullptr;
+    b.bytes = 0;
+}
+
+static hipEvent_t take_event(Context* c) {
+    if (!c->event_pool.empty()) {
+        hipEvent_t e = c->event_pool.back();
+        c->event_pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
+}
+
+void prof_begin(Context* c, const char* name) {
+    if (!c->profiling) return;
+    KernelTimer t;
+    t.name = name;
+    t.start = take_event(c);
+    t.stop = take_event(c);
+    if (!t.start || !t.stop) return;
+    (void)hipEventRecord(t.start, c->stream);
+    c->pending.push_back(t);
+}
+
+void prof_end(Context* c) {
+    if (!c->profiling || c->pending.empty()) return;
+    (void)hipEventRecord(c->pending.back().stop, c->stream);
+}
+
+int check_launch(Context* c, const char* name) {
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return fail(c, HFMM_ERR_DEVICE, std::string("launch ") + name + ": " + hipGetErrorString(e));
+    return HFMM_OK;
+}
+
+static int collect_times(Context* c) {
+    if (c->pending.empty()) return HFMM_OK;
+    HFMM_HIP(c, hipStreamSynchronize(c->stream));
+    for (auto& t : c->pending) {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, t.start, t.stop) == hipSuccess) {
+            auto it = std::find_if(c->totals.begin(), c->totals.end(),
+                                   [&](const hfmm_kernel_time& k) { return t.name == k.name; });
+            if (it == c->totals.end()) {
+                hfmm_kernel_time k{};
+                std::strncpy(k.name, t.name.c_str(), sizeof(k.name) - 1);
+                c->totals.push_back(k);
+                it = c->totals.end() - 1;
+            }
+            it->total_ms += ms;
+            it->launches += 1;
+        }
+        c->event_pool.push_back(t.start);
+        c->event_pool.push_back(t.stop);
+    }
+    c->pending.clear();
+    return HFMM_OK;
+}
+
+// Uploads the operator tables.  The compressed operators are packed per
+// canonical index as row-major [nn][kpad] blocks (U then V), kpad = rank
+// rounded up to 16 with zero padding, the layout the M2L kernel gathers from.
+static int upload_operators(Context* c) {
+    const ChebOperators& op = c->ops;
+    const int p = op.p, nn = op.nn;
+    c->LD = (nn + 3) & ~3;
+    HFMM_TRY(ensure(c, c->d_Troots, sizeof(double) * p * p));
+    HFMM_HIP(c, hipMemcpy(c->d_Troots.ptr, op.Troots.data(), sizeof(double) * p * p, hipMemcpyHostToDevice));
+    HFMM_TRY(ensure(c, c->d_child, sizeof(double) * op.child.size()));
+    HFMM_HIP(c, hipMemcpy(c->d_child.ptr, op.child.data(), sizeof(double) * op.child.size(), hipMemcpyHostToDevice));
+    HFMM_TRY(ensure(c, c->d_pidx, sizeof(uint32_t) * 343));
+    HFMM_HIP(c, hipMemcpy(c->d_pidx.ptr, op.pidx.data(), sizeof(uint32_t) * 343, hipMemcpyHostToDevice));
+    HFMM_TRY(ensure(c, c->d_pvec, sizeof(uint16_t) * op.pvec.size()));
+    HFMM_HIP(c, hipMemcpy(c->d_pvec.ptr, op.pvec.data(), sizeof(uint16_t) * op.pvec.size(), hipMemcpyHostToDevice));
+    std::vector<int32_t> rank(343, 0);
+    std::vector<int64_t> off(343, 0);
+    std::vector<double> packed;
+    int kpmax = 16;
+    for (int idx = 0; idx < 343; ++idx) {
+        const int k = op.rank[idx];
+        if (!k) continue;
+        const int kpad = (k + 15) & ~15;
+        kpmax = std::max(kpmax, kpad);
+        rank[idx] = k;
+        off[idx] = static_cast<int64_t>(packed.size());
+        packed.resize(packed.size() + static_cast<size_t>(2) * nn * kpad, 0.0);
+        double* U = packed.data() + off[idx];
+        double* V = U + static_cast<size_t>(nn) * kpad;
+        const std::vector<double>& K = op.K[idx];
+        for (int r = 0; r < k; ++r)
+            for (int n = 0; n < nn; ++n) {
+                U[static_cast<size_t>(n) * kpad + r] = K[static_cast<size_t>(r) * nn + n];
+                V[static_cast<size_t>(n) * kpad + r] = K[static_cast<size_t>(k) * nn + static_cast<size_t>(r) * nn + n];
+            }
+    }
+    c->kpad_max = kpmax;
+    HFMM_TRY(ensure(c, c->d_rank, sizeof(int32_t) * 343));
+    HFMM_HIP(c, hipMemcpy(c->d_rank.ptr, rank.data(), sizeof(int32_t) * 343, hipMemcpyHostToDevice));
+    HFMM_TRY(ensure(c, c->d_opoff, sizeof(int64_t) * 343));
+    HFMM_HIP(c, hipMemcpy(c->d_opoff.ptr, off.data(), sizeof(int64_t) * 343, hipMemcpyHostToDevice));
+    HFMM_TRY(ensure(c, c->d_ops, sizeof(double) * packed.size()));
+    HFMM_HIP(c, hipMemcpy(c->d_ops.ptr, packed.data(), sizeof(double) * packed.size(), hipMemcpyHostToDevice));
+    std::vector<int16_t> sp(8 * kFarSlots);
+    for (int e = 0; e < 8; ++e) {
+        FarSlot s[kFarSlots];
+        far_slots(e, s);
+        for (int j = 0; j < kFarSlots; ++j) sp[e * kFarSlots + j] = s[j].pos;
+    }
+    HFMM_TRY(ensure(c, c->d_slotpos, sizeof(int16_t) * sp.size()));
+    HFMM_HIP(c, hipMemcpy(c->d_slotpos.ptr, sp.data(), sizeof(int16_t) * sp.size(), hipMemcpyHostToDevice));
+    return HFMM_OK;
+}
+
+static void free_all(Context* c) {
+    DevBuf* bufs[] = {&c->d_Troots, &c->d_child, &c->d_pidx, &c->d_pvec, &c->d_rank, &c->d_opoff, &c->d_ops,
+                      &c->d_slotpos, &c->d_in, &c->d_keys, &c->d_keys2, &c->d_vals, &c->d_vals2, &c->d_hist,
+                      &c->d_scan_part, &c->d_flags, &c->d_scan, &c->d_sorted, &c->d_perm, &c->d_pleaf,
+                      &c->d_count, &c->d_cellkey, &c->d_cellkey_tmp, &c->d_childstart, &c->d_leafstart,
+                      &c->d_p2p, &c->d_m2l, &c->d_order, &c->d_pcount, &c->d_M, &c->d_L, &c->d_res_sorted,
+                      &c->d_res};
+    for (DevBuf* b : bufs) release(*b);
+}
+
+}  // namespace hfmm
+
+using hfmm::Context;
+
+struct hfmm_ctx {
+    Context c;
+};
+
+#define CTX_CHECK(ctx)                    \
+    do {                                  \
+        if (!(ctx)) return HFMM_ERR_ARG;  \
+    } while (0)
+
+extern "C" {
+
+const char* hfmm_version(void) { return "scalfmm_amd 0.1 (gfx950, FP64 Chebyshev 1/r)"; }
+
+int hfmm_create(const hfmm_params* params, hfmm_ctx** out) {
+    if (!params || !out) return HFMM_ERR_ARG;
+    *out = nullptr;
+    if (params->order < 5 || params->order > 8) return HFMM_ERR_ARG;
+    if (params->tree_height < 3 || params->tree_height > 20) return HFMM_ERR_ARG;
+    if (!(params->box_width > 0) || !std::isfinite(params->box_width)) return HFMM_ERR_ARG;
+    if (params->precision != HFMM_PRECISION_FP64) return HFMM_ERR_ARG;
+    hfmm_ctx* h = new (std::nothrow) hfmm_ctx();
+    if (!h) return HFMM_ERR_NOMEM;
+    Context* c = &h->c;
+    c->P = *params;
+    c->device = params->device;
+    if (hipSetDevice(c->device) != hipSuccess || hipStreamCreate(&c->stream) != hipSuccess) {
+        delete h;
+        return HFMM_ERR_DEVICE;
+    }
+    std::string err;
+    int st = hfmm::build_cheb_operators(params->order, params->epsilon, c->ops, err);
+    if (st == HFMM_OK) st = hfmm::upload_operators(c);
+    if (st != HFMM_OK) {
+        hfmm::free_all(c);
+        (void)hipStreamDestroy(c->stream);
+        delete h;
+        return st;
+    }
+    *out = h;
+    return HFMM_OK;
+}
+
+int hfmm_destroy(hfmm_ctx* h) {
+    CTX_CHECK(h);
+    Context* c = &h->c;
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    for (auto& t : c->pending) {
+        (void)hipEventDestroy(t.start);
+        (void)hipEventDestroy(t.stop);
+    }
+    for (auto e : c->event_pool) (void)hipEventDestroy(e);
+    hfmm::free_all(c);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete h;
+    return HFMM_OK;
+}
+
+const char* hfmm_last_error(const hfmm_ctx* h) { return h ? h->c.err.c_str() : "null context"; }
+
+int hfmm_set_particles(hfmm_ctx* h, int64_t n, const double* x, const double* y, const double* z, const double* q) {
+    CTX_CHECK(h);
+    Context* c = &h->c;
+    if (n <= 0 || !x || !y || !z || !q) return hfmm::fail(c, HFMM_ERR_ARG, "set_particles: bad arguments");
+    HFMM_HIP(c, hipSetDevice(c->device));
+    HFMM_TRY(hfmm::ensure(c, c->d_in, sizeof(double) * 4 * n));
+    double* d = c->d_in.as<double>();
+    HFMM_HIP(c, hipMemcpyAsync(d, x, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
+    HFMM_HIP(c, hipMemcpyAsync(d + n, y, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
+    HFMM_HIP(c, hipMemcpyAsync(d + 2 * n, z, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
+    HFMM_HIP(c, hipMemcpyAsync(d + 3 * n, q, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
+    HFMM_HIP(c, hipStreamSynchronize(c->stream));
+    c->n = n;
+    c->x = d;
+    c->y = d + n;
+    c->z = d + 2 * n;
+    c->q = d + 3 * n;
+    c->tree_built = false;
+    c->evaluated = false;
+    return HFMM_OK;
+}
+
+int hfmm_set_particles_device(hfmm_ctx* h, int64_t n, const double* x, const double* y, const double* z,
+                              const double* q) {
+    CTX_CHECK(h);
+    Context* c = &h->c;
+    if (n <= 0 || !x || !y || !z || !q) return hfmm::fail(c, HFMM_ERR_ARG, "set_particles_device: bad arguments");
+    c->n = n;
+    c->x = x;
+    c->y = y;
+    c->z = z;
+    c->q = q;
+    c->tree_built = false;
+    c->evaluated = false;
+    return HFMM_OK;
+}
+
+int hfmm_build_tree(hfmm_ctx* h) {
+    CTX_CHECK(h);
+    Context* c = &h->c;
+    if (c->n <= 0) return hfmm::fail(c, HFMM_ERR_STATE, "build_tree before set_particles");
+    HFMM_HIP(c, hipSetDevice(c->device));
+    c->tree_built = false;
+    return hfmm::build_tree_device(c);
+}
+
+int hfmm_evaluate(hfmm_ctx* h, uint32_t ops) {
+    CTX_CHECK(h);
+    Context* c = &h->c;
+    if (!c->tree_built) return hfmm::fail(c, HFMM_ERR_STATE, "evaluate before build_tree");
+    if (ops & ~HFMM_NEAR_AND_FAR) return hfmm::fail(c, HFMM_ERR_ARG, "unknown operation bits");
+    HFMM_HIP(c, hipSetDevice(c->device));
+    const int L = c->height() - 1;
+    const size_t exp_bytes = sizeof(double) * static_cast<size_t>(c->total_cells) * c->LD;
+    HFMM_TRY(hfmm::ensure(c, c->d_M, exp_bytes));
+    HFMM_TRY(hfmm::ensure(c, c->d_L, exp_bytes));
+    HFMM_TRY(hfmm::ensure(c, c->d_res_sorted, sizeof(double) * 4 * c->n));
+    HFMM_TRY(hfmm::ensure(c, c->d_res, sizeof(double) * 4 * c->n));
+    // upward pass
+    if (ops & HFMM_P2M) HFMM_TRY(hfmm::run_p2m(c));
+    else HFMM_HIP(c, hipMemsetAsync(c->d_M.ptr, 0, exp_bytes, c->stream));
+    if (ops & HFMM_M2M)
+        for (int l = L - 1; l >= 2; --l) HFMM_TRY(hfmm::run_m2m(c, l));
+    // transfer pass: M2L writes every local of levels 2..h-1
+    if (ops & HFMM_M2L)
+        for (int l = 2; l <= L; ++l) HFMM_TRY(hfmm::run_m2l(c, l));
+    else
+        HFMM_HIP(c, hipMemsetAsync(c->d_L.ptr, 0, exp_bytes, c->stream));
+    // downward pass
+    if (ops & HFMM_L2L)
+        for (int l = 2; l <= L - 1; ++l) HFMM_TRY(hfmm::run_l2l(c, l));
+    // near field initialises the results, the merge adds the far field
+    if (ops & HFMM_P2P) HFMM_TRY(hfmm::run_p2p(c));
+    else HFMM_TRY(hfmm::zero_results(c));
+    if (ops & HFMM_L2P) HFMM_TRY(hfmm::run_l2p(c));
+    HFMM_TRY(hfmm::scatter_results(c));
+    c->evaluated = true;
+    return HFMM_OK;
+}
+
+int hfmm_synchronize(hfmm_ctx* h) {
+    CTX_CHECK(h);
+    Context* c = &h->c;
+    HFMM_HIP(c, hipStreamSynchronize(c->stream));
+    return HFMM_OK;
+}
+
+int hfmm_get_results(hfmm_ctx* h, double* pot, double* fx, double* fy, double* fz) {
+    CTX_CHECK(h);
+    Context* c = &h->c;
+    if (!c->evaluated) return hfmm::fail(c, HFMM_ERR_STATE, "get_results before evaluate");
+    const double* r = c->d_res.as<double>();
+    double* outs[4] = {pot, fx, fy, fz};
+    for (int k = 0; k < 4; ++k)
+        if (outs[k])
+            HFMM_HIP(c, hipMemcpyAsync(outs[k], r + static_cast<size_t>(k) * c->n, sizeof(double) * c->n,
+                                       hipMemcpyDeviceToHost, c->stream));
+    HFMM_HIP(c, hipStreamSynchronize(c->stream));
+    return HFMM_OK;
+}
+
+int hfmm_get_results_device(hfmm_ctx* h, const double** pot, const double** fx, const double** fy, const double** fz) {
+    CTX_CHECK(h);
+    Context* c = &h->c;
+    if (!c->evaluated) return hfmm::fail(c, HFMM_ERR_STATE, "get_results before evaluate");
+    const double* r = c->d_res.as<double>();
+    if (pot) *pot = r;
+    if (fx) *fx = r + c->n;
+    if (fy) *fy = r + 2 * c->n;
+    if (fz) *fz = r + 3 * c->n;
+    return HFMM_OK;
+}
+
+int hfmm_get_tree_info(hfmm_ctx* h, hfmm_tree_info* info) {
+    CTX_CHECK(h);
+    Context* c = &h->c;
+    if (!info) return HFMM_ERR_ARG;
+    if (!c->tree_built) return hfmm::fail(c, HFMM_ERR_STATE, "tree not built");
+    std::memset(info, 0, sizeof(*info));
+    info->n_particles = c->n;
+    info->height = c->height();
+    info->order = c->order();
+    const int L = c->height() - 1;
+    for (int l = 2; l <= L; ++l) info->n_cells[l] = c->ncell[l];
+    // list sizes: count the non-empty slots on the host
+    std::vector<int32_t> s(static_cast<size_t>(c->ncell[L]) * hfmm::kNbrSlots);
+    HFMM_HIP(c, hipMemcpy(s.data(), c->d_p2p.ptr, sizeof(int32_t) * s.size(), hipMemcpyDeviceToHost));
+    info->p2p_entries = std::count_if(s.begin(), s.end(), [](int32_t v) { return v >= 0; });
+    for (int l = 2; l <= L; ++l) {
+        std::vector<int32_t> m(static_cast<size_t>(c->ncell[l]) * hfmm::kFarSlots);
+        HFMM_HIP(c, hipMemcpy(m.data(), c->d_m2l.as<int32_t>() + static_cast<size_t>(c->m2l_off[l]) * hfmm::kFarSlots,
+                              sizeof(int32_t) * m.size(), hipMemcpyDeviceToHost));
+        info->m2l_entries[l] = std::count_if(m.begin(), m.end(), [](int32_t v) { return v >= 0; });
+    }
+    return HFMM_OK;
+}
+
+int hfmm_get_sorted_keys(hfmm_ctx* h, int64_t* keys, int64_t* perm) {
+    CTX_CHECK(h);
+    Context* c = &h->c;
+    if (!c->tree_built) return hfmm::fail(c, HFMM_ERR_STATE, "tree not built");
+    HFMM_HIP(c, hipStreamSynchronize(c->stream));
+    if (keys) HFMM_HIP(c, hipMemcpy(keys, c->d_keys.ptr, sizeof(int64_t) * c->n, hipMemcpyDeviceToHost));
+    if (perm) {
+        std::vector<int32_t> p(c->n);
+        HFMM_HIP(c, hipMemcpy(p.data(), c->d_perm.ptr, sizeof(int32_t) * c->n, hipMemcpyDeviceToHost));
+        for (int64_t i = 0; i < c->n; ++i) perm[i] = p[i];
+    }
+    return HFMM_OK;
+}
+
+int hfmm_get_cell_keys(hfmm_ctx* h, int32_t level, int64_t* keys) {
+    CTX_CHECK(h);
+    Context* c = &h->c;
+    if (!c->tree_built) return hfmm::fail(c, HFMM_ERR_STATE, "tree not built");
+    if (level < 2 || level >= c->height() || !keys) return hfmm::fail(c, HFMM_ERR_ARG, "bad level");
+    HFMM_HIP(c, hipStreamSynchronize(c->stream));
+    HFMM_HIP(c, hipMemcpy(keys, c->d_cellkey.as<uint64_t>() + c->lvl_off[level], sizeof(int64_t) * c->ncell[level],
+                          hipMemcpyDeviceToHost));
+    return HFMM_OK;
+}
+
+int hfmm_get_leaf_ranges(hfmm_ctx* h, int64_t* start, int64_t* count) {
+    CTX_CHECK(h);
+    Context* c = &h->c;
+    if (!c->tree_built) return hfmm::fail(c, HFMM_ERR_STATE, "tree not built");
+    const int64_t nl = c->ncell[c->height() - 1];
+    std::vector<int32_t> s(nl + 1);
+    HFMM_HIP(c, hipStreamSynchronize(c->stream));
+    HFMM_HIP(c, hipMemcpy(s.data(), c->d_leafstart.ptr, sizeof(int32_t) * (nl + 1), hipMemcpyDeviceToHost));
+    for (int64_t i = 0; i < nl; ++i) {
+        if (start) start[i] = s[i];
+        if (count) count[i] = s[i + 1] - s[i];
+    }
+    return HFMM_OK;
+}
+
+int hfmm_get_p2p_list(hfmm_ctx* h, int64_t* offsets, int64_t* nbr, int32_t* pos) {
+    CTX_CHECK(h);
+    Context* c = &h->c;
+    if (!c->tree_built) return hfmm::fail(c, HFMM_ERR_STATE, "tree not built");
+    const int64_t nl = c->ncell[c->height() - 1];
+    std::vector<int32_t> s(static_cast<size_t>(nl) * hfmm::kNbrSlots);
+    HFMM_HIP(c, hipStreamSynchronize(c->stream));
+    HFMM_HIP(c, hipMemcpy(s.data(), c->d_p2p.ptr, sizeof(int32_t) * s.size(), hipMemcpyDeviceToHost));
+    int64_t e = 0;
+    if (offsets) offsets[0] = 0;
+    for (int64_t i = 0; i < nl; ++i) {
+        for (int j = 0; j < hfmm::kNbrSlots; ++j) {
+            const int32_t v = s[static_cast<size_t>(i) * hfmm::kNbrSlots + j];
+            if (v < 0) continue;
+            if (nbr) nbr[e] = v;
+            if (pos) pos[e] = j < 13 ? j : j + 1;  // slots skip the centre position 13
+            ++e;
+        }
+        if (offsets) offsets[i + 1] = e;
+    }
+    return HFMM_OK;
+}
+
+int hfmm_get_m2l_list(hfmm_ctx* h, int32_t level, int64_t* offsets, int64_t* src, int32_t* pos) {
+    CTX_CHECK(h);
+    Context* c = &h->c;
+    if (!c->tree_built) return hfmm::fail(c, HFMM_ERR_STATE, "tree not built");
+    if (level < 2 || level >= c->height()) return hfmm::fail(c, HFMM_ERR_ARG, "bad level");
+    const int64_t nc = c->ncell[level];
+    std::vector<int32_t> s(static_cast<size_t>(nc) * hfmm::kFarSlots);
+    std::vector<int64_t> keys(nc);
+    HFMM_HIP(c, hipStreamSynchronize(c->stream));
+    HFMM_HIP(c, hipMemcpy(s.data(), c->d_m2l.as<int32_t>() + static_cast<size_t>(c->m2l_off[level]) * hfmm::kFarSlots,
+                          sizeof(int32_t) * s.size(), hipMemcpyDeviceToHost));
+    HFMM_HIP(c, hipMemcpy(keys.data(), c->d_cellkey.as<uint64_t>() + c->lvl_off[level], sizeof(int64_t) * nc,
+                          hipMemcpyDeviceToHost));
+    hfmm::FarSlot table[8][hfmm::kFarSlots];
+    for (int e = 0; e < 8; ++e) hfmm::far_slots(e, table[e]);
+    int64_t e = 0;
+    if (offsets) offsets[0] = 0;
+    for (int64_t i = 0; i < nc; ++i) {
+        const int par = static_cast<int>(keys[i] & 7);
+        for (int j = 0; j < hfmm::kFarSlots; ++j) {
+            const int32_t v = s[static_cast<size_t>(i) * hfmm::kFarSlots + j];
+            if (v < 0) continue;
+            if (src) src[e] = v;
+            if (pos) pos[e] = table[par][j].pos;
+            ++e;
+        }
+        if (offsets) offsets[i + 1] = e;
+    }
+    return HFMM_OK;
+}
+
+int hfmm_get_operator(hfmm_ctx* h, int32_t idx, int32_t* rank, double* K) {
+    CTX_CHECK(h);
+    Context* c = &h->c;
+    if (idx < 0 || idx >= 343) return hfmm::fail(c, HFMM_ERR_ARG, "bad operator index");
+    const int k = c->ops.rank[idx];
+    if (rank) *rank = k;
+    if (K && k) std::memcpy(K, c->ops.K[idx].data(), sizeof(double) * c->ops.K[idx].size());
+    return HFMM_OK;
+}
+
+int hfmm_get_expansions(hfmm_ctx* h, int32_t level, double* M, double* L) {
+    CTX_CHECK(h);
+    Context* c = &h->c;
+    if (!c->evaluated) return hfmm::fail(c, HFMM_ERR_STATE, "get_expansions before evaluate");
+    if (level < 2 || level >= c->height()) return hfmm::fail(c, HFMM_ERR_ARG, "bad level");
+    const int64_t nc = c->ncell[level];
+    const int nn = c->ops.nn;
+    std::vector<double> tmp(static_cast<size_t>(nc) * c->LD);
+    HFMM_HIP(c, hipStreamSynchronize(c->stream));
+    for (int w = 0; w < 2; ++w) {
+        double* out = w == 0 ? M : L;
+        if (!out) continue;
+        const hfmm::DevBuf& b = w == 0 ? c->d_M : c->d_L;
+        HFMM_HIP(c, hipMemcpy(tmp.data(), b.as<double>() + static_cast<size_t>(c->lvl_off[level]) * c->LD,
+                              sizeof(double) * tmp.size(), hipMemcpyDeviceToHost));
+        for (int64_t i = 0; i < nc; ++i)
+            std::memcpy(out + static_cast<size_t>(i) * nn, tmp.data() + static_cast<size_t>(i) * c->LD,
+                        sizeof(double) * nn);
+    }
+    return HFMM_OK;
+}
+
+int hfmm_set_profiling(hfmm_ctx* h, int32_t enable) {
+    CTX_CHECK(h);
+    h->c.profiling = enable != 0;
+    return HFMM_OK;
+}
+
+int hfmm_get_kernel_times(hfmm_ctx* h, hfmm_kernel_time* out, int32_t max, int32_t* count) {
+    CTX_CHECK(h);
+    Context* c = &h->c;
+    HFMM_TRY(hfmm::collect_times(c));
+    const int32_t m = static_cast<int32_t>(c->totals.size());
+    if (count) *count = m;
+    for (int32_t i = 0; i < std::min(m, max); ++i)
+        if (out) out[i] = c->totals[i];
+    return HFMM_OK;
+}
+
+int hfmm_reset_kernel_times(hfmm_ctx* h) {
+    CTX_CHECK(h);
+    Context* c = &h->c;
+    HFMM_TRY(hfmm::collect_times(c));
+    c->totals.clear();
+    return HFMM_OK;
+}
+
+int hfmm_selftest_mfma_f64(hfmm_ctx* h, double* out) {
+    CTX_CHECK(h);
+    if (!out) return HFMM_ERR_ARG;
+    return hfmm::selftest_mfma_f64(&h->c, out);
+}
+
+}  // extern "C"

@@ -1,0 +1,138 @@
+// hfmm_internal.hpp -- context and device-buffer plumbing shared by the host
+// API (hfmm_api.cpp) and the device code (tree_device.hip, fmm_device.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/scalfmm_hip.h"
+#include "cheb_setup.hpp"
+
+namespace hfmm {
+
+constexpr int kMaxLevels = 21;
+constexpr int kNbrSlots = 26;   // P2P neighbour slots per leaf
+constexpr int kFarSlots = 189;  // M2L slots per cell
+constexpr int kTileTargets = 16;  // targets per M2L tile (the MFMA N dimension)
+
+// Grow-only device allocation.
+struct DevBuf {
+    void* ptr = nullptr;
+    size_t bytes = 0;
+    template <typename T>
+    T* as() const { return static_cast<T*>(ptr); }
+};
+
+struct KernelTimer {
+    std::string name;
+    hipEvent_t start, stop;
+};
+
+struct Context {
+    hfmm_params P{};
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+
+    // ---- operators (host + device copies)
+    ChebOperators ops;
+    int LD = 0;              // leading dimension of one expansion (nn rounded up to 4)
+    int kpad_max = 0;        // largest padded rank
+    DevBuf d_Troots;         // [p][p]
+    DevBuf d_child;          // [8][3][p][p]
+    DevBuf d_pidx;           // [343] uint32
+    DevBuf d_pvec;           // [343][nn] uint16
+    DevBuf d_rank;           // [343] int32 (by canonical index)
+    DevBuf d_opoff;          // [343] int64 offset of U_c (doubles) in d_ops
+    DevBuf d_ops;            // packed U_c, V_c row-major [nn][kpad]
+    DevBuf d_slotpos;        // [8][189] int16
+
+    // ---- particles
+    int64_t n = 0;
+    const double* x = nullptr;  // input order (device)
+    const double* y = nullptr;
+    const double* z = nullptr;
+    const double* q = nullptr;
+    DevBuf d_in;  // owned copy when set from host: 4*n doubles
+
+    // ---- tree
+    bool tree_built = false;
+    bool evaluated = false;
+    DevBuf d_keys, d_keys2, d_vals, d_vals2;  // sort buffers (uint64 / int32)
+    DevBuf d_hist;                            // radix histogram / scan scratch
+    DevBuf d_scan_part;                       // scan partials
+    DevBuf d_flags, d_scan;                   // int32 per element
+    DevBuf d_sorted;                          // sx, sy, sz, sq (4*n doubles)
+    DevBuf d_perm;                            // int32 sorted -> input index
+    DevBuf d_pleaf;                           // int32 leaf of each sorted particle
+    DevBuf d_count;                           // small device counters (int64)
+    int64_t ncell[kMaxLevels] = {0};
+    int64_t lvl_off[kMaxLevels + 1] = {0};    // offsets in the cell arrays
+    int64_t total_cells = 0;
+    DevBuf d_cellkey;                         // uint64 [total_cells]
+    DevBuf d_cellkey_tmp;                     // scratch per level
+    DevBuf d_childstart;                      // int32 [total_cells + levels] (ncell[l]+1 per level)
+    int64_t cs_off[kMaxLevels] = {0};
+    DevBuf d_leafstart;                       // int32 [nleaf+1]
+    DevBuf d_p2p;                             // int32 [nleaf][26]
+    DevBuf d_m2l;                             // int32 [cells of levels 2..h-1][189]
+    int64_t m2l_off[kMaxLevels] = {0};        // offset (in cells) of a level in d_m2l
+    DevBuf d_order;                           // int32 parity-sorted cell index per level (same offsets as d_m2l)
+    DevBuf d_pcount;                          // int32 [levels][8]
+
+    // ---- expansions and results
+    DevBuf d_M, d_L;      // [total_cells][LD]
+    DevBuf d_res_sorted;  // pot, fx, fy, fz (sorted order)
+    DevBuf d_res;         // pot, fx, fy, fz (input order)
+
+    // ---- profiling
+    bool profiling = false;
+    std::vector<KernelTimer> pending;
+    std::vector<hipEvent_t> event_pool;
+    std::vector<hfmm_kernel_time> totals;
+
+    int height() const { return P.tree_height; }
+    int order() const { return P.order; }
+};
+
+// Error helpers ---------------------------------------------------------------
+int fail(Context* c, int code, const std::string& msg);
+#define HFMM_HIP(ctx, call)                                                              \
+    do {                                                                                 \
+        hipError_t e_ = (call);                                                          \
+        if (e_ != hipSuccess)                                                            \
+            return ::hfmm::fail((ctx), HFMM_ERR_DEVICE,                                  \
+                                std::string(#call) + ": " + hipGetErrorString(e_));      \
+    } while (0)
+#define HFMM_TRY(expr)            \
+    do {                          \
+        int s_ = (expr);          \
+        if (s_ != HFMM_OK) return s_; \
+    } while (0)
+
+int ensure(Context* c, DevBuf& b, size_t bytes);
+void release(DevBuf& b);
+
+// Profiling: bracket a launch by events when enabled.
+void prof_begin(Context* c, const char* name);
+void prof_end(Context* c);
+int check_launch(Context* c, const char* name);
+
+// Tree construction (tree_device.hip)
+int build_tree_device(Context* c);
+
+// Passes (fmm_device.hip)
+int run_p2m(Context* c);
+int run_m2m(Context* c, int level);  // parents at `level`, children at level+1
+int run_m2l(Context* c, int level);
+int run_l2l(Context* c, int level);  // parents at `level`
+int run_l2p(Context* c);
+int run_p2p(Context* c);
+int zero_results(Context* c);
+int scatter_results(Context* c);
+int selftest_mfma_f64(Context* c, double* out);
+
+}  // namespace hfmm

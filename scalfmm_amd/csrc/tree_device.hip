@@ -1,0 +1,535 @@
+// tree_device.hip -- octree construction on the device (FP64 keys, int32
+// indices): leaf coordinates and Morton keys, a stable LSD radix sort,
+// run-length encoded leaves, parent levels, P2P neighbour slots and M2L
+// interaction slots in the reference's enumeration order.
+//
+// Follows (reference paths relative to its source tree):
+//   coordinates   Src/Containers/FCoordinateComputer.hpp:15-24, 42-54
+//   Morton keys   Src/Containers/FTreeCoordinate.hpp:91-142
+//   sort/leaves   Src/GroupTree/Core/FGroupTree.hpp:192-360
+//   P2P list      Src/Containers/FTreeCoordinate.hpp:189-214
+//   M2L list      Src/Containers/FTreeCoordinate.hpp:219-264
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+
+#include "hfmm_internal.hpp"
+
+namespace hfmm {
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int kItems = 8;
+constexpr int kTile = kBlock * kItems;  // elements per scan / sort tile
+
+// ---------------------------------------------------------------------------
+// Morton helpers: bit 3k <- z_k, 3k+1 <- y_k, 3k+2 <- x_k.
+__device__ __forceinline__ uint64_t spread3(uint32_t v) {
+    uint64_t x = v & 0x1fffffu;
+    x = (x | (x << 32)) & 0x1f00000000ffffull;
+    x = (x | (x << 16)) & 0x1f0000ff0000ffull;
+    x = (x | (x << 8)) & 0x100f00f00f00f00full;
+    x = (x | (x << 4)) & 0x10c30c30c30c30c3ull;
+    x = (x | (x << 2)) & 0x1249249249249249ull;
+    return x;
+}
+__device__ __forceinline__ uint32_t compact3(uint64_t x) {
+    x &= 0x1249249249249249ull;
+    x = (x ^ (x >> 2)) & 0x10c30c30c30c30c3ull;
+    x = (x ^ (x >> 4)) & 0x100f00f00f00f00full;
+    x = (x ^ (x >> 8)) & 0x1f0000ff0000ffull;
+    x = (x ^ (x >> 16)) & 0x1f00000000ffffull;
+    x = (x ^ (x >> 32)) & 0x1fffffull;
+    return static_cast<uint32_t>(x);
+}
+__device__ __forceinline__ uint64_t morton3(int x, int y, int z) {
+    return (spread3(x) << 2) | (spread3(y) << 1) | spread3(z);
+}
+__device__ __forceinline__ void demorton3(uint64_t k, int& x, int& y, int& z) {
+    z = compact3(k);
+    y = compact3(k >> 1);
+    x = compact3(k >> 2);
+}
+
+// Lower-bound search of a key in a sorted array; -1 when absent.
+__device__ __forceinline__ int find_key(const uint64_t* __restrict__ keys, int n, uint64_t key) {
+    int lo = 0, hi = n;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (keys[mid] < key) lo = mid + 1; else hi = mid;
+    }
+    return (lo < n && keys[lo] == key) ? lo : -1;
+}
+
+// ---------------------------------------------------------------------------
+// Exclusive scan (three passes: tile sums, scan of tile sums, tile scans).
+template <typename T>
+__device__ T block_exclusive_scan(T v, T* sh, T& total) {
+    const int t = threadIdx.x;
+    sh[t] = v;
+    __syncthreads();
+    for (int d = 1; d < kBlock; d <<= 1) {
+        const T add = t >= d ? sh[t - d] : T(0);
+        __syncthreads();
+        sh[t] += add;
+        __syncthreads();
+    }
+    total = sh[kBlock - 1];
+    const T incl = sh[t];
+    __syncthreads();
+    return incl - v;
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_scan_reduce(const T* __restrict__ in, int64_t n, T* __restrict__ part) {
+    __shared__ T sh[kBlock];
+    const int64_t base = static_cast<int64_t>(blockIdx.x) * kTile + threadIdx.x * kItems;
+    T s = 0;
+    for (int i = 0; i < kItems; ++i)
+        if (base + i < n) s += in[base + i];
+    T total;
+    block_exclusive_scan<T>(s, sh, total);
+    if (threadIdx.x == 0) part[blockIdx.x] = total;
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_scan_top(T* __restrict__ part, int64_t nparts, T* __restrict__ total_out) {
+    __shared__ T sh[kBlock];
+    T carry = 0;
+    for (int64_t c0 = 0; c0 < nparts; c0 += kTile) {
+        const int64_t base = c0 + threadIdx.x * kItems;
+        T v[kItems];
+        T s = 0;
+        for (int i = 0; i < kItems; ++i) {
+            v[i] = base + i < nparts ? part[base + i] : T(0);
+            s += v[i];
+        }
+        T tot;
+        T ex = block_exclusive_scan<T>(s, sh, tot) + carry;
+        for (int i = 0; i < kItems; ++i) {
+            if (base + i < nparts) part[base + i] = ex;
+            ex += v[i];
+        }
+        carry += tot;
+    }
+    if (threadIdx.x == 0 && total_out) *total_out = carry;
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_scan_apply(const T* in, T* out, int64_t n, const T* __restrict__ part) {
+    __shared__ T sh[kBlock];
+    const int64_t base = static_cast<int64_t>(blockIdx.x) * kTile + threadIdx.x * kItems;
+    T v[kItems];
+    T s = 0;
+    for (int i = 0; i < kItems; ++i) {
+        v[i] = base + i < n ? in[base + i] : T(0);
+        s += v[i];
+    }
+    T tot;
+    T ex = block_exclusive_scan<T>(s, sh, tot) + part[blockIdx.x];
+    for (int i = 0; i < kItems; ++i) {
+        if (base + i < n) out[base + i] = ex;
+        ex += v[i];
+    }
+}
+
+// out = exclusive scan of in (may alias); *total_dev = sum.
+template <typename T>
+int exclusive_scan(Context* c, const T* in, T* out, int64_t n, T* total_dev, const char* tag) {
+    if (n <= 0) {
+        HFMM_HIP(c, hipMemsetAsync(total_dev, 0, sizeof(T), c->stream));
+        return HFMM_OK;
+    }
+    const int64_t tiles = (n + kTile - 1) / kTile;
+    HFMM_TRY(ensure(c, c->d_scan_part, sizeof(T) * static_cast<size_t>(tiles + 1)));
+    T* part = c->d_scan_part.as<T>();
+    prof_begin(c, tag);
+    hipLaunchKernelGGL(k_scan_reduce<T>, dim3(static_cast<unsigned>(tiles)), dim3(kBlock), 0, c->stream, in, n, part);
+    hipLaunchKernelGGL(k_scan_top<T>, dim3(1), dim3(kBlock), 0, c->stream, part, tiles, total_dev);
+    hipLaunchKernelGGL(k_scan_apply<T>, dim3(static_cast<unsigned>(tiles)), dim3(kBlock), 0, c->stream, in, out, n,
+                       part);
+    prof_end(c);
+    return check_launch(c, tag);
+}
+
+// ---------------------------------------------------------------------------
+// Stable LSD radix sort of (uint64 key, int32 value), 8-bit digits.
+__global__ __launch_bounds__(kBlock) void k_radix_hist(const uint64_t* __restrict__ keys, int64_t n, int shift,
+                                                      int32_t* __restrict__ hist, int64_t ntiles) {
+    __shared__ int32_t s[256];
+    s[threadIdx.x] = 0;
+    __syncthreads();
+    const int64_t base = static_cast<int64_t>(blockIdx.x) * kTile;
+    for (int r = 0; r < kItems; ++r) {
+        const int64_t e = base + r * kBlock + threadIdx.x;
+        if (e < n) atomicAdd(&s[(keys[e] >> shift) & 255u], 1);
+    }
+    __syncthreads();
+    hist[static_cast<int64_t>(threadIdx.x) * ntiles + blockIdx.x] = s[threadIdx.x];
+}
+
+// Scatter one tile: elements are visited in rounds of 256 consecutive
+// elements; within a wave the rank among equal digits comes from ballots,
+// across the four waves from per-digit counters, so the order of equal keys
+// is preserved.
+__global__ __launch_bounds__(kBlock) void k_radix_scatter(const uint64_t* __restrict__ kin,
+                                                         const int32_t* __restrict__ vin, int64_t n, int shift,
+                                                         const int32_t* __restrict__ offs, int64_t ntiles,
+                                                         uint64_t* __restrict__ kout, int32_t* __restrict__ vout) {
+    __shared__ int32_t s_base[256];
+    __shared__ int32_t s_wave[4][256];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    s_base[tid] = offs[static_cast<int64_t>(tid) * ntiles + blockIdx.x];
+    const int64_t base = static_cast<int64_t>(blockIdx.x) * kTile;
+    const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    for (int r = 0; r < kItems; ++r) {
+        const int64_t e = base + r * kBlock + tid;
+        const bool valid = e < n;
+        const uint64_t key = valid ? kin[e] : 0ull;
+        const int digit = valid ? static_cast<int>((key >> shift) & 255u) : 0;
+        for (int w = 0; w < 4; ++w) s_wave[w][tid] = 0;
+        __syncthreads();
+        uint64_t peers = __ballot(valid);
+        for (int b = 0; b < 8; ++b) {
+            const bool bit = (digit >> b) & 1;
+            const uint64_t m = __ballot(bit);
+            peers &= bit ? m : ~m;
+        }
+        const int rank = __popcll(peers & lt_mask);
+        if (valid && rank == 0) s_wave[wave][digit] = __popcll(peers);
+        __syncthreads();
+        {
+            int run = s_base[tid];
+            for (int w = 0; w < 4; ++w) {
+                const int cnt = s_wave[w][tid];
+                s_wave[w][tid] = run;
+                run += cnt;
+            }
+            s_base[tid] = run;
+        }
+        __syncthreads();
+        if (valid) {
+            const int dst = s_wave[wave][digit] + rank;
+            kout[dst] = key;
+            vout[dst] = vin[e];
+        }
+        __syncthreads();
+    }
+}
+
+// Sorts (keys, vals) in place by the low `nbits` bits of the keys, using the
+// context's second key/value buffers; the result ends in keys/vals.
+int radix_sort(Context* c, uint64_t* keys, int32_t* vals, uint64_t* keys2, int32_t* vals2, int64_t n, int nbits) {
+    if (n <= 1) return HFMM_OK;
+    const int64_t ntiles = (n + kTile - 1) / kTile;
+    HFMM_TRY(ensure(c, c->d_hist, sizeof(int32_t) * static_cast<size_t>(256 * ntiles + 1)));
+    HFMM_TRY(ensure(c, c->d_count, sizeof(int64_t) * 16));
+    int32_t* hist = c->d_hist.as<int32_t>();
+    const int passes = (nbits + 7) / 8;
+    uint64_t *ka = keys, *kb = keys2;
+    int32_t *va = vals, *vb = vals2;
+    for (int p = 0; p < passes; ++p) {
+        prof_begin(c, "radix_hist");
+        hipLaunchKernelGGL(k_radix_hist, dim3(static_cast<unsigned>(ntiles)), dim3(kBlock), 0, c->stream, ka, n, 8 * p,
+                           hist, ntiles);
+        prof_end(c);
+        HFMM_TRY(check_launch(c, "radix_hist"));
+        HFMM_TRY(exclusive_scan<int32_t>(c, hist, hist, 256 * ntiles,
+                                         reinterpret_cast<int32_t*>(c->d_count.as<int64_t>() + 15), "radix_scan"));
+        prof_begin(c, "radix_scatter");
+        hipLaunchKernelGGL(k_radix_scatter, dim3(static_cast<unsigned>(ntiles)), dim3(kBlock), 0, c->stream, ka, va, n,
+                           8 * p, hist, ntiles, kb, vb);
+        prof_end(c);
+        HFMM_TRY(check_launch(c, "radix_scatter"));
+        std::swap(ka, kb);
+        std::swap(va, vb);
+    }
+    if (ka != keys) {
+        HFMM_HIP(c, hipMemcpyAsync(keys, ka, sizeof(uint64_t) * n, hipMemcpyDeviceToDevice, c->stream));
+        HFMM_HIP(c, hipMemcpyAsync(vals, va, sizeof(int32_t) * n, hipMemcpyDeviceToDevice, c->stream));
+    }
+    return HFMM_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Leaf coordinates and keys (FCoordinateComputer::GetTreeCoordinate):
+// c = rel == width ? 2^(h-1)-1 : int(rel / leafwidth), rel in [0, width].
+__global__ __launch_bounds__(kBlock) void k_morton(int64_t n, const double* __restrict__ x,
+                                                  const double* __restrict__ y, const double* __restrict__ z,
+                                                  double cx, double cy, double cz, double width, double leafw,
+                                                  int lim, uint64_t* __restrict__ keys, int32_t* __restrict__ vals,
+                                                  int32_t* __restrict__ err) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double rel[3] = {x[i] - cx, y[i] - cy, z[i] - cz};
+    int co[3];
+    bool bad = false;
+    for (int d = 0; d < 3; ++d) {
+        if (!(rel[d] >= 0 && rel[d] <= width)) { bad = true; co[d] = 0; continue; }
+        co[d] = rel[d] == width ? lim - 1 : static_cast<int>(rel[d] / leafw);
+        if (co[d] >= lim) { bad = true; co[d] = lim - 1; }
+    }
+    if (bad) atomicOr(err, 1);
+    keys[i] = morton3(co[0], co[1], co[2]);
+    vals[i] = static_cast<int32_t>(i);
+}
+
+__global__ __launch_bounds__(kBlock) void k_gather_particles(int64_t n, const int32_t* __restrict__ idx,
+                                                            const double* __restrict__ x, const double* __restrict__ y,
+                                                            const double* __restrict__ z, const double* __restrict__ q,
+                                                            double* __restrict__ out) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int32_t o = idx[i];
+    out[i] = x[o];
+    out[n + i] = y[o];
+    out[2 * n + i] = z[o];
+    out[3 * n + i] = q[o];
+}
+
+// flags[i] = 1 where (keys[i] >> shift) starts a new group.
+__global__ __launch_bounds__(kBlock) void k_head_flags(int64_t n, const uint64_t* __restrict__ keys, int shift,
+                                                      int32_t* __restrict__ flags) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    flags[i] = (i == 0 || (keys[i] >> shift) != (keys[i - 1] >> shift)) ? 1 : 0;
+}
+
+// Writes one entry per group: key >> shift and the index of its first element;
+// group_of[i] (optional) = group of element i.
+__global__ __launch_bounds__(kBlock) void k_write_groups(int64_t n, const uint64_t* __restrict__ keys, int shift,
+                                                        const int32_t* __restrict__ flags,
+                                                        const int32_t* __restrict__ pos, uint64_t* __restrict__ gkey,
+                                                        int32_t* __restrict__ gstart, int32_t* __restrict__ group_of) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int32_t g = pos[i] + flags[i] - 1;
+    if (group_of) group_of[i] = g;
+    if (flags[i]) {
+        gkey[g] = keys[i] >> shift;
+        gstart[g] = static_cast<int32_t>(i);
+    }
+}
+
+__global__ void k_set_int(int32_t* p, int32_t v) { *p = v; }
+
+// P2P neighbour slots of every leaf: 26 slots in ascending position order,
+// slot -> leaf index or -1 (outside the box or empty).
+__global__ __launch_bounds__(kBlock) void k_p2p_slots(int nleaf, const uint64_t* __restrict__ lkeys, int lim,
+                                                     int32_t* __restrict__ slots) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nleaf) return;
+    int x, y, z;
+    demorton3(lkeys[i], x, y, z);
+    int s = 0;
+    for (int dx = -1; dx <= 1; ++dx)
+        for (int dy = -1; dy <= 1; ++dy)
+            for (int dz = -1; dz <= 1; ++dz) {
+                if (!dx && !dy && !dz) continue;
+                const int nx = x + dx, ny = y + dy, nz = z + dz;
+                int f = -1;
+                if (nx >= 0 && nx < lim && ny >= 0 && ny < lim && nz >= 0 && nz < lim)
+                    f = find_key(lkeys, nleaf, morton3(nx, ny, nz));
+                slots[static_cast<int64_t>(i) * kNbrSlots + s] = f;
+                ++s;
+            }
+}
+
+// M2L slots of every cell of a level: 189 slots in the reference enumeration
+// order (parent neighbour x,y,z from -1 to 1, then cousin 0..7, keeping the
+// non-adjacent ones); slot -> cell index in the level or -1.
+__global__ __launch_bounds__(kBlock) void k_m2l_slots(int ncell, const uint64_t* __restrict__ ckeys, int plim,
+                                                     int32_t* __restrict__ slots) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ncell) return;
+    int x, y, z;
+    demorton3(ckeys[i], x, y, z);
+    const int px = x >> 1, py = y >> 1, pz = z >> 1;
+    const int bx = x & 1, by = y & 1, bz = z & 1;
+    int s = 0;
+    for (int dx = -1; dx <= 1; ++dx)
+        for (int dy = -1; dy <= 1; ++dy)
+            for (int dz = -1; dz <= 1; ++dz) {
+                if (!dx && !dy && !dz) continue;
+                const int ox = px + dx, oy = py + dy, oz = pz + dz;
+                const bool inside = ox >= 0 && ox < plim && oy >= 0 && oy < plim && oz >= 0 && oz < plim;
+                const uint64_t pk = inside ? morton3(ox, oy, oz) : 0ull;
+                for (int cz = 0; cz < 8; ++cz) {
+                    const int xd = 2 * dx + ((cz >> 2) & 1) - bx;
+                    const int yd = 2 * dy + ((cz >> 1) & 1) - by;
+                    const int zd = 2 * dz + (cz & 1) - bz;
+                    if (abs(xd) <= 1 && abs(yd) <= 1 && abs(zd) <= 1) continue;
+                    slots[static_cast<int64_t>(i) * kFarSlots + s] =
+                        inside ? find_key(ckeys, ncell, (pk << 3) | static_cast<uint64_t>(cz)) : -1;
+                    ++s;
+                }
+            }
+}
+
+__global__ __launch_bounds__(kBlock) void k_parity_keys(int ncell, const uint64_t* __restrict__ ckeys,
+                                                       uint64_t* __restrict__ pk, int32_t* __restrict__ idx,
+                                                       int32_t* __restrict__ pcount) {
+    __shared__ int32_t s[8];
+    if (threadIdx.x < 8) s[threadIdx.x] = 0;
+    __syncthreads();
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < ncell) {
+        const int par = static_cast<int>(ckeys[i] & 7u);
+        pk[i] = static_cast<uint64_t>(par);
+        idx[i] = i;
+        atomicAdd(&s[par], 1);
+    }
+    __syncthreads();
+    if (threadIdx.x < 8 && s[threadIdx.x]) atomicAdd(&pcount[threadIdx.x], s[threadIdx.x]);
+}
+
+inline unsigned blocks_for(int64_t n) { return static_cast<unsigned>((n + kBlock - 1) / kBlock); }
+
+// Run-length groups of sorted keys >> shift: writes group keys and starts,
+// returns the count on the host.
+int make_groups(Context* c, const uint64_t* keys, int64_t n, int shift, uint64_t* gkey, int32_t* gstart,
+                int32_t* group_of, int64_t& count, const char* tag) {
+    HFMM_TRY(ensure(c, c->d_flags, sizeof(int32_t) * static_cast<size_t>(n + 1)));
+    HFMM_TRY(ensure(c, c->d_scan, sizeof(int32_t) * static_cast<size_t>(n + 1)));
+    int32_t* flags = c->d_flags.as<int32_t>();
+    int32_t* pos = c->d_scan.as<int32_t>();
+    int32_t* total = reinterpret_cast<int32_t*>(c->d_count.as<int64_t>() + 14);
+    prof_begin(c, tag);
+    hipLaunchKernelGGL(k_head_flags, dim3(blocks_for(n)), dim3(kBlock), 0, c->stream, n, keys, shift, flags);
+    prof_end(c);
+    HFMM_TRY(check_launch(c, tag));
+    HFMM_TRY(exclusive_scan<int32_t>(c, flags, pos, n, total, "group_scan"));
+    prof_begin(c, tag);
+    hipLaunchKernelGGL(k_write_groups, dim3(blocks_for(n)), dim3(kBlock), 0, c->stream, n, keys, shift, flags, pos,
+                       gkey, gstart, group_of);
+    prof_end(c);
+    HFMM_TRY(check_launch(c, tag));
+    int32_t h_total = 0;
+    HFMM_HIP(c, hipMemcpyAsync(&h_total, total, sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
+    HFMM_HIP(c, hipStreamSynchronize(c->stream));
+    count = h_total;
+    return HFMM_OK;
+}
+
+}  // namespace
+
+int build_tree_device(Context* c) {
+    const int64_t n = c->n;
+    const int h = c->height();
+    const int L = h - 1;
+    if (n <= 0) return fail(c, HFMM_ERR_ARG, "no particles");
+    if (n > (int64_t(1) << 30)) return fail(c, HFMM_ERR_ARG, "more than 2^30 particles");
+    HFMM_TRY(ensure(c, c->d_keys, sizeof(uint64_t) * n));
+    HFMM_TRY(ensure(c, c->d_keys2, sizeof(uint64_t) * n));
+    HFMM_TRY(ensure(c, c->d_vals, sizeof(int32_t) * n));
+    HFMM_TRY(ensure(c, c->d_vals2, sizeof(int32_t) * n));
+    HFMM_TRY(ensure(c, c->d_count, sizeof(int64_t) * 16));
+    uint64_t* keys = c->d_keys.as<uint64_t>();
+    int32_t* vals = c->d_vals.as<int32_t>();
+    int32_t* err = reinterpret_cast<int32_t*>(c->d_count.as<int64_t>() + 13);
+    HFMM_HIP(c, hipMemsetAsync(err, 0, sizeof(int32_t), c->stream));
+
+    // 1. keys (boxCorner = center + (-(width/2)), FGroupTree.hpp:183)
+    const double half = c->P.box_width / 2;
+    const double leafw = c->P.box_width / static_cast<double>(1 << L);
+    prof_begin(c, "morton");
+    hipLaunchKernelGGL(k_morton, dim3(blocks_for(n)), dim3(kBlock), 0, c->stream, n, c->x, c->y, c->z,
+                       c->P.box_center[0] + (-half), c->P.box_center[1] + (-half), c->P.box_center[2] + (-half),
+                       c->P.box_width, leafw, 1 << L, keys, vals, err);
+    prof_end(c);
+    HFMM_TRY(check_launch(c, "morton"));
+    int32_t h_err = 0;
+    HFMM_HIP(c, hipMemcpyAsync(&h_err, err, sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
+    HFMM_HIP(c, hipStreamSynchronize(c->stream));
+    if (h_err) return fail(c, HFMM_ERR_OUTSIDE_BOX, "a particle lies outside the simulation box");
+
+    // 2. stable sort by key (ties keep the input order)
+    HFMM_TRY(radix_sort(c, keys, vals, c->d_keys2.as<uint64_t>(), c->d_vals2.as<int32_t>(), n, 3 * L));
+
+    // 3. sorted particle SoA
+    HFMM_TRY(ensure(c, c->d_sorted, sizeof(double) * 4 * n));
+    HFMM_TRY(ensure(c, c->d_perm, sizeof(int32_t) * n));
+    HFMM_HIP(c, hipMemcpyAsync(c->d_perm.ptr, vals, sizeof(int32_t) * n, hipMemcpyDeviceToDevice, c->stream));
+    prof_begin(c, "gather");
+    hipLaunchKernelGGL(k_gather_particles, dim3(blocks_for(n)), dim3(kBlock), 0, c->stream, n, vals, c->x, c->y, c->z,
+                       c->q, c->d_sorted.as<double>());
+    prof_end(c);
+    HFMM_TRY(check_launch(c, "gather"));
+
+    // 4. leaves, then the parent levels down to 2; the cell keys of every level
+    // are first produced into a per-level scratch table of capacity n.
+    const int64_t cap = n;
+    HFMM_TRY(ensure(c, c->d_cellkey_tmp, sizeof(uint64_t) * static_cast<size_t>(cap) * h));
+    HFMM_TRY(ensure(c, c->d_leafstart, sizeof(int32_t) * (n + 1)));
+    HFMM_TRY(ensure(c, c->d_pleaf, sizeof(int32_t) * n));
+    // child-start table: per level ncell+1 entries, at most cap+1 each
+    HFMM_TRY(ensure(c, c->d_childstart, sizeof(int32_t) * static_cast<size_t>(cap + 1) * h));
+    uint64_t* tkeys = c->d_cellkey_tmp.as<uint64_t>();
+    int32_t* cstart = c->d_childstart.as<int32_t>();
+    for (int l = 0; l < kMaxLevels; ++l) c->ncell[l] = 0;
+    int64_t nl = 0;
+    HFMM_TRY(make_groups(c, keys, n, 0, tkeys + static_cast<size_t>(L) * cap, c->d_leafstart.as<int32_t>(),
+                         c->d_pleaf.as<int32_t>(), nl, "leaves"));
+    c->ncell[L] = nl;
+    hipLaunchKernelGGL(k_set_int, dim3(1), dim3(1), 0, c->stream, c->d_leafstart.as<int32_t>() + nl,
+                       static_cast<int32_t>(n));
+    for (int l = L - 1; l >= 2; --l) {
+        int64_t np = 0;
+        int32_t* cs = cstart + static_cast<size_t>(l) * (cap + 1);
+        HFMM_TRY(make_groups(c, tkeys + static_cast<size_t>(l + 1) * cap, c->ncell[l + 1], 3,
+                             tkeys + static_cast<size_t>(l) * cap, cs, nullptr, np, "levels"));
+        c->ncell[l] = np;
+        hipLaunchKernelGGL(k_set_int, dim3(1), dim3(1), 0, c->stream, cs + np, static_cast<int32_t>(c->ncell[l + 1]));
+        c->cs_off[l] = static_cast<int64_t>(l) * (cap + 1);
+    }
+    // pack the cell keys of levels 2..h-1 contiguously
+    int64_t off = 0;
+    for (int l = 0; l <= L; ++l) {
+        c->lvl_off[l] = off;
+        if (l >= 2) off += c->ncell[l];
+    }
+    c->lvl_off[L + 1] = off;
+    c->total_cells = off;
+    HFMM_TRY(ensure(c, c->d_cellkey, sizeof(uint64_t) * static_cast<size_t>(off)));
+    for (int l = 2; l <= L; ++l)
+        HFMM_HIP(c, hipMemcpyAsync(c->d_cellkey.as<uint64_t>() + c->lvl_off[l], tkeys + static_cast<size_t>(l) * cap,
+                                   sizeof(uint64_t) * c->ncell[l], hipMemcpyDeviceToDevice, c->stream));
+
+    // 5. P2P slots of the leaves
+    HFMM_TRY(ensure(c, c->d_p2p, sizeof(int32_t) * static_cast<size_t>(nl) * kNbrSlots));
+    prof_begin(c, "p2p_lists");
+    hipLaunchKernelGGL(k_p2p_slots, dim3(blocks_for(nl)), dim3(kBlock), 0, c->stream, static_cast<int>(nl),
+                       c->d_cellkey.as<uint64_t>() + c->lvl_off[L], 1 << L, c->d_p2p.as<int32_t>());
+    prof_end(c);
+    HFMM_TRY(check_launch(c, "p2p_lists"));
+
+    // 6. M2L slots and parity-sorted target order of levels 2..h-1
+    HFMM_TRY(ensure(c, c->d_m2l, sizeof(int32_t) * static_cast<size_t>(off) * kFarSlots));
+    HFMM_TRY(ensure(c, c->d_order, sizeof(int32_t) * static_cast<size_t>(off)));
+    HFMM_TRY(ensure(c, c->d_pcount, sizeof(int32_t) * 8 * kMaxLevels));
+    HFMM_HIP(c, hipMemsetAsync(c->d_pcount.ptr, 0, sizeof(int32_t) * 8 * kMaxLevels, c->stream));
+    for (int l = 2; l <= L; ++l) {
+        const int64_t nc = c->ncell[l];
+        c->m2l_off[l] = c->lvl_off[l];
+        const uint64_t* ck = c->d_cellkey.as<uint64_t>() + c->lvl_off[l];
+        prof_begin(c, "m2l_lists");
+        hipLaunchKernelGGL(k_m2l_slots, dim3(blocks_for(nc)), dim3(kBlock), 0, c->stream, static_cast<int>(nc), ck,
+                           1 << (l - 1), c->d_m2l.as<int32_t>() + static_cast<size_t>(c->m2l_off[l]) * kFarSlots);
+        prof_end(c);
+        HFMM_TRY(check_launch(c, "m2l_lists"));
+        // parity order: stable sort of (key & 7) carrying the cell index
+        int32_t* ord = c->d_order.as<int32_t>() + c->lvl_off[l];
+        prof_begin(c, "parity");
+        // (the per-level key scratch is free once the keys are packed)
+        hipLaunchKernelGGL(k_parity_keys, dim3(blocks_for(nc)), dim3(kBlock), 0, c->stream, static_cast<int>(nc), ck,
+                           tkeys, ord, c->d_pcount.as<int32_t>() + 8 * l);
+        prof_end(c);
+        HFMM_TRY(check_launch(c, "parity"));
+        HFMM_TRY(radix_sort(c, tkeys, ord, tkeys + cap, c->d_vals2.as<int32_t>(), nc, 3));
+    }
+    c->tree_built = true;
+    c->evaluated = false;
+    return HFMM_OK;
+}
+
+}  // namespace hfmm

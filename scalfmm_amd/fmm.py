@@ -1,0 +1,251 @@
+"""Host-side mirror of the reference's Chebyshev 1/r path over the HIP engine.
+
+Names follow the reference (paths relative to its source tree):
+
+  FChebSymKernel(tree_height, box_width, box_center, order, epsilon)
+      Src/Kernels/Chebyshev/FChebSymKernel.hpp:110-140 (ORDER is a template
+      parameter there, an argument here; epsilon defaults to 10^-order)
+  FGroupTree(tree_height, box_width, box_center, x, y, z, q)
+      Src/GroupTree/Core/FGroupTree.hpp:178-360 (built on the device)
+  FGroupTaskAlgorithm(tree, kernel).execute(ops)
+      Src/GroupTree/Core/FGroupTaskAlgorithm.hpp:82-111 and
+      Src/Core/FCoreCommon.hpp:73-78, with the FFmmOperations bits
+
+Errors the reference turns into aborts (a particle outside the box,
+FCoordinateComputer.hpp:18) or exceptions (FChebSymM2LHandler.hpp:156-160)
+raise FmmError here.  Results come back in the input order; force = q grad phi.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _abi
+from ._abi import FAR_FIELD, L2L, L2P, M2L, M2M, NEAR_AND_FAR, NEAR_FIELD, P2M, P2P  # noqa: F401
+
+FFmmP2P, FFmmP2M, FFmmM2M, FFmmM2L, FFmmL2L, FFmmL2P = P2P, P2M, M2M, M2L, L2L, L2P
+FFmmNearField, FFmmFarField, FFmmNearAndFarFields = NEAR_FIELD, FAR_FIELD, NEAR_AND_FAR
+
+
+class FmmError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"[{code}] {msg}")
+        self.code = code
+
+
+def _dp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _ip64(a):
+    return a.ctypes.data_as(C.POINTER(C.c_int64))
+
+
+def _ip32(a):
+    return a.ctypes.data_as(C.POINTER(C.c_int32))
+
+
+class Engine:
+    """One device context: operators, particles, tree and expansions."""
+
+    def __init__(self, tree_height, box_width, box_center, order, epsilon=0.0, device=0,
+                 precision=_abi.PRECISION_FP64):
+        self._L = _abi.lib()
+        p = _abi.Params()
+        p.tree_height = int(tree_height)
+        p.order = int(order)
+        p.box_width = float(box_width)
+        for d in range(3):
+            p.box_center[d] = float(box_center[d])
+        p.epsilon = float(epsilon or 0.0)
+        p.precision = int(precision)
+        p.device = int(device)
+        h = C.c_void_p()
+        rc = self._L.hfmm_create(C.byref(p), C.byref(h))
+        if rc:
+            raise FmmError(rc, "hfmm_create failed")
+        self.h = h
+        self.params = p
+        self.n = 0
+        self.height = int(tree_height)
+        self.order = int(order)
+        self.nnodes = self.order ** 3
+
+    def close(self):
+        if getattr(self, "h", None):
+            self._L.hfmm_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc):
+        if rc:
+            raise FmmError(rc, self._L.hfmm_last_error(self.h).decode())
+
+    # ---- data ---------------------------------------------------------------
+    def set_particles(self, x, y, z, q):
+        arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (x, y, z, q)]
+        n = len(arrs[0])
+        if any(len(a) != n for a in arrs):
+            raise ValueError("x, y, z, q must have the same length")
+        self._check(self._L.hfmm_set_particles(self.h, n, *(_dp(a) for a in arrs)))
+        self.n = n
+
+    def set_particles_device(self, n, x_ptr, y_ptr, z_ptr, q_ptr):
+        self._check(self._L.hfmm_set_particles_device(self.h, int(n), x_ptr, y_ptr, z_ptr, q_ptr))
+        self.n = int(n)
+
+    def build_tree(self):
+        self._check(self._L.hfmm_build_tree(self.h))
+
+    def evaluate(self, ops=NEAR_AND_FAR):
+        self._check(self._L.hfmm_evaluate(self.h, int(ops)))
+
+    def synchronize(self):
+        self._check(self._L.hfmm_synchronize(self.h))
+
+    def results(self):
+        out = [np.empty(self.n) for _ in range(4)]
+        self._check(self._L.hfmm_get_results(self.h, *(_dp(a) for a in out)))
+        return tuple(out)
+
+    # ---- inspection -----------------------------------------------------------
+    def tree_info(self):
+        ti = _abi.TreeInfo()
+        self._check(self._L.hfmm_get_tree_info(self.h, C.byref(ti)))
+        return ti
+
+    def sorted_keys(self):
+        k = np.empty(self.n, np.int64)
+        p = np.empty(self.n, np.int64)
+        self._check(self._L.hfmm_get_sorted_keys(self.h, _ip64(k), _ip64(p)))
+        return k, p
+
+    def cell_keys(self, level):
+        ti = self.tree_info()
+        k = np.empty(max(ti.n_cells[level], 1), np.int64)
+        self._check(self._L.hfmm_get_cell_keys(self.h, level, _ip64(k)))
+        return k[: ti.n_cells[level]]
+
+    def leaf_ranges(self):
+        nl = self.tree_info().n_cells[self.height - 1]
+        s = np.empty(nl, np.int64)
+        c = np.empty(nl, np.int64)
+        self._check(self._L.hfmm_get_leaf_ranges(self.h, _ip64(s), _ip64(c)))
+        return s, c
+
+    def p2p_list(self):
+        ti = self.tree_info()
+        nl = ti.n_cells[self.height - 1]
+        off = np.empty(nl + 1, np.int64)
+        nb = np.empty(max(ti.p2p_entries, 1), np.int64)
+        ps = np.empty(max(ti.p2p_entries, 1), np.int32)
+        self._check(self._L.hfmm_get_p2p_list(self.h, _ip64(off), _ip64(nb), _ip32(ps)))
+        return off, nb[: ti.p2p_entries], ps[: ti.p2p_entries]
+
+    def m2l_list(self, level):
+        ti = self.tree_info()
+        nc = ti.n_cells[level]
+        m = ti.m2l_entries[level]
+        off = np.empty(nc + 1, np.int64)
+        sr = np.empty(max(m, 1), np.int64)
+        ps = np.empty(max(m, 1), np.int32)
+        self._check(self._L.hfmm_get_m2l_list(self.h, level, _ip64(off), _ip64(sr), _ip32(ps)))
+        return off, sr[:m], ps[:m]
+
+    def operator(self, idx):
+        r = C.c_int32()
+        self._check(self._L.hfmm_get_operator(self.h, idx, C.byref(r), None))
+        k = r.value
+        if k == 0:
+            return None, None
+        K = np.empty(2 * k * self.nnodes)
+        self._check(self._L.hfmm_get_operator(self.h, idx, C.byref(r), _dp(K)))
+        U = K[: k * self.nnodes].reshape(k, self.nnodes).T
+        V = K[k * self.nnodes:].reshape(k, self.nnodes).T
+        return U, V
+
+    def canonical_ranks(self):
+        out = {}
+        for i in range(2, 4):
+            for j in range(0, i + 1):
+                for k in range(0, j + 1):
+                    r = C.c_int32()
+                    self._check(self._L.hfmm_get_operator(self.h, (i + 3) * 49 + (j + 3) * 7 + (k + 3),
+                                                          C.byref(r), None))
+                    out[(i, j, k)] = r.value
+        return out
+
+    def expansions(self, level):
+        nc = self.tree_info().n_cells[level]
+        M = np.empty(max(nc, 1) * self.nnodes)
+        Lx = np.empty(max(nc, 1) * self.nnodes)
+        self._check(self._L.hfmm_get_expansions(self.h, level, _dp(M), _dp(Lx)))
+        return M[: nc * self.nnodes].reshape(nc, -1), Lx[: nc * self.nnodes].reshape(nc, -1)
+
+    # ---- timing ---------------------------------------------------------------
+    def set_profiling(self, on=True):
+        self._check(self._L.hfmm_set_profiling(self.h, 1 if on else 0))
+
+    def kernel_times(self):
+        cnt = C.c_int32()
+        buf = (_abi.KernelTime * 64)()
+        self._check(self._L.hfmm_get_kernel_times(self.h, buf, 64, C.byref(cnt)))
+        return {buf[i].name.decode(): (buf[i].total_ms, buf[i].launches) for i in range(min(cnt.value, 64))}
+
+    def reset_kernel_times(self):
+        self._check(self._L.hfmm_reset_kernel_times(self.h))
+
+    def selftest_mfma_f64(self):
+        out = np.empty(256)
+        self._check(self._L.hfmm_selftest_mfma_f64(self.h, _dp(out)))
+        return out.reshape(16, 16)
+
+
+class FChebSymKernel:
+    """Kernel parameters of FChebSymKernel<double, ..., FInterpMatrixKernelR, ORDER>."""
+
+    def __init__(self, tree_height, box_width, box_center, order, epsilon=None, device=0):
+        self.tree_height = int(tree_height)
+        self.box_width = float(box_width)
+        self.box_center = tuple(float(c) for c in box_center)
+        self.order = int(order)
+        self.epsilon = float(epsilon) if epsilon else 0.0
+        self.device = device
+        self.engine = Engine(tree_height, box_width, box_center, order, self.epsilon, device)
+
+
+class FGroupTree:
+    """Particles of one simulation box; the octree itself lives on the device."""
+
+    def __init__(self, tree_height, box_width, box_center, x, y, z, q):
+        self.tree_height = int(tree_height)
+        self.box_width = float(box_width)
+        self.box_center = tuple(float(c) for c in box_center)
+        self.x, self.y, self.z, self.q = (np.ascontiguousarray(a, dtype=np.float64) for a in (x, y, z, q))
+        self.potentials = self.forces_x = self.forces_y = self.forces_z = None
+
+
+class FGroupTaskAlgorithm:
+    """execute(ops) builds the tree on the device and runs the selected passes."""
+
+    def __init__(self, tree: FGroupTree, kernel: FChebSymKernel):
+        if (tree.tree_height, tree.box_width, tree.box_center) != (kernel.tree_height, kernel.box_width,
+                                                                   kernel.box_center):
+            raise ValueError("tree and kernel disagree on height / box")
+        self.tree = tree
+        self.kernel = kernel
+
+    def execute(self, operations=FFmmNearAndFarFields):
+        e = self.kernel.engine
+        t = self.tree
+        e.set_particles(t.x, t.y, t.z, t.q)
+        e.build_tree()
+        e.evaluate(operations)
+        t.potentials, t.forces_x, t.forces_y, t.forces_z = e.results()
+        return t
