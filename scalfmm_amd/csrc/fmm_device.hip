@@ -324,8 +324,9 @@ __global__ __launch_bounds__(256) void k_p2p_f64(int64_t n, const int32_t* __res
 // The permutations are applied as row gathers of the operators, the partial C
 // tiles meet in LDS (double buffered: one barrier per slot), and every target
 // belongs to exactly one tile, so L is written without atomics.
-// MFMA 16x16x4 f64 operand layout: A[i][k] in lane i + 16k, B[k][j] in lane
-// j + 16k, D[4*(l/16)+r][l%16] in register r of lane l.
+// MFMA 16x16x4 f64 operand layout (checked by hfmm_selftest_mfma_f64):
+// A[i][k] in lane i + 16k, B[k][j] in lane j + 16k, and register r of lane l
+// holds D[4r + l/16][l%16].
 constexpr int kM2LWaves = 4;
 constexpr int kMaxRankTiles = 4;
 
@@ -406,7 +407,7 @@ __global__ __launch_bounds__(256) void k_m2l_f64(const int32_t* __restrict__ slo
         for (int r = 0; r < kMaxRankTiles; ++r)
             if (r < kt)
 #pragma unroll
-                for (int i = 0; i < 4; ++i) Cw[(16 * r + 4 * kq + i) * 16 + col] = c1[r][i];
+                for (int i = 0; i < 4; ++i) Cw[(16 * r + 4 * i + kq) * 16 + col] = c1[r][i];
         __syncthreads();
         // GEMM2 (this wave's row tiles; B = sum of the W partial C tiles)
         const double* Cb = smem + static_cast<size_t>(buf * W) * kpadmax * 16;
@@ -439,7 +440,7 @@ __global__ __launch_bounds__(256) void k_m2l_f64(const int32_t* __restrict__ slo
         if (nt < NT) {
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
-                const int row = 16 * nt + 4 * kq + i;
+                const int row = 16 * nt + 4 * i + kq;
                 if (row < NN) Llev[static_cast<int64_t>(tgt) * LD + row] = scale * acc[t][i];
             }
         }
@@ -452,7 +453,7 @@ __global__ void k_mfma_probe(double* out) {
     const double b = 0.5 + (l / 16) * 16 + (l % 16);    // B[l/16][l%16]
     d4 c = {0., 0., 0., 0.};
     c = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
-    for (int i = 0; i < 4; ++i) out[(4 * (l / 16) + i) * 16 + (l % 16)] = c[i];
+    for (int i = 0; i < 4; ++i) out[(4 * i + l / 16) * 16 + (l % 16)] = c[i];
 }
 
 __global__ __launch_bounds__(256) void k_scatter(int64_t n, const int32_t* __restrict__ perm,
