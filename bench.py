@@ -96,6 +96,19 @@ def cpu_baseline(seconds_budget=30.0):
                       f"{st.t_total:.2f} s, M2L {st.t_m2l:.2f} s, P2P {st.t_p2p:.2f} s, wall {wall:.2f} s"}
 
 
+def copy_calibration(mbytes=512, reps=3):
+    """Elementwise copy of known size (read N bytes, write N bytes) for scaling
+    the FETCH_SIZE / WRITE_SIZE counters of a profiled run."""
+    import torch
+    n = mbytes * (1 << 20) // 8
+    src = torch.ones(n, dtype=torch.float64, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(reps):
+        torch.mul(src, 1.0, out=dst)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration_bytes_read": n * 8, "copy_calibration_bytes_written": n * 8, "reps": reps}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -106,7 +119,12 @@ def main():
     ap.add_argument("--order", type=int, default=7)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-accuracy", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="run only a 512 MiB elementwise copy (counter calibration)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration()
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this engine runs on one GPU; multi-GPU sharding is a design note only "
                                    "(see DESIGN.md), so --gpus must be 1"}))

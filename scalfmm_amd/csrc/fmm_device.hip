@@ -123,7 +123,7 @@ __global__ __launch_bounds__(64) void k_p2m(int nleaf, const uint64_t* __restric
 #pragma unroll
     for (int t = 0; t < PER; ++t) {
         const int idx = lane + 64 * t;
-        if (idx < NN) M[static_cast<int64_t>(leaf) * LD + idx] = acc[t];
+        if (idx < LD) M[static_cast<int64_t>(leaf) * LD + idx] = idx < NN ? acc[t] : 0.;  // zero padding
     }
 }
 
@@ -182,7 +182,7 @@ __global__ __launch_bounds__(64) void k_m2m(const int32_t* __restrict__ cstart, 
 #pragma unroll
     for (int t = 0; t < PER; ++t) {
         const int idx = lane + 64 * t;
-        if (idx < NN) Mp[static_cast<int64_t>(parent) * LD + idx] = acc[t];
+        if (idx < LD) Mp[static_cast<int64_t>(parent) * LD + idx] = idx < NN ? acc[t] : 0.;  // zero padding
     }
 }
 
@@ -233,6 +233,8 @@ __global__ __launch_bounds__(64) void k_l2l(const int32_t* __restrict__ cstart, 
 
 // ---------------------------------------------------------------------------
 // L2P: one wave per leaf, one lane per particle; adds into the sorted results.
+//   phi = sum_k Sz_k sum_j Sy_j sum_i Sx_i L[k][j][i], gradients with dS on one
+// axis; Sx, dSx, Sy, dSy stay in registers, Sz and dSz are formed per k.
 template <int P>
 __global__ __launch_bounds__(64) void k_l2p(int nleaf, const uint64_t* __restrict__ lkeys,
                                            const int32_t* __restrict__ lstart, const double* __restrict__ sp,
@@ -250,35 +252,63 @@ __global__ __launch_bounds__(64) void k_l2p(int nleaf, const uint64_t* __restric
     const int s0 = lstart[leaf], s1 = lstart[leaf + 1];
     for (int e = s0 + lane; e < s1; e += 64) {
         const double pos[3] = {sp[e], sp[n + e], sp[2 * n + e]};
-        double S[3][P], dS[3][P], jac[3];
+        double Sx[P], dSx[P], Sy[P], dSy[P];
+        const double xi0 = (2. * pos[0] - b[0] - a[0]) / (b[0] - a[0]);
+        const double xi1 = (2. * pos[1] - b[1] - a[1]) / (b[1] - a[1]);
+        const double xi2 = (2. * pos[2] - b[2] - a[2]) / (b[2] - a[2]);
+        interp_weights<P, true>(xi0, sT, Sx, dSx);
+        interp_weights<P, true>(xi1, sT, Sy, dSy);
+        // T_o(xi2) and o U_{o-1}(xi2) for the z weights
+        double Tz[P], dTz[P];
+        {
+            double t0 = 1., t1 = xi2, u0 = 1., u1 = 2. * xi2;
+            Tz[0] = 1.;
+            dTz[0] = 0.;
+            if (P > 1) { Tz[1] = xi2; dTz[1] = 1.; }
 #pragma unroll
-        for (int d = 0; d < 3; ++d) {
-            const double xi = (2. * pos[d] - b[d] - a[d]) / (b[d] - a[d]);
-            interp_weights<P, true>(xi, sT, S[d], dS[d]);
-            jac[d] = 2. / (b[d] - a[d]);
+            for (int o = 2; o < P; ++o) {
+                const double t2 = 2. * xi2 * t1 - t0;
+                t0 = t1; t1 = t2;
+                Tz[o] = t2;
+                dTz[o] = o * u1;
+                const double u2 = 2. * xi2 * u1 - u0;
+                u0 = u1; u1 = u2;
+            }
         }
         double ph = 0., gx = 0., gy = 0., gz = 0.;
+#pragma unroll 1
         for (int k = 0; k < P; ++k) {
+            double sz = 1. / P, dsz = 0.;
+#pragma unroll
+            for (int o = 1; o < P; ++o) {
+                const double t = sT[o * P + k];
+                sz += 2. / P * Tz[o] * t;
+                dsz += 2. / P * dTz[o] * t;
+            }
+            double tp = 0., tx = 0., ty = 0.;
 #pragma unroll
             for (int j = 0; j < P; ++j) {
-                double s0v = 0., s1v = 0.;
+                double va = 0., vb = 0.;
 #pragma unroll
                 for (int i = 0; i < P; ++i) {
                     const double l = sL[(k * P + j) * P + i];
-                    s0v += l * S[0][i];
-                    s1v += l * dS[0][i];
+                    va += l * Sx[i];
+                    vb += l * dSx[i];
                 }
-                ph += s0v * S[1][j] * S[2][k];
-                gx += s1v * S[1][j] * S[2][k];
-                gy += s0v * dS[1][j] * S[2][k];
-                gz += s0v * S[1][j] * dS[2][k];
+                tp += va * Sy[j];
+                tx += vb * Sy[j];
+                ty += va * dSy[j];
             }
+            ph += tp * sz;
+            gx += tx * sz;
+            gy += ty * sz;
+            gz += tp * dsz;
         }
         const double q = sp[3 * n + e];
         res[e] += ph;
-        res[n + e] += gx * jac[0] * q;
-        res[2 * n + e] += gy * jac[1] * q;
-        res[3 * n + e] += gz * jac[2] * q;
+        res[n + e] += gx * (2. / (b[0] - a[0])) * q;
+        res[2 * n + e] += gy * (2. / (b[1] - a[1])) * q;
+        res[3 * n + e] += gz * (2. / (b[2] - a[2])) * q;
     }
 }
 
@@ -316,132 +346,184 @@ __global__ __launch_bounds__(256) void k_p2p_f64(int64_t n, const int32_t* __res
 }
 
 // ---------------------------------------------------------------------------
-// M2L on FP64 MFMA (v_mfma_f64_16x16x4f64).  A workgroup of 4 waves handles a
-// tile of 16 targets of one parity at one level.  For every far-field slot j
-// (relative position pos, canonical operator c, node permutation pv):
-//   GEMM1  C (kpad x 16) = V_c[pv[n]][r]^T * M_src[n]    K = n split over the 4 waves
-//   GEMM2  acc (nn x 16) += U_c[pv[n]][r] * C            rows split over the 4 waves
-// The permutations are applied as row gathers of the operators, the partial C
-// tiles meet in LDS (double buffered: one barrier per slot), and every target
-// belongs to exactly one tile, so L is written without atomics.
+// M2L on FP64 MFMA (v_mfma_f64_16x16x4f64), all levels in one launch.
+//
+// A workgroup of 4 waves owns a tile of 32 targets (two 16-column MFMA tiles)
+// of one parity at one level, and walks the 189 far-field slots of that
+// parity.  For slot j (relative position pos, canonical operator c with rank
+// k, node permutation pv):
+//   GEMM1  C (kpad x 32) = V_c[pv[n]][r]^T * M_src[n]   work units (rank tile,
+//          column tile) spread over the waves; B read as 32-byte vectors
+//   GEMM2  acc (nn x 32) += U_c[pv[n]][r] * C          row tiles spread over
+//          the waves; A read as 32-byte vectors along r
+// The permutations are row gathers of the L2-resident operators; C passes
+// through LDS (double buffered, one barrier per slot); every target belongs to
+// exactly one tile, so L is written without atomics.  Node indices are padded
+// to LD (multiple of 16): padded permutation entries point at a zero operator
+// row and the multipole padding is zero, so no lane needs a bounds mask.
+// Workgroups are remapped so that each XCD walks a contiguous range of tiles
+// (neighbouring targets share sources through that XCD's L2).
 // MFMA 16x16x4 f64 operand layout (checked by hfmm_selftest_mfma_f64):
 // A[i][k] in lane i + 16k, B[k][j] in lane j + 16k, and register r of lane l
 // holds D[4r + l/16][l%16].
 constexpr int kM2LWaves = 4;
-constexpr int kMaxRankTiles = 4;
+constexpr int kM2LCols = 2;       // 16-wide column tiles per workgroup
+constexpr int kM2LTile = 16 * kM2LCols;
+constexpr int kMaxRankTiles = 4;  // kpad <= 64
+
+struct M2LLevels {
+    int nlev;
+    int tile_begin[kMaxLevels + 1];  // prefix of (upper bounds of) tiles per level
+    int64_t cell_off[kMaxLevels];    // first cell of the level in the cell arrays
+    int pcount_off[kMaxLevels];      // parity counts of the level
+    double scale[kMaxLevels];
+};
 
 template <int P>
-__global__ __launch_bounds__(256) void k_m2l_f64(const int32_t* __restrict__ slots, const int32_t* __restrict__ order,
-                                                const int32_t* __restrict__ pcount, const int16_t* __restrict__ slotpos,
-                                                const uint32_t* __restrict__ pidx, const uint16_t* __restrict__ pvec,
-                                                const int32_t* __restrict__ rank, const int64_t* __restrict__ opoff,
-                                                const double* __restrict__ ops, const double* __restrict__ Mlev,
-                                                double* __restrict__ Llev, double scale, int LD, int kpadmax) {
+__global__ __launch_bounds__(256) void k_m2l_f64(M2LLevels lv, const int32_t* __restrict__ slots,
+                                                const int32_t* __restrict__ order, const int32_t* __restrict__ pcount,
+                                                const int16_t* __restrict__ slotpos, const uint32_t* __restrict__ pidx,
+                                                const uint16_t* __restrict__ pvec, const int32_t* __restrict__ rank,
+                                                const int64_t* __restrict__ opoff, const double* __restrict__ ops,
+                                                const double* __restrict__ M, double* __restrict__ L, int LD,
+                                                int kpadmax) {
     constexpr int NN = P * P * P;
     constexpr int W = kM2LWaves;
     constexpr int NT = (NN + 15) / 16;
     constexpr int NTW = (NT + W - 1) / W;
-    constexpr int KS = (NN + 3) / 4;
-    constexpr int KSW = (KS + W - 1) / W;
-    extern __shared__ double smem[];  // [2][W][kpadmax][16]
+    extern __shared__ double smem[];  // [2][kpadmax][kM2LTile]
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int col = lane & 15, kq = lane >> 4;
+    const int KB = LD / 16;  // 16-node blocks
 
+    // XCD-aware tile order: workgroup i runs on XCD i % 8
+    const int nwg = gridDim.x;
+    const int per_xcd = (nwg + 7) / 8;
+    const int xcd = blockIdx.x % 8;
+    const int tile = xcd * per_xcd + blockIdx.x / 8;
+    if (tile >= lv.tile_begin[lv.nlev]) return;
+    int li = 0;
+    while (tile >= lv.tile_begin[li + 1]) ++li;
+    const int ltile = tile - lv.tile_begin[li];
+    const int32_t* pc = pcount + lv.pcount_off[li];
     int e = -1, first = 0, cnt = 0;
     {
         int tacc = 0, cacc = 0;
         for (int q = 0; q < 8; ++q) {
-            const int c = pcount[q];
-            const int t = (c + 15) / 16;
-            if (e < 0 && static_cast<int>(blockIdx.x) < tacc + t) {
+            const int c = pc[q];
+            const int t = (c + kM2LTile - 1) / kM2LTile;
+            if (e < 0 && ltile < tacc + t) {
                 e = q;
-                first = cacc + (static_cast<int>(blockIdx.x) - tacc) * 16;
-                cnt = min(16, c - (static_cast<int>(blockIdx.x) - tacc) * 16);
+                first = cacc + (ltile - tacc) * kM2LTile;
+                cnt = min(kM2LTile, c - (ltile - tacc) * kM2LTile);
             }
             tacc += t;
             cacc += c;
         }
     }
     if (e < 0) return;  // uniform over the workgroup
-    const int tgt = col < cnt ? order[first + col] : -1;
-
-    d4 acc[NTW];
+    const int64_t coff = lv.cell_off[li];
+    const int32_t* lslots = slots + coff * kFarSlots;
+    const double* Ml = M + coff * LD;
+    double* Ll = L + coff * LD;
+    int tgt[kM2LCols];
 #pragma unroll
-    for (int t = 0; t < NTW; ++t) acc[t] = d4{0., 0., 0., 0.};
+    for (int ct = 0; ct < kM2LCols; ++ct) {
+        const int cc = 16 * ct + col;
+        tgt[ct] = cc < cnt ? order[coff + first + cc] : -1;
+    }
+
+    d4 acc[NTW][kM2LCols];
+#pragma unroll
+    for (int t = 0; t < NTW; ++t)
+#pragma unroll
+        for (int ct = 0; ct < kM2LCols; ++ct) acc[t][ct] = d4{0., 0., 0., 0.};
     int buf = 0;
-    const int ks0 = wave * KSW;
-    const int ks1 = min(KS, ks0 + KSW);
     for (int j = 0; j < kFarSlots; ++j) {
-        const int src = tgt >= 0 ? slots[static_cast<int64_t>(tgt) * kFarSlots + j] : -1;
-        if (__ballot(src >= 0) == 0) continue;  // same answer in every wave
+        int src[kM2LCols];
+        bool any = false;
+#pragma unroll
+        for (int ct = 0; ct < kM2LCols; ++ct) {
+            src[ct] = tgt[ct] >= 0 ? lslots[static_cast<int64_t>(tgt[ct]) * kFarSlots + j] : -1;
+            any |= src[ct] >= 0;
+        }
+        if (__ballot(any) == 0) continue;  // same answer in every wave
         const int pos = slotpos[e * kFarSlots + j];
         const int cidx = static_cast<int>(pidx[pos]);
         const int k = rank[cidx];
         const int kpad = (k + 15) & ~15;
         const int kt = kpad >> 4;
         const double* U = ops + opoff[cidx];
-        const double* V = U + static_cast<size_t>(NN) * kpad;
-        const uint16_t* pv = pvec + static_cast<size_t>(pos) * NN;
-        double* Cw = smem + static_cast<size_t>(buf * W + wave) * kpadmax * 16;
-        // GEMM1 (partial over this wave's share of the nodes)
-        d4 c1[kMaxRankTiles];
-#pragma unroll
-        for (int r = 0; r < kMaxRankTiles; ++r) c1[r] = d4{0., 0., 0., 0.};
-        for (int ks = ks0; ks < ks1; ++ks) {
-            const int nidx = 4 * ks + kq;
-            double bv = 0.;
-            int vrow = -1;
-            if (nidx < NN) {
-                vrow = static_cast<int>(pv[nidx]) * kpad;
-                if (src >= 0) bv = Mlev[static_cast<int64_t>(src) * LD + nidx];
+        const double* V = U + static_cast<size_t>(NN + 1) * kpad;
+        const uint16_t* pv = pvec + static_cast<size_t>(pos) * LD;
+        double* Cb = smem + static_cast<size_t>(buf) * kpadmax * kM2LTile;
+        // GEMM1: units (rank tile rt, column tile ct)
+        for (int u = wave; u < kt * kM2LCols; u += W) {
+            const int rt = u / kM2LCols, ct = u % kM2LCols;
+            const int s = src[ct];
+            const double* Ms = Ml + static_cast<int64_t>(s < 0 ? 0 : s) * LD;
+            d4 c1 = d4{0., 0., 0., 0.};
+            for (int kb = 0; kb < KB; ++kb) {
+                const int n0 = 16 * kb + 4 * kq;
+                d4 bv = d4{0., 0., 0., 0.};
+                if (s >= 0) bv = *reinterpret_cast<const d4*>(Ms + n0);
+                const ushort4 rows = *reinterpret_cast<const ushort4*>(pv + n0);
+                const double a0 = V[static_cast<int>(rows.x) * kpad + 16 * rt + col];
+                const double a1 = V[static_cast<int>(rows.y) * kpad + 16 * rt + col];
+                const double a2 = V[static_cast<int>(rows.z) * kpad + 16 * rt + col];
+                const double a3 = V[static_cast<int>(rows.w) * kpad + 16 * rt + col];
+                c1 = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, bv[0], c1, 0, 0, 0);
+                c1 = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, bv[1], c1, 0, 0, 0);
+                c1 = __builtin_amdgcn_mfma_f64_16x16x4f64(a2, bv[2], c1, 0, 0, 0);
+                c1 = __builtin_amdgcn_mfma_f64_16x16x4f64(a3, bv[3], c1, 0, 0, 0);
             }
+            // MFMA step s used node n0 + s in lane group kq: the K order is a
+            // permutation of the nodes shared by A and B, so C is unchanged.
 #pragma unroll
-            for (int r = 0; r < kMaxRankTiles; ++r) {
-                if (r < kt) {
-                    const double av = vrow >= 0 ? V[vrow + 16 * r + col] : 0.;
-                    c1[r] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, c1[r], 0, 0, 0);
-                }
-            }
+            for (int i = 0; i < 4; ++i) Cb[(16 * rt + 4 * i + kq) * kM2LTile + 16 * ct + col] = c1[i];
         }
-#pragma unroll
-        for (int r = 0; r < kMaxRankTiles; ++r)
-            if (r < kt)
-#pragma unroll
-                for (int i = 0; i < 4; ++i) Cw[(16 * r + 4 * i + kq) * 16 + col] = c1[r][i];
         __syncthreads();
-        // GEMM2 (this wave's row tiles; B = sum of the W partial C tiles)
-        const double* Cb = smem + static_cast<size_t>(buf * W) * kpadmax * 16;
+        // GEMM2: this wave's row tiles, both column tiles
         int urow[NTW];
 #pragma unroll
         for (int t = 0; t < NTW; ++t) {
-            const int row = 16 * (wave + W * t) + col;
-            urow[t] = row < NN ? static_cast<int>(pv[row]) * kpad : -1;
+            const int nt = wave + W * t;
+            urow[t] = nt < NT ? static_cast<int>(pv[16 * nt + col]) * kpad : 0;
         }
-        const int kk_end = (k + 3) >> 2;
-        for (int kk = 0; kk < kk_end; ++kk) {
-            const int r = 4 * kk + kq;
-            double bv = 0.;
+        for (int kb = 0; kb < kt; ++kb) {
+            const int r0 = 16 * kb + 4 * kq;
+            double b[4][kM2LCols];
 #pragma unroll
-            for (int w = 0; w < W; ++w) bv += Cb[(static_cast<size_t>(w) * kpadmax + r) * 16 + col];
+            for (int s2 = 0; s2 < 4; ++s2)
+#pragma unroll
+                for (int ct = 0; ct < kM2LCols; ++ct) b[s2][ct] = Cb[(r0 + s2) * kM2LTile + 16 * ct + col];
 #pragma unroll
             for (int t = 0; t < NTW; ++t) {
                 if (wave + W * t < NT) {
-                    const double av = urow[t] >= 0 ? U[urow[t] + r] : 0.;
-                    acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc[t], 0, 0, 0);
+                    const d4 av = *reinterpret_cast<const d4*>(U + urow[t] + r0);
+#pragma unroll
+                    for (int s2 = 0; s2 < 4; ++s2)
+#pragma unroll
+                        for (int ct = 0; ct < kM2LCols; ++ct)
+                            acc[t][ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[s2], b[s2][ct], acc[t][ct], 0, 0, 0);
                 }
             }
         }
         buf ^= 1;
     }
-    if (tgt < 0) return;
+    const double scale = lv.scale[li];
 #pragma unroll
-    for (int t = 0; t < NTW; ++t) {
-        const int nt = wave + W * t;
-        if (nt < NT) {
+    for (int ct = 0; ct < kM2LCols; ++ct) {
+        if (tgt[ct] < 0) continue;
 #pragma unroll
-            for (int i = 0; i < 4; ++i) {
-                const int row = 16 * nt + 4 * i + kq;
-                if (row < NN) Llev[static_cast<int64_t>(tgt) * LD + row] = scale * acc[t][i];
+        for (int t = 0; t < NTW; ++t) {
+            const int nt = wave + W * t;
+            if (nt < NT) {
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    const int row = 16 * nt + 4 * i + kq;
+                    if (row < NN) Ll[static_cast<int64_t>(tgt[ct]) * LD + row] = scale * acc[t][ct][i];
+                }
             }
         }
     }
@@ -529,20 +611,29 @@ int l2p_impl(Context* c) {
 }
 
 template <int P>
-int m2l_impl(Context* c, int l) {
-    const int64_t nc = c->ncell[l];
-    const unsigned tiles = static_cast<unsigned>((nc + 15) / 16 + 8);
-    const size_t shmem = sizeof(double) * 2 * kM2LWaves * static_cast<size_t>(c->kpad_max) * 16;
-    // 2^l cells per axis: scale factor 2 / (width / 2^l) (FInterpMatrixKernel.hpp:124-133)
-    const double scale = 2. / (c->P.box_width / static_cast<double>(int64_t(1) << l));
+int m2l_impl(Context* c) {
+    const int L = c->height() - 1;
+    M2LLevels lv{};
+    lv.nlev = 0;
+    int tiles = 0;
+    lv.tile_begin[0] = 0;
+    for (int l = 2; l <= L; ++l) {
+        const int i = lv.nlev++;
+        lv.cell_off[i] = c->lvl_off[l];
+        lv.pcount_off[i] = 8 * l;
+        // 2^l cells per axis: scale factor 2 / (width / 2^l) (FInterpMatrixKernel.hpp:124-133)
+        lv.scale[i] = 2. / (c->P.box_width / static_cast<double>(int64_t(1) << l));
+        tiles += static_cast<int>((c->ncell[l] + kM2LTile - 1) / kM2LTile + 8);
+        lv.tile_begin[i + 1] = tiles;
+    }
+    const unsigned grid = static_cast<unsigned>((tiles + 7) / 8 * 8);
+    const size_t shmem = sizeof(double) * 2 * static_cast<size_t>(c->kpad_max) * kM2LTile;
     prof_begin(c, "m2l");
-    hipLaunchKernelGGL(k_m2l_f64<P>, dim3(tiles), dim3(64 * kM2LWaves), shmem, c->stream,
-                       c->d_m2l.as<int32_t>() + static_cast<size_t>(c->m2l_off[l]) * kFarSlots,
-                       c->d_order.as<int32_t>() + c->lvl_off[l], c->d_pcount.as<int32_t>() + 8 * l,
+    hipLaunchKernelGGL(k_m2l_f64<P>, dim3(grid), dim3(64 * kM2LWaves), shmem, c->stream, lv,
+                       c->d_m2l.as<int32_t>(), c->d_order.as<int32_t>(), c->d_pcount.as<int32_t>(),
                        c->d_slotpos.as<int16_t>(), c->d_pidx.as<uint32_t>(), c->d_pvec.as<uint16_t>(),
-                       c->d_rank.as<int32_t>(), c->d_opoff.as<int64_t>(), c->d_ops.as<double>(),
-                       c->d_M.as<double>() + static_cast<size_t>(c->lvl_off[l]) * c->LD,
-                       c->d_L.as<double>() + static_cast<size_t>(c->lvl_off[l]) * c->LD, scale, c->LD, c->kpad_max);
+                       c->d_rank.as<int32_t>(), c->d_opoff.as<int64_t>(), c->d_ops.as<double>(), c->d_M.as<double>(),
+                       c->d_L.as<double>(), c->LD, c->kpad_max);
     prof_end(c);
     return check_launch(c, "m2l");
 }
@@ -578,9 +669,9 @@ int run_l2p(Context* c) {
     HFMM_DISPATCH(c->order(), CALL)
 #undef CALL
 }
-int run_m2l(Context* c, int l) {
+int run_m2l(Context* c) {
     if (c->kpad_max > 16 * kMaxRankTiles) return fail(c, HFMM_ERR_ARG, "operator rank above 64 is not supported");
-#define CALL(P) m2l_impl<P>(c, l)
+#define CALL(P) m2l_impl<P>(c)
     HFMM_DISPATCH(c->order(), CALL)
 #undef CALL
 }

@@ -99,20 +99,25 @@ static int collect_times(Context* c) {
 }
 
 // Uploads the operator tables.  The compressed operators are packed per
-// canonical index as row-major [nn][kpad] blocks (U then V), kpad = rank
-// rounded up to 16 with zero padding, the layout the M2L kernel gathers from.
+// canonical index as row-major [nn+1][kpad] blocks (U then V): kpad = rank
+// rounded up to 16 with zero columns, and a zero row nn that the padded node
+// permutations point at, so the M2L kernel needs no bounds masks.
 static int upload_operators(Context* c) {
     const ChebOperators& op = c->ops;
     const int p = op.p, nn = op.nn;
-    c->LD = (nn + 3) & ~3;
+    c->LD = (nn + 15) & ~15;
     HFMM_TRY(ensure(c, c->d_Troots, sizeof(double) * p * p));
     HFMM_HIP(c, hipMemcpy(c->d_Troots.ptr, op.Troots.data(), sizeof(double) * p * p, hipMemcpyHostToDevice));
     HFMM_TRY(ensure(c, c->d_child, sizeof(double) * op.child.size()));
     HFMM_HIP(c, hipMemcpy(c->d_child.ptr, op.child.data(), sizeof(double) * op.child.size(), hipMemcpyHostToDevice));
     HFMM_TRY(ensure(c, c->d_pidx, sizeof(uint32_t) * 343));
     HFMM_HIP(c, hipMemcpy(c->d_pidx.ptr, op.pidx.data(), sizeof(uint32_t) * 343, hipMemcpyHostToDevice));
-    HFMM_TRY(ensure(c, c->d_pvec, sizeof(uint16_t) * op.pvec.size()));
-    HFMM_HIP(c, hipMemcpy(c->d_pvec.ptr, op.pvec.data(), sizeof(uint16_t) * op.pvec.size(), hipMemcpyHostToDevice));
+    // node permutations padded to LD entries per position (padding -> zero row nn)
+    std::vector<uint16_t> pvp(static_cast<size_t>(343) * c->LD, static_cast<uint16_t>(nn));
+    for (int pos = 0; pos < 343; ++pos)
+        for (int n = 0; n < nn; ++n) pvp[static_cast<size_t>(pos) * c->LD + n] = op.pvec[static_cast<size_t>(pos) * nn + n];
+    HFMM_TRY(ensure(c, c->d_pvec, sizeof(uint16_t) * pvp.size()));
+    HFMM_HIP(c, hipMemcpy(c->d_pvec.ptr, pvp.data(), sizeof(uint16_t) * pvp.size(), hipMemcpyHostToDevice));
     std::vector<int32_t> rank(343, 0);
     std::vector<int64_t> off(343, 0);
     std::vector<double> packed;
@@ -124,9 +129,9 @@ static int upload_operators(Context* c) {
         kpmax = std::max(kpmax, kpad);
         rank[idx] = k;
         off[idx] = static_cast<int64_t>(packed.size());
-        packed.resize(packed.size() + static_cast<size_t>(2) * nn * kpad, 0.0);
+        packed.resize(packed.size() + static_cast<size_t>(2) * (nn + 1) * kpad, 0.0);
         double* U = packed.data() + off[idx];
-        double* V = U + static_cast<size_t>(nn) * kpad;
+        double* V = U + static_cast<size_t>(nn + 1) * kpad;
         const std::vector<double>& K = op.K[idx];
         for (int r = 0; r < k; ++r)
             for (int n = 0; n < nn; ++n) {
@@ -291,7 +296,7 @@ int hfmm_evaluate(hfmm_ctx* h, uint32_t ops) {
         for (int l = L - 1; l >= 2; --l) HFMM_TRY(hfmm::run_m2m(c, l));
     // transfer pass: M2L writes every local of levels 2..h-1
     if (ops & HFMM_M2L)
-        for (int l = 2; l <= L; ++l) HFMM_TRY(hfmm::run_m2l(c, l));
+        HFMM_TRY(hfmm::run_m2l(c));
     else
         HFMM_HIP(c, hipMemsetAsync(c->d_L.ptr, 0, exp_bytes, c->stream));
     // downward pass

@@ -127,7 +127,7 @@ int build_tree_device(Context* c);
 // Passes (fmm_device.hip)
 int run_p2m(Context* c);
 int run_m2m(Context* c, int level);  // parents at `level`, children at level+1
-int run_m2l(Context* c, int level);
+int run_m2l(Context* c);  // every level 2..h-1 in one launch
 int run_l2l(Context* c, int level);  // parents at `level`
 int run_l2p(Context* c);
 int run_p2p(Context* c);
