@@ -391,16 +391,23 @@ __global__ __launch_bounds__(256) void k_m2l_f64(M2LLevels lv, const int32_t* __
     constexpr int W = kM2LWaves;
     constexpr int NT = (NN + 15) / 16;
     constexpr int NTW = (NT + W - 1) / W;
-    extern __shared__ double smem[];  // [2][kpadmax][kM2LTile]
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    constexpr int TC = kM2LTile;
+    // LDS: per-slot tables, two node-permutation buffers, two partial C tiles
+    __shared__ int16_t s_pos[kFarSlots];
+    __shared__ int32_t s_rank[kFarSlots];
+    __shared__ int64_t s_off[kFarSlots];
+    extern __shared__ double smem[];  // C[2][kpadmax][TC] then pv[2][LD] (uint16)
+    double* Cp = smem;
+    uint16_t* pvs = reinterpret_cast<uint16_t*>(smem + 2 * static_cast<size_t>(kpadmax) * TC);
+    const int tid = threadIdx.x;
+    const int lane = tid & 63, wave = tid >> 6;
     const int col = lane & 15, kq = lane >> 4;
     const int KB = LD / 16;  // 16-node blocks
 
     // XCD-aware tile order: workgroup i runs on XCD i % 8
     const int nwg = gridDim.x;
     const int per_xcd = (nwg + 7) / 8;
-    const int xcd = blockIdx.x % 8;
-    const int tile = xcd * per_xcd + blockIdx.x / 8;
+    const int tile = (blockIdx.x % 8) * per_xcd + blockIdx.x / 8;
     if (tile >= lv.tile_begin[lv.nlev]) return;
     int li = 0;
     while (tile >= lv.tile_begin[li + 1]) ++li;
@@ -411,92 +418,130 @@ __global__ __launch_bounds__(256) void k_m2l_f64(M2LLevels lv, const int32_t* __
         int tacc = 0, cacc = 0;
         for (int q = 0; q < 8; ++q) {
             const int c = pc[q];
-            const int t = (c + kM2LTile - 1) / kM2LTile;
+            const int t = (c + TC - 1) / TC;
             if (e < 0 && ltile < tacc + t) {
                 e = q;
-                first = cacc + (ltile - tacc) * kM2LTile;
-                cnt = min(kM2LTile, c - (ltile - tacc) * kM2LTile);
+                first = cacc + (ltile - tacc) * TC;
+                cnt = min(TC, c - (ltile - tacc) * TC);
             }
             tacc += t;
             cacc += c;
         }
     }
     if (e < 0) return;  // uniform over the workgroup
+    for (int j = tid; j < kFarSlots; j += 64 * W) {
+        const int pos = slotpos[e * kFarSlots + j];
+        const int c = static_cast<int>(pidx[pos]);
+        s_pos[j] = static_cast<int16_t>(pos);
+        s_rank[j] = rank[c];
+        s_off[j] = opoff[c];
+    }
     const int64_t coff = lv.cell_off[li];
     const int32_t* lslots = slots + coff * kFarSlots;
     const double* Ml = M + coff * LD;
     double* Ll = L + coff * LD;
     int tgt[kM2LCols];
+    int nsrc[kM2LCols];
 #pragma unroll
     for (int ct = 0; ct < kM2LCols; ++ct) {
         const int cc = 16 * ct + col;
         tgt[ct] = cc < cnt ? order[coff + first + cc] : -1;
+        nsrc[ct] = tgt[ct] >= 0 ? lslots[static_cast<int64_t>(tgt[ct]) * kFarSlots] : -1;
     }
+    __syncthreads();
 
     d4 acc[NTW][kM2LCols];
 #pragma unroll
     for (int t = 0; t < NTW; ++t)
 #pragma unroll
         for (int ct = 0; ct < kM2LCols; ++ct) acc[t][ct] = d4{0., 0., 0., 0.};
-    int buf = 0;
+    int nproc = 0;
     for (int j = 0; j < kFarSlots; ++j) {
         int src[kM2LCols];
         bool any = false;
 #pragma unroll
         for (int ct = 0; ct < kM2LCols; ++ct) {
-            src[ct] = tgt[ct] >= 0 ? lslots[static_cast<int64_t>(tgt[ct]) * kFarSlots + j] : -1;
+            src[ct] = nsrc[ct];
             any |= src[ct] >= 0;
+            // prefetch the next slot's sources
+            nsrc[ct] = (tgt[ct] >= 0 && j + 1 < kFarSlots) ? lslots[static_cast<int64_t>(tgt[ct]) * kFarSlots + j + 1]
+                                                            : -1;
         }
         if (__ballot(any) == 0) continue;  // same answer in every wave
-        const int pos = slotpos[e * kFarSlots + j];
-        const int cidx = static_cast<int>(pidx[pos]);
-        const int k = rank[cidx];
+        const int pos = s_pos[j];
+        const int k = s_rank[j];
         const int kpad = (k + 15) & ~15;
         const int kt = kpad >> 4;
-        const double* U = ops + opoff[cidx];
+        const double* U = ops + s_off[j];
         const double* V = U + static_cast<size_t>(NN + 1) * kpad;
-        const uint16_t* pv = pvec + static_cast<size_t>(pos) * LD;
-        double* Cb = smem + static_cast<size_t>(buf) * kpadmax * kM2LTile;
-        // GEMM1: units (rank tile rt, column tile ct)
-        for (int u = wave; u < kt * kM2LCols; u += W) {
-            const int rt = u / kM2LCols, ct = u % kM2LCols;
-            const int s = src[ct];
-            const double* Ms = Ml + static_cast<int64_t>(s < 0 ? 0 : s) * LD;
-            d4 c1 = d4{0., 0., 0., 0.};
-            for (int kb = 0; kb < KB; ++kb) {
-                const int n0 = 16 * kb + 4 * kq;
-                d4 bv = d4{0., 0., 0., 0.};
-                if (s >= 0) bv = *reinterpret_cast<const d4*>(Ms + n0);
-                const ushort4 rows = *reinterpret_cast<const ushort4*>(pv + n0);
-                const double a0 = V[static_cast<int>(rows.x) * kpad + 16 * rt + col];
-                const double a1 = V[static_cast<int>(rows.y) * kpad + 16 * rt + col];
-                const double a2 = V[static_cast<int>(rows.z) * kpad + 16 * rt + col];
-                const double a3 = V[static_cast<int>(rows.w) * kpad + 16 * rt + col];
-                c1 = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, bv[0], c1, 0, 0, 0);
-                c1 = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, bv[1], c1, 0, 0, 0);
-                c1 = __builtin_amdgcn_mfma_f64_16x16x4f64(a2, bv[2], c1, 0, 0, 0);
-                c1 = __builtin_amdgcn_mfma_f64_16x16x4f64(a3, bv[3], c1, 0, 0, 0);
-            }
-            // MFMA step s used node n0 + s in lane group kq: the K order is a
-            // permutation of the nodes shared by A and B, so C is unchanged.
+        // alternate the permutation buffer per processed slot: a wave still in
+        // the previous slot's GEMM2 reads the other buffer
+        uint16_t* pv = pvs + (nproc & 1) * LD;
+        ++nproc;
+        {
+            const ushort4* g = reinterpret_cast<const ushort4*>(pvec + static_cast<size_t>(pos) * LD);
+            for (int i = tid; i < LD / 4; i += 64 * W) reinterpret_cast<ushort4*>(pv)[i] = g[i];
+        }
+        __syncthreads();  // pv ready; every wave is past the previous slot's GEMM2
+        // GEMM1: units (rank tile, K half) for kt <= 2, (rank tile) for kt > 2;
+        // a unit covers both column tiles so each operator load feeds two MFMAs
+        const int halves = kt <= 2 ? 2 : 1;
+        const int kb_per = (KB + halves - 1) / halves;
+        for (int u = wave; u < kt * halves; u += W) {
+            const int rt = u / halves, h = u % halves;
+            const int kb0 = h * kb_per, kb1 = min(KB, kb0 + kb_per);
+            d4 c1[kM2LCols];
 #pragma unroll
-            for (int i = 0; i < 4; ++i) Cb[(16 * rt + 4 * i + kq) * kM2LTile + 16 * ct + col] = c1[i];
+            for (int ct = 0; ct < kM2LCols; ++ct) c1[ct] = d4{0., 0., 0., 0.};
+            const double* Vc = V + 16 * rt + col;
+            for (int kb = kb0; kb < kb1; ++kb) {
+                const int n0 = 16 * kb + 4 * kq;
+                d4 bv[kM2LCols];
+#pragma unroll
+                for (int ct = 0; ct < kM2LCols; ++ct) {
+                    bv[ct] = d4{0., 0., 0., 0.};
+                    if (src[ct] >= 0) bv[ct] = *reinterpret_cast<const d4*>(Ml + static_cast<int64_t>(src[ct]) * LD + n0);
+                }
+                const ushort4 rows = *reinterpret_cast<const ushort4*>(pv + n0);
+                const double a0 = Vc[static_cast<int>(rows.x) * kpad];
+                const double a1 = Vc[static_cast<int>(rows.y) * kpad];
+                const double a2 = Vc[static_cast<int>(rows.z) * kpad];
+                const double a3 = Vc[static_cast<int>(rows.w) * kpad];
+#pragma unroll
+                for (int ct = 0; ct < kM2LCols; ++ct) {
+                    c1[ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, bv[ct][0], c1[ct], 0, 0, 0);
+                    c1[ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, bv[ct][1], c1[ct], 0, 0, 0);
+                    c1[ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(a2, bv[ct][2], c1[ct], 0, 0, 0);
+                    c1[ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(a3, bv[ct][3], c1[ct], 0, 0, 0);
+                }
+            }
+            // MFMA step s used node n0 + s in lane group kq: a permutation of the
+            // K order shared by A and B, so the product is unchanged.
+            double* Cw = Cp + static_cast<size_t>(h) * kpadmax * TC;
+#pragma unroll
+            for (int ct = 0; ct < kM2LCols; ++ct)
+#pragma unroll
+                for (int i = 0; i < 4; ++i) Cw[(16 * rt + 4 * i + kq) * TC + 16 * ct + col] = c1[ct][i];
         }
         __syncthreads();
-        // GEMM2: this wave's row tiles, both column tiles
+        // GEMM2: this wave's row tiles, both column tiles; B = sum of the K halves
         int urow[NTW];
 #pragma unroll
         for (int t = 0; t < NTW; ++t) {
             const int nt = wave + W * t;
             urow[t] = nt < NT ? static_cast<int>(pv[16 * nt + col]) * kpad : 0;
         }
+        const double* C1 = Cp + static_cast<size_t>(kpadmax) * TC;
         for (int kb = 0; kb < kt; ++kb) {
             const int r0 = 16 * kb + 4 * kq;
             double b[4][kM2LCols];
 #pragma unroll
             for (int s2 = 0; s2 < 4; ++s2)
 #pragma unroll
-                for (int ct = 0; ct < kM2LCols; ++ct) b[s2][ct] = Cb[(r0 + s2) * kM2LTile + 16 * ct + col];
+                for (int ct = 0; ct < kM2LCols; ++ct) {
+                    const int idx = (r0 + s2) * TC + 16 * ct + col;
+                    b[s2][ct] = halves == 2 ? Cp[idx] + C1[idx] : Cp[idx];
+                }
 #pragma unroll
             for (int t = 0; t < NTW; ++t) {
                 if (wave + W * t < NT) {
@@ -509,7 +554,6 @@ __global__ __launch_bounds__(256) void k_m2l_f64(M2LLevels lv, const int32_t* __
                 }
             }
         }
-        buf ^= 1;
     }
     const double scale = lv.scale[li];
 #pragma unroll
@@ -627,7 +671,7 @@ int m2l_impl(Context* c) {
         lv.tile_begin[i + 1] = tiles;
     }
     const unsigned grid = static_cast<unsigned>((tiles + 7) / 8 * 8);
-    const size_t shmem = sizeof(double) * 2 * static_cast<size_t>(c->kpad_max) * kM2LTile;
+    const size_t shmem = sizeof(double) * 2 * static_cast<size_t>(c->kpad_max) * kM2LTile + sizeof(uint16_t) * 2 * c->LD;
     prof_begin(c, "m2l");
     hipLaunchKernelGGL(k_m2l_f64<P>, dim3(grid), dim3(64 * kM2LWaves), shmem, c->stream, lv,
                        c->d_m2l.as<int32_t>(), c->d_order.as<int32_t>(), c->d_pcount.as<int32_t>(),
