@@ -361,8 +361,8 @@ __global__ __launch_bounds__(256) void k_p2p_f64(int64_t n, const int32_t* __res
 // exactly one tile, so L is written without atomics.  Node indices are padded
 // to LD (multiple of 16): padded permutation entries point at a zero operator
 // row and the multipole padding is zero, so no lane needs a bounds mask.
-// Workgroups are remapped so that each XCD walks a contiguous range of tiles
-// (neighbouring targets share sources through that XCD's L2).
+// The eight parities run on the eight XCDs side by side, so the XCDs read the
+// same source multipoles at about the same time.
 // MFMA 16x16x4 f64 operand layout (checked by hfmm_selftest_mfma_f64):
 // A[i][k] in lane i + 16k, B[k][j] in lane j + 16k, and register r of lane l
 // holds D[4r + l/16][l%16].
@@ -404,31 +404,23 @@ __global__ __launch_bounds__(256) void k_m2l_f64(M2LLevels lv, const int32_t* __
     const int col = lane & 15, kq = lane >> 4;
     const int KB = LD / 16;  // 16-node blocks
 
-    // XCD-aware tile order: workgroup i runs on XCD i % 8
-    const int nwg = gridDim.x;
-    const int per_xcd = (nwg + 7) / 8;
-    const int tile = (blockIdx.x % 8) * per_xcd + blockIdx.x / 8;
-    if (tile >= lv.tile_begin[lv.nlev]) return;
+    // Workgroup i runs on XCD i % 8.  Parity e = i % 8 and the tile index
+    // within the parity = i / 8, so the eight XCDs walk the same parents (one
+    // child index each) at the same time and share the source multipoles
+    // through the memory-side cache.
+    const int bid = blockIdx.x;
+    if (bid >= lv.tile_begin[lv.nlev]) return;
     int li = 0;
-    while (tile >= lv.tile_begin[li + 1]) ++li;
-    const int ltile = tile - lv.tile_begin[li];
+    while (bid >= lv.tile_begin[li + 1]) ++li;
+    const int lb = bid - lv.tile_begin[li];
+    const int e = lb % 8;
+    const int ptile = lb / 8;
     const int32_t* pc = pcount + lv.pcount_off[li];
-    int e = -1, first = 0, cnt = 0;
-    {
-        int tacc = 0, cacc = 0;
-        for (int q = 0; q < 8; ++q) {
-            const int c = pc[q];
-            const int t = (c + TC - 1) / TC;
-            if (e < 0 && ltile < tacc + t) {
-                e = q;
-                first = cacc + (ltile - tacc) * TC;
-                cnt = min(TC, c - (ltile - tacc) * TC);
-            }
-            tacc += t;
-            cacc += c;
-        }
-    }
-    if (e < 0) return;  // uniform over the workgroup
+    int first = 0;
+    for (int q = 0; q < e; ++q) first += pc[q];
+    const int cnt = min(TC, pc[e] - ptile * TC);
+    if (cnt <= 0) return;  // uniform over the workgroup
+    first += ptile * TC;
     for (int j = tid; j < kFarSlots; j += 64 * W) {
         const int pos = slotpos[e * kFarSlots + j];
         const int c = static_cast<int>(pidx[pos]);
@@ -494,13 +486,23 @@ __global__ __launch_bounds__(256) void k_m2l_f64(M2LLevels lv, const int32_t* __
 #pragma unroll
             for (int ct = 0; ct < kM2LCols; ++ct) c1[ct] = d4{0., 0., 0., 0.};
             const double* Vc = V + 16 * rt + col;
+            const double* Ms[kM2LCols];
+#pragma unroll
+            for (int ct = 0; ct < kM2LCols; ++ct)
+                Ms[ct] = Ml + static_cast<int64_t>(src[ct] < 0 ? 0 : src[ct]) * LD + 4 * kq;
+            // one block of look-ahead on the multipole stream
+            d4 bnext[kM2LCols];
+#pragma unroll
+            for (int ct = 0; ct < kM2LCols; ++ct)
+                bnext[ct] = (src[ct] >= 0 && kb0 < kb1) ? *reinterpret_cast<const d4*>(Ms[ct] + 16 * kb0)
+                                                        : d4{0., 0., 0., 0.};
             for (int kb = kb0; kb < kb1; ++kb) {
                 const int n0 = 16 * kb + 4 * kq;
                 d4 bv[kM2LCols];
 #pragma unroll
                 for (int ct = 0; ct < kM2LCols; ++ct) {
-                    bv[ct] = d4{0., 0., 0., 0.};
-                    if (src[ct] >= 0) bv[ct] = *reinterpret_cast<const d4*>(Ml + static_cast<int64_t>(src[ct]) * LD + n0);
+                    bv[ct] = bnext[ct];
+                    if (src[ct] >= 0 && kb + 1 < kb1) bnext[ct] = *reinterpret_cast<const d4*>(Ms[ct] + 16 * (kb + 1));
                 }
                 const ushort4 rows = *reinterpret_cast<const ushort4*>(pv + n0);
                 const double a0 = Vc[static_cast<int>(rows.x) * kpad];
@@ -667,10 +669,12 @@ int m2l_impl(Context* c) {
         lv.pcount_off[i] = 8 * l;
         // 2^l cells per axis: scale factor 2 / (width / 2^l) (FInterpMatrixKernel.hpp:124-133)
         lv.scale[i] = 2. / (c->P.box_width / static_cast<double>(int64_t(1) << l));
-        tiles += static_cast<int>((c->ncell[l] + kM2LTile - 1) / kM2LTile + 8);
+        // segment of 8 x (tiles of the largest parity); the parity counts are
+        // on the device, ceil(ncell/32) bounds every parity's tile count
+        tiles += 8 * static_cast<int>((c->ncell[l] + kM2LTile - 1) / kM2LTile);
         lv.tile_begin[i + 1] = tiles;
     }
-    const unsigned grid = static_cast<unsigned>((tiles + 7) / 8 * 8);
+    const unsigned grid = static_cast<unsigned>(tiles);
     const size_t shmem = sizeof(double) * 2 * static_cast<size_t>(c->kpad_max) * kM2LTile + sizeof(uint16_t) * 2 * c->LD;
     prof_begin(c, "m2l");
     hipLaunchKernelGGL(k_m2l_f64<P>, dim3(grid), dim3(64 * kM2LWaves), shmem, c->stream, lv,
