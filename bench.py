@@ -96,6 +96,18 @@ def cpu_baseline(seconds_budget=30.0):
                       f"{st.t_total:.2f} s, M2L {st.t_m2l:.2f} s, P2P {st.t_p2p:.2f} s, wall {wall:.2f} s"}
 
 
+def profiled_traffic(n, h, p):
+    """HBM bytes per M2L launch from the committed rocprofv3 summary of this
+    workload (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, see the summary), or None."""
+    path = os.path.join(ROOT, "profiles", f"m2l_fp64_uniform{n // 1_000_000}M_h{h}_p{p}.md")
+    if n % 1_000_000 or not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch:"):
+            return float(line.split(":", 1)[1])
+    return None
+
+
 def copy_calibration(mbytes=512, reps=3):
     """Elementwise copy of known size (read N bytes, write N bytes) for scaling
     the FETCH_SIZE / WRITE_SIZE counters of a profiled run."""
@@ -175,7 +187,8 @@ def main():
         "config": {"workload": f"uniform-{n}-h{h}-p{p}-fp64", "particles": n, "height": h, "order": p,
                    "epsilon": 10.0 ** -p, "precision": "fp64", "distribution": "uniform unit cube, drand48 seed 2"},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP64_MFMA_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / FP64_MFMA_PEAK_TFLOPS, "traffic": None, "kernel": "k_m2l_f64",
+                     "frac": achieved / FP64_MFMA_PEAK_TFLOPS, "traffic": profiled_traffic(n, h, p),
+                     "traffic_source": "profiles/ rocprofv3 FETCH_SIZE/WRITE_SIZE summary", "kernel": "k_m2l_f64",
                      "algorithmic_flops_per_step": flops, "m2l_interactions": n_m2l,
                      "m2l_ms_per_step": m2l_ms_step, "m2l_launches_per_step": m2l_launches / args.steps},
         "kernel_ms_per_step": {k: v[0] / args.steps for k, v in sorted(times.items())},
