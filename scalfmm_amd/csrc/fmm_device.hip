@@ -385,24 +385,30 @@ __global__ __launch_bounds__(256) void k_m2l_f64(M2LLevels lv, const int32_t* __
                                                 const int16_t* __restrict__ slotpos, const uint32_t* __restrict__ pidx,
                                                 const uint16_t* __restrict__ pvec, const int32_t* __restrict__ rank,
                                                 const int64_t* __restrict__ opoff, const double* __restrict__ ops,
-                                                const double* __restrict__ M, double* __restrict__ L, int LD,
-                                                int kpadmax) {
+                                                const double* __restrict__ M, const double* __restrict__ Mzero,
+                                                double* __restrict__ L, int LD, int kpadmax) {
     constexpr int NN = P * P * P;
     constexpr int W = kM2LWaves;
     constexpr int NT = (NN + 15) / 16;
     constexpr int NTW = (NT + W - 1) / W;
     constexpr int TC = kM2LTile;
-    // LDS: per-slot tables, two node-permutation buffers, two partial C tiles
+    constexpr int KTM = kMaxRankTiles;
+    // LDS: per-slot tables, two node-permutation buffers, W partial C tiles
     __shared__ int16_t s_pos[kFarSlots];
     __shared__ int32_t s_rank[kFarSlots];
     __shared__ int64_t s_off[kFarSlots];
-    extern __shared__ double smem[];  // C[2][kpadmax][TC] then pv[2][LD] (uint16)
+    extern __shared__ double smem[];  // C[W][kpadmax][TC] then pv[2][LD] (uint16)
     double* Cp = smem;
-    uint16_t* pvs = reinterpret_cast<uint16_t*>(smem + 2 * static_cast<size_t>(kpadmax) * TC);
+    uint16_t* pvs = reinterpret_cast<uint16_t*>(smem + static_cast<size_t>(W) * kpadmax * TC);
     const int tid = threadIdx.x;
-    const int lane = tid & 63, wave = tid >> 6;
+    const int lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);  // uniform per wave
     const int col = lane & 15, kq = lane >> 4;
     const int KB = LD / 16;  // 16-node blocks
+    // this wave's quarter of the node range in GEMM1
+    const int kbq0 = (KB * wave) / W, kbq1 = (KB * (wave + 1)) / W;
+    // this wave's row tiles in GEMM2: nt = wave + W t, t < ntw
+    const int ntw = (NT - wave + W - 1) / W;
 
     // Workgroup i runs on XCD i % 8.  Parity e = i % 8 and the tile index
     // within the parity = i / 8, so the eight XCDs walk the same parents (one
@@ -449,13 +455,14 @@ __global__ __launch_bounds__(256) void k_m2l_f64(M2LLevels lv, const int32_t* __
         for (int ct = 0; ct < kM2LCols; ++ct) acc[t][ct] = d4{0., 0., 0., 0.};
     int nproc = 0;
     for (int j = 0; j < kFarSlots; ++j) {
-        int src[kM2LCols];
+        const double* Mrow[kM2LCols];
         bool any = false;
 #pragma unroll
         for (int ct = 0; ct < kM2LCols; ++ct) {
-            src[ct] = nsrc[ct];
-            any |= src[ct] >= 0;
-            // prefetch the next slot's sources
+            const int s = nsrc[ct];
+            any |= s >= 0;
+            // a missing source reads the zero multipole (no divergent branches)
+            Mrow[ct] = (s >= 0 ? Ml + static_cast<int64_t>(s) * LD : Mzero) + 4 * kq;
             nsrc[ct] = (tgt[ct] >= 0 && j + 1 < kFarSlots) ? lslots[static_cast<int64_t>(tgt[ct]) * kFarSlots + j + 1]
                                                             : -1;
         }
@@ -475,65 +482,53 @@ __global__ __launch_bounds__(256) void k_m2l_f64(M2LLevels lv, const int32_t* __
             for (int i = tid; i < LD / 4; i += 64 * W) reinterpret_cast<ushort4*>(pv)[i] = g[i];
         }
         __syncthreads();  // pv ready; every wave is past the previous slot's GEMM2
-        // GEMM1: units (rank tile, K half) for kt <= 2, (rank tile) for kt > 2;
-        // a unit covers both column tiles so each operator load feeds two MFMAs
-        const int halves = kt <= 2 ? 2 : 1;
-        const int kb_per = (KB + halves - 1) / halves;
-        for (int u = wave; u < kt * halves; u += W) {
-            const int rt = u / halves, h = u % halves;
-            const int kb0 = h * kb_per, kb1 = min(KB, kb0 + kb_per);
-            d4 c1[kM2LCols];
+        // GEMM1: this wave's quarter of the nodes, every rank tile, both column
+        // tiles; one multipole load feeds kt MFMAs, one operator load two.
+        {
+            d4 c1[KTM][kM2LCols];
 #pragma unroll
-            for (int ct = 0; ct < kM2LCols; ++ct) c1[ct] = d4{0., 0., 0., 0.};
-            const double* Vc = V + 16 * rt + col;
-            const double* Ms[kM2LCols];
+            for (int rt = 0; rt < KTM; ++rt)
 #pragma unroll
-            for (int ct = 0; ct < kM2LCols; ++ct)
-                Ms[ct] = Ml + static_cast<int64_t>(src[ct] < 0 ? 0 : src[ct]) * LD + 4 * kq;
-            // one block of look-ahead on the multipole stream
-            d4 bnext[kM2LCols];
-#pragma unroll
-            for (int ct = 0; ct < kM2LCols; ++ct)
-                bnext[ct] = (src[ct] >= 0 && kb0 < kb1) ? *reinterpret_cast<const d4*>(Ms[ct] + 16 * kb0)
-                                                        : d4{0., 0., 0., 0.};
-            for (int kb = kb0; kb < kb1; ++kb) {
-                const int n0 = 16 * kb + 4 * kq;
+                for (int ct = 0; ct < kM2LCols; ++ct) c1[rt][ct] = d4{0., 0., 0., 0.};
+            const double* Vc = V + col;
+            for (int kb = kbq0; kb < kbq1; ++kb) {
                 d4 bv[kM2LCols];
 #pragma unroll
-                for (int ct = 0; ct < kM2LCols; ++ct) {
-                    bv[ct] = bnext[ct];
-                    if (src[ct] >= 0 && kb + 1 < kb1) bnext[ct] = *reinterpret_cast<const d4*>(Ms[ct] + 16 * (kb + 1));
-                }
-                const ushort4 rows = *reinterpret_cast<const ushort4*>(pv + n0);
-                const double a0 = Vc[static_cast<int>(rows.x) * kpad];
-                const double a1 = Vc[static_cast<int>(rows.y) * kpad];
-                const double a2 = Vc[static_cast<int>(rows.z) * kpad];
-                const double a3 = Vc[static_cast<int>(rows.w) * kpad];
+                for (int ct = 0; ct < kM2LCols; ++ct) bv[ct] = *reinterpret_cast<const d4*>(Mrow[ct] + 16 * kb);
+                const ushort4 rows = *reinterpret_cast<const ushort4*>(pv + 16 * kb + 4 * kq);
+                const int r0 = static_cast<int>(rows.x) * kpad, r1 = static_cast<int>(rows.y) * kpad;
+                const int r2 = static_cast<int>(rows.z) * kpad, r3 = static_cast<int>(rows.w) * kpad;
 #pragma unroll
-                for (int ct = 0; ct < kM2LCols; ++ct) {
-                    c1[ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, bv[ct][0], c1[ct], 0, 0, 0);
-                    c1[ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, bv[ct][1], c1[ct], 0, 0, 0);
-                    c1[ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(a2, bv[ct][2], c1[ct], 0, 0, 0);
-                    c1[ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(a3, bv[ct][3], c1[ct], 0, 0, 0);
+                for (int rt = 0; rt < KTM; ++rt) {
+                    if (rt < kt) {
+                        const double a0 = Vc[r0 + 16 * rt], a1 = Vc[r1 + 16 * rt];
+                        const double a2 = Vc[r2 + 16 * rt], a3 = Vc[r3 + 16 * rt];
+#pragma unroll
+                        for (int ct = 0; ct < kM2LCols; ++ct) {
+                            c1[rt][ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, bv[ct][0], c1[rt][ct], 0, 0, 0);
+                            c1[rt][ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, bv[ct][1], c1[rt][ct], 0, 0, 0);
+                            c1[rt][ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(a2, bv[ct][2], c1[rt][ct], 0, 0, 0);
+                            c1[rt][ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(a3, bv[ct][3], c1[rt][ct], 0, 0, 0);
+                        }
+                    }
                 }
             }
-            // MFMA step s used node n0 + s in lane group kq: a permutation of the
-            // K order shared by A and B, so the product is unchanged.
-            double* Cw = Cp + static_cast<size_t>(h) * kpadmax * TC;
+            // MFMA step s used node 16 kb + 4 kq + s in lane group kq: a
+            // permutation of the K order shared by A and B.
+            double* Cw = Cp + static_cast<size_t>(wave) * kpadmax * TC;
 #pragma unroll
-            for (int ct = 0; ct < kM2LCols; ++ct)
+            for (int rt = 0; rt < KTM; ++rt)
+                if (rt < kt)
 #pragma unroll
-                for (int i = 0; i < 4; ++i) Cw[(16 * rt + 4 * i + kq) * TC + 16 * ct + col] = c1[ct][i];
+                    for (int ct = 0; ct < kM2LCols; ++ct)
+#pragma unroll
+                        for (int i = 0; i < 4; ++i) Cw[(16 * rt + 4 * i + kq) * TC + 16 * ct + col] = c1[rt][ct][i];
         }
         __syncthreads();
-        // GEMM2: this wave's row tiles, both column tiles; B = sum of the K halves
+        // GEMM2: this wave's row tiles, both column tiles; B = sum of the W partials
         int urow[NTW];
 #pragma unroll
-        for (int t = 0; t < NTW; ++t) {
-            const int nt = wave + W * t;
-            urow[t] = nt < NT ? static_cast<int>(pv[16 * nt + col]) * kpad : 0;
-        }
-        const double* C1 = Cp + static_cast<size_t>(kpadmax) * TC;
+        for (int t = 0; t < NTW; ++t) urow[t] = t < ntw ? static_cast<int>(pv[16 * (wave + W * t) + col]) * kpad : 0;
         for (int kb = 0; kb < kt; ++kb) {
             const int r0 = 16 * kb + 4 * kq;
             double b[4][kM2LCols];
@@ -542,17 +537,22 @@ __global__ __launch_bounds__(256) void k_m2l_f64(M2LLevels lv, const int32_t* __
 #pragma unroll
                 for (int ct = 0; ct < kM2LCols; ++ct) {
                     const int idx = (r0 + s2) * TC + 16 * ct + col;
-                    b[s2][ct] = halves == 2 ? Cp[idx] + C1[idx] : Cp[idx];
+                    double v = Cp[idx];
+#pragma unroll
+                    for (int w = 1; w < W; ++w) v += Cp[static_cast<size_t>(w) * kpadmax * TC + idx];
+                    b[s2][ct] = v;
                 }
+            d4 av[NTW];
+#pragma unroll
+            for (int t = 0; t < NTW; ++t) av[t] = *reinterpret_cast<const d4*>(U + urow[t] + r0);
 #pragma unroll
             for (int t = 0; t < NTW; ++t) {
-                if (wave + W * t < NT) {
-                    const d4 av = *reinterpret_cast<const d4*>(U + urow[t] + r0);
+                if (t < ntw) {
 #pragma unroll
                     for (int s2 = 0; s2 < 4; ++s2)
 #pragma unroll
                         for (int ct = 0; ct < kM2LCols; ++ct)
-                            acc[t][ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[s2], b[s2][ct], acc[t][ct], 0, 0, 0);
+                            acc[t][ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[t][s2], b[s2][ct], acc[t][ct], 0, 0, 0);
                 }
             }
         }
@@ -563,8 +563,8 @@ __global__ __launch_bounds__(256) void k_m2l_f64(M2LLevels lv, const int32_t* __
         if (tgt[ct] < 0) continue;
 #pragma unroll
         for (int t = 0; t < NTW; ++t) {
-            const int nt = wave + W * t;
-            if (nt < NT) {
+            if (t < ntw) {
+                const int nt = wave + W * t;
 #pragma unroll
                 for (int i = 0; i < 4; ++i) {
                     const int row = 16 * nt + 4 * i + kq;
@@ -675,13 +675,18 @@ int m2l_impl(Context* c) {
         lv.tile_begin[i + 1] = tiles;
     }
     const unsigned grid = static_cast<unsigned>(tiles);
-    const size_t shmem = sizeof(double) * 2 * static_cast<size_t>(c->kpad_max) * kM2LTile + sizeof(uint16_t) * 2 * c->LD;
+    const size_t shmem =
+        sizeof(double) * kM2LWaves * static_cast<size_t>(c->kpad_max) * kM2LTile + sizeof(uint16_t) * 2 * c->LD;
+    if (shmem > 64 * 1024)
+        HFMM_HIP(c, hipFuncSetAttribute(reinterpret_cast<const void*>(&k_m2l_f64<P>),
+                                        hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(shmem)));
     prof_begin(c, "m2l");
     hipLaunchKernelGGL(k_m2l_f64<P>, dim3(grid), dim3(64 * kM2LWaves), shmem, c->stream, lv,
                        c->d_m2l.as<int32_t>(), c->d_order.as<int32_t>(), c->d_pcount.as<int32_t>(),
                        c->d_slotpos.as<int16_t>(), c->d_pidx.as<uint32_t>(), c->d_pvec.as<uint16_t>(),
                        c->d_rank.as<int32_t>(), c->d_opoff.as<int64_t>(), c->d_ops.as<double>(), c->d_M.as<double>(),
-                       c->d_L.as<double>(), c->LD, c->kpad_max);
+                       c->d_M.as<double>() + static_cast<size_t>(c->total_cells) * c->LD, c->d_L.as<double>(), c->LD,
+                       c->kpad_max);
     prof_end(c);
     return check_launch(c, "m2l");
 }

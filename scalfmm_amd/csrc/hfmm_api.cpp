@@ -285,7 +285,10 @@ int hfmm_evaluate(hfmm_ctx* h, uint32_t ops) {
     HFMM_HIP(c, hipSetDevice(c->device));
     const int L = c->height() - 1;
     const size_t exp_bytes = sizeof(double) * static_cast<size_t>(c->total_cells) * c->LD;
-    HFMM_TRY(hfmm::ensure(c, c->d_M, exp_bytes));
+    // one extra zero multipole after the last cell: the M2L kernel reads it for missing sources
+    HFMM_TRY(hfmm::ensure(c, c->d_M, exp_bytes + sizeof(double) * c->LD));
+    HFMM_HIP(c, hipMemsetAsync(c->d_M.as<double>() + static_cast<size_t>(c->total_cells) * c->LD, 0,
+                               sizeof(double) * c->LD, c->stream));
     HFMM_TRY(hfmm::ensure(c, c->d_L, exp_bytes));
     HFMM_TRY(hfmm::ensure(c, c->d_res_sorted, sizeof(double) * 4 * c->n));
     HFMM_TRY(hfmm::ensure(c, c->d_res, sizeof(double) * 4 * c->n));
