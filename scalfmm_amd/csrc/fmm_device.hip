@@ -379,23 +379,154 @@ struct M2LLevels {
     double scale[kMaxLevels];
 };
 
+// Per-workgroup state of an M2L tile.
+template <int P>
+struct M2LTileState {
+    static constexpr int NN = P * P * P;
+    static constexpr int NT = (NN + 15) / 16;
+    static constexpr int NTW = (NT + kM2LWaves - 1) / kM2LWaves;
+    d4 acc[NTW][kM2LCols];
+};
+
+// Runs the slots [i0, i1) of the tile's (rank-ordered) slot list, all with
+// KT rank tiles (kpad = 16 KT), so every MFMA accumulator update is
+// unconditional.  Per slot: GEMM1 over this wave's node quarter into its
+// partial C tile, barrier, GEMM2 over this wave's row tiles.
+template <int P, int KT>
+__device__ __forceinline__ void m2l_segment(M2LTileState<P>& st, int i0, int i1, const int16_t* s_slot,
+                                            const int16_t* s_pos, const int64_t* s_off, const int32_t* lslots,
+                                            const int (&tgt)[kM2LCols], int (&nsrc)[kM2LCols],
+                                            const double* __restrict__ Ml, const double* __restrict__ Mzero,
+                                            const double* __restrict__ ops, const uint16_t* __restrict__ pvec,
+                                            uint16_t* pvs, double* Cp, int& nproc, int LD, int kpadmax, int tid,
+                                            int wave, int col, int kq, int kbq0, int kbq1) {
+    constexpr int NN = P * P * P;
+    constexpr int W = kM2LWaves;
+    constexpr int NTW = M2LTileState<P>::NTW;
+    constexpr int TC = kM2LTile;
+    constexpr int kpad = 16 * KT;
+    for (int i = i0; i < i1; ++i) {
+        const double* Mrow[kM2LCols];
+        bool any = false;
+#pragma unroll
+        for (int ct = 0; ct < kM2LCols; ++ct) {
+            const int s = nsrc[ct];
+            any |= s >= 0;
+            // a missing source reads the zero multipole (no divergent branches)
+            Mrow[ct] = (s >= 0 ? Ml + static_cast<int64_t>(s) * LD : Mzero) + 4 * kq;
+            nsrc[ct] = (tgt[ct] >= 0 && i + 1 < kFarSlots)
+                           ? lslots[static_cast<int64_t>(tgt[ct]) * kFarSlots + s_slot[i + 1]]
+                           : -1;
+        }
+        if (__ballot(any) == 0) continue;  // same answer in every wave
+        const int pos = s_pos[i];
+        const double* U = ops + s_off[i];
+        const double* V = U + static_cast<size_t>(NN + 1) * kpad;
+        // alternate the permutation buffer per processed slot: a wave still in
+        // the previous slot's GEMM2 reads the other buffer
+        uint16_t* pv = pvs + (nproc & 1) * LD;
+        ++nproc;
+        {
+            const ushort4* g = reinterpret_cast<const ushort4*>(pvec + static_cast<size_t>(pos) * LD);
+            for (int q = tid; q < LD / 4; q += 64 * W) reinterpret_cast<ushort4*>(pv)[q] = g[q];
+        }
+        __syncthreads();  // pv ready; every wave is past the previous slot's GEMM2
+        {
+            d4 c1[KT][kM2LCols];
+#pragma unroll
+            for (int rt = 0; rt < KT; ++rt)
+#pragma unroll
+                for (int ct = 0; ct < kM2LCols; ++ct) c1[rt][ct] = d4{0., 0., 0., 0.};
+            const double* Vc = V + col;
+            for (int kb = kbq0; kb < kbq1; ++kb) {
+                const d4 b0 = *reinterpret_cast<const d4*>(Mrow[0] + 16 * kb);
+                const d4 b1 = *reinterpret_cast<const d4*>(Mrow[1] + 16 * kb);
+                const ushort4 rows = *reinterpret_cast<const ushort4*>(pv + 16 * kb + 4 * kq);
+                const double* v0 = Vc + static_cast<int>(rows.x) * kpad;
+                const double* v1 = Vc + static_cast<int>(rows.y) * kpad;
+                const double* v2 = Vc + static_cast<int>(rows.z) * kpad;
+                const double* v3 = Vc + static_cast<int>(rows.w) * kpad;
+                double a[KT][4];
+#pragma unroll
+                for (int rt = 0; rt < KT; ++rt) {
+                    a[rt][0] = v0[16 * rt];
+                    a[rt][1] = v1[16 * rt];
+                    a[rt][2] = v2[16 * rt];
+                    a[rt][3] = v3[16 * rt];
+                }
+#pragma unroll
+                for (int s2 = 0; s2 < 4; ++s2)
+#pragma unroll
+                    for (int rt = 0; rt < KT; ++rt) {
+                        c1[rt][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[rt][s2], b0[s2], c1[rt][0], 0, 0, 0);
+                        c1[rt][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[rt][s2], b1[s2], c1[rt][1], 0, 0, 0);
+                    }
+            }
+            // MFMA step s used node 16 kb + 4 kq + s in lane group kq: a
+            // permutation of the K order shared by A and B.
+            double* Cw = Cp + static_cast<size_t>(wave) * kpadmax * TC;
+#pragma unroll
+            for (int rt = 0; rt < KT; ++rt)
+#pragma unroll
+                for (int ct = 0; ct < kM2LCols; ++ct)
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) Cw[(16 * rt + 4 * r + kq) * TC + 16 * ct + col] = c1[rt][ct][r];
+        }
+        __syncthreads();
+        // GEMM2: row tiles nt = wave + W t for all t < NTW (a wave with fewer
+        // real tiles computes throw-away ones on row 0 at no cost: the barrier
+        // waits for the busiest wave anyway); B = sum of the W partial C tiles.
+        int urow[NTW];
+#pragma unroll
+        for (int t = 0; t < NTW; ++t) {
+            const int row = 16 * (wave + W * t) + col;
+            urow[t] = row < LD ? static_cast<int>(pv[row]) * kpad : 0;
+        }
+#pragma unroll
+        for (int kb = 0; kb < KT; ++kb) {
+            const int r0 = 16 * kb + 4 * kq;
+            d4 av[NTW];
+#pragma unroll
+            for (int t = 0; t < NTW; ++t) av[t] = *reinterpret_cast<const d4*>(U + urow[t] + r0);
+            double b[4][kM2LCols];
+#pragma unroll
+            for (int s2 = 0; s2 < 4; ++s2)
+#pragma unroll
+                for (int ct = 0; ct < kM2LCols; ++ct) {
+                    const int idx = (r0 + s2) * TC + 16 * ct + col;
+                    double v = Cp[idx];
+#pragma unroll
+                    for (int w = 1; w < W; ++w) v += Cp[static_cast<size_t>(w) * kpadmax * TC + idx];
+                    b[s2][ct] = v;
+                }
+#pragma unroll
+            for (int s2 = 0; s2 < 4; ++s2)
+#pragma unroll
+                for (int t = 0; t < NTW; ++t)
+#pragma unroll
+                    for (int ct = 0; ct < kM2LCols; ++ct)
+                        st.acc[t][ct] =
+                            __builtin_amdgcn_mfma_f64_16x16x4f64(av[t][s2], b[s2][ct], st.acc[t][ct], 0, 0, 0);
+        }
+    }
+}
+
 template <int P>
 __global__ __launch_bounds__(256) void k_m2l_f64(M2LLevels lv, const int32_t* __restrict__ slots,
                                                 const int32_t* __restrict__ order, const int32_t* __restrict__ pcount,
-                                                const int16_t* __restrict__ slotpos, const uint32_t* __restrict__ pidx,
-                                                const uint16_t* __restrict__ pvec, const int32_t* __restrict__ rank,
+                                                const int16_t* __restrict__ slotord, const int32_t* __restrict__ slotseg,
+                                                const uint32_t* __restrict__ pidx, const uint16_t* __restrict__ pvec,
                                                 const int64_t* __restrict__ opoff, const double* __restrict__ ops,
                                                 const double* __restrict__ M, const double* __restrict__ Mzero,
                                                 double* __restrict__ L, int LD, int kpadmax) {
     constexpr int NN = P * P * P;
     constexpr int W = kM2LWaves;
-    constexpr int NT = (NN + 15) / 16;
-    constexpr int NTW = (NT + W - 1) / W;
+    constexpr int NT = M2LTileState<P>::NT;
+    constexpr int NTW = M2LTileState<P>::NTW;
     constexpr int TC = kM2LTile;
-    constexpr int KTM = kMaxRankTiles;
-    // LDS: per-slot tables, two node-permutation buffers, W partial C tiles
+    // LDS: rank-ordered slot tables, two node-permutation buffers, W partial C tiles
+    __shared__ int16_t s_slot[kFarSlots];
     __shared__ int16_t s_pos[kFarSlots];
-    __shared__ int32_t s_rank[kFarSlots];
     __shared__ int64_t s_off[kFarSlots];
     extern __shared__ double smem[];  // C[W][kpadmax][TC] then pv[2][LD] (uint16)
     double* Cp = smem;
@@ -407,7 +538,7 @@ __global__ __launch_bounds__(256) void k_m2l_f64(M2LLevels lv, const int32_t* __
     const int KB = LD / 16;  // 16-node blocks
     // this wave's quarter of the node range in GEMM1
     const int kbq0 = (KB * wave) / W, kbq1 = (KB * (wave + 1)) / W;
-    // this wave's row tiles in GEMM2: nt = wave + W t, t < ntw
+    // this wave's real row tiles in GEMM2: nt = wave + W t, t < ntw
     const int ntw = (NT - wave + W - 1) / W;
 
     // Workgroup i runs on XCD i % 8.  Parity e = i % 8 and the tile index
@@ -427,148 +558,58 @@ __global__ __launch_bounds__(256) void k_m2l_f64(M2LLevels lv, const int32_t* __
     const int cnt = min(TC, pc[e] - ptile * TC);
     if (cnt <= 0) return;  // uniform over the workgroup
     first += ptile * TC;
-    for (int j = tid; j < kFarSlots; j += 64 * W) {
-        const int pos = slotpos[e * kFarSlots + j];
-        const int c = static_cast<int>(pidx[pos]);
-        s_pos[j] = static_cast<int16_t>(pos);
-        s_rank[j] = rank[c];
-        s_off[j] = opoff[c];
+    for (int i = tid; i < kFarSlots; i += 64 * W) {
+        const int j = slotord[(e * kFarSlots + i) * 2];
+        const int pos = slotord[(e * kFarSlots + i) * 2 + 1];
+        s_slot[i] = static_cast<int16_t>(j);
+        s_pos[i] = static_cast<int16_t>(pos);
+        s_off[i] = opoff[pidx[pos]];
     }
+    int seg[5];
+#pragma unroll
+    for (int q = 0; q < 5; ++q) seg[q] = slotseg[e * 5 + q];
     const int64_t coff = lv.cell_off[li];
     const int32_t* lslots = slots + coff * kFarSlots;
     const double* Ml = M + coff * LD;
     double* Ll = L + coff * LD;
     int tgt[kM2LCols];
-    int nsrc[kM2LCols];
 #pragma unroll
     for (int ct = 0; ct < kM2LCols; ++ct) {
         const int cc = 16 * ct + col;
         tgt[ct] = cc < cnt ? order[coff + first + cc] : -1;
-        nsrc[ct] = tgt[ct] >= 0 ? lslots[static_cast<int64_t>(tgt[ct]) * kFarSlots] : -1;
     }
     __syncthreads();
+    int nsrc[kM2LCols];
+#pragma unroll
+    for (int ct = 0; ct < kM2LCols; ++ct)
+        nsrc[ct] = tgt[ct] >= 0 ? lslots[static_cast<int64_t>(tgt[ct]) * kFarSlots + s_slot[0]] : -1;
 
-    d4 acc[NTW][kM2LCols];
+    M2LTileState<P> st;
 #pragma unroll
     for (int t = 0; t < NTW; ++t)
 #pragma unroll
-        for (int ct = 0; ct < kM2LCols; ++ct) acc[t][ct] = d4{0., 0., 0., 0.};
+        for (int ct = 0; ct < kM2LCols; ++ct) st.acc[t][ct] = d4{0., 0., 0., 0.};
     int nproc = 0;
-    for (int j = 0; j < kFarSlots; ++j) {
-        const double* Mrow[kM2LCols];
-        bool any = false;
-#pragma unroll
-        for (int ct = 0; ct < kM2LCols; ++ct) {
-            const int s = nsrc[ct];
-            any |= s >= 0;
-            // a missing source reads the zero multipole (no divergent branches)
-            Mrow[ct] = (s >= 0 ? Ml + static_cast<int64_t>(s) * LD : Mzero) + 4 * kq;
-            nsrc[ct] = (tgt[ct] >= 0 && j + 1 < kFarSlots) ? lslots[static_cast<int64_t>(tgt[ct]) * kFarSlots + j + 1]
-                                                            : -1;
-        }
-        if (__ballot(any) == 0) continue;  // same answer in every wave
-        const int pos = s_pos[j];
-        const int k = s_rank[j];
-        const int kpad = (k + 15) & ~15;
-        const int kt = kpad >> 4;
-        const double* U = ops + s_off[j];
-        const double* V = U + static_cast<size_t>(NN + 1) * kpad;
-        // alternate the permutation buffer per processed slot: a wave still in
-        // the previous slot's GEMM2 reads the other buffer
-        uint16_t* pv = pvs + (nproc & 1) * LD;
-        ++nproc;
-        {
-            const ushort4* g = reinterpret_cast<const ushort4*>(pvec + static_cast<size_t>(pos) * LD);
-            for (int i = tid; i < LD / 4; i += 64 * W) reinterpret_cast<ushort4*>(pv)[i] = g[i];
-        }
-        __syncthreads();  // pv ready; every wave is past the previous slot's GEMM2
-        // GEMM1: this wave's quarter of the nodes, every rank tile, both column
-        // tiles; one multipole load feeds kt MFMAs, one operator load two.
-        {
-            d4 c1[KTM][kM2LCols];
-#pragma unroll
-            for (int rt = 0; rt < KTM; ++rt)
-#pragma unroll
-                for (int ct = 0; ct < kM2LCols; ++ct) c1[rt][ct] = d4{0., 0., 0., 0.};
-            const double* Vc = V + col;
-            for (int kb = kbq0; kb < kbq1; ++kb) {
-                d4 bv[kM2LCols];
-#pragma unroll
-                for (int ct = 0; ct < kM2LCols; ++ct) bv[ct] = *reinterpret_cast<const d4*>(Mrow[ct] + 16 * kb);
-                const ushort4 rows = *reinterpret_cast<const ushort4*>(pv + 16 * kb + 4 * kq);
-                const int r0 = static_cast<int>(rows.x) * kpad, r1 = static_cast<int>(rows.y) * kpad;
-                const int r2 = static_cast<int>(rows.z) * kpad, r3 = static_cast<int>(rows.w) * kpad;
-#pragma unroll
-                for (int rt = 0; rt < KTM; ++rt) {
-                    if (rt < kt) {
-                        const double a0 = Vc[r0 + 16 * rt], a1 = Vc[r1 + 16 * rt];
-                        const double a2 = Vc[r2 + 16 * rt], a3 = Vc[r3 + 16 * rt];
-#pragma unroll
-                        for (int ct = 0; ct < kM2LCols; ++ct) {
-                            c1[rt][ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, bv[ct][0], c1[rt][ct], 0, 0, 0);
-                            c1[rt][ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, bv[ct][1], c1[rt][ct], 0, 0, 0);
-                            c1[rt][ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(a2, bv[ct][2], c1[rt][ct], 0, 0, 0);
-                            c1[rt][ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(a3, bv[ct][3], c1[rt][ct], 0, 0, 0);
-                        }
-                    }
-                }
-            }
-            // MFMA step s used node 16 kb + 4 kq + s in lane group kq: a
-            // permutation of the K order shared by A and B.
-            double* Cw = Cp + static_cast<size_t>(wave) * kpadmax * TC;
-#pragma unroll
-            for (int rt = 0; rt < KTM; ++rt)
-                if (rt < kt)
-#pragma unroll
-                    for (int ct = 0; ct < kM2LCols; ++ct)
-#pragma unroll
-                        for (int i = 0; i < 4; ++i) Cw[(16 * rt + 4 * i + kq) * TC + 16 * ct + col] = c1[rt][ct][i];
-        }
-        __syncthreads();
-        // GEMM2: this wave's row tiles, both column tiles; B = sum of the W partials
-        int urow[NTW];
-#pragma unroll
-        for (int t = 0; t < NTW; ++t) urow[t] = t < ntw ? static_cast<int>(pv[16 * (wave + W * t) + col]) * kpad : 0;
-        for (int kb = 0; kb < kt; ++kb) {
-            const int r0 = 16 * kb + 4 * kq;
-            double b[4][kM2LCols];
-#pragma unroll
-            for (int s2 = 0; s2 < 4; ++s2)
-#pragma unroll
-                for (int ct = 0; ct < kM2LCols; ++ct) {
-                    const int idx = (r0 + s2) * TC + 16 * ct + col;
-                    double v = Cp[idx];
-#pragma unroll
-                    for (int w = 1; w < W; ++w) v += Cp[static_cast<size_t>(w) * kpadmax * TC + idx];
-                    b[s2][ct] = v;
-                }
-            d4 av[NTW];
-#pragma unroll
-            for (int t = 0; t < NTW; ++t) av[t] = *reinterpret_cast<const d4*>(U + urow[t] + r0);
-#pragma unroll
-            for (int t = 0; t < NTW; ++t) {
-                if (t < ntw) {
-#pragma unroll
-                    for (int s2 = 0; s2 < 4; ++s2)
-#pragma unroll
-                        for (int ct = 0; ct < kM2LCols; ++ct)
-                            acc[t][ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[t][s2], b[s2][ct], acc[t][ct], 0, 0, 0);
-                }
-            }
-        }
-    }
+    m2l_segment<P, 1>(st, seg[0], seg[1], s_slot, s_pos, s_off, lslots, tgt, nsrc, Ml, Mzero, ops, pvec, pvs, Cp,
+                      nproc, LD, kpadmax, tid, wave, col, kq, kbq0, kbq1);
+    m2l_segment<P, 2>(st, seg[1], seg[2], s_slot, s_pos, s_off, lslots, tgt, nsrc, Ml, Mzero, ops, pvec, pvs, Cp,
+                      nproc, LD, kpadmax, tid, wave, col, kq, kbq0, kbq1);
+    m2l_segment<P, 3>(st, seg[2], seg[3], s_slot, s_pos, s_off, lslots, tgt, nsrc, Ml, Mzero, ops, pvec, pvs, Cp,
+                      nproc, LD, kpadmax, tid, wave, col, kq, kbq0, kbq1);
+    m2l_segment<P, 4>(st, seg[3], seg[4], s_slot, s_pos, s_off, lslots, tgt, nsrc, Ml, Mzero, ops, pvec, pvs, Cp,
+                      nproc, LD, kpadmax, tid, wave, col, kq, kbq0, kbq1);
     const double scale = lv.scale[li];
 #pragma unroll
     for (int ct = 0; ct < kM2LCols; ++ct) {
         if (tgt[ct] < 0) continue;
 #pragma unroll
         for (int t = 0; t < NTW; ++t) {
-            if (t < ntw) {
+            if (t < ntw) {  // real row tiles of this wave
                 const int nt = wave + W * t;
 #pragma unroll
-                for (int i = 0; i < 4; ++i) {
-                    const int row = 16 * nt + 4 * i + kq;
-                    if (row < NN) Ll[static_cast<int64_t>(tgt[ct]) * LD + row] = scale * acc[t][ct][i];
+                for (int r = 0; r < 4; ++r) {
+                    const int row = 16 * nt + 4 * r + kq;
+                    if (row < NN) Ll[static_cast<int64_t>(tgt[ct]) * LD + row] = scale * st.acc[t][ct][r];
                 }
             }
         }
@@ -683,8 +724,8 @@ int m2l_impl(Context* c) {
     prof_begin(c, "m2l");
     hipLaunchKernelGGL(k_m2l_f64<P>, dim3(grid), dim3(64 * kM2LWaves), shmem, c->stream, lv,
                        c->d_m2l.as<int32_t>(), c->d_order.as<int32_t>(), c->d_pcount.as<int32_t>(),
-                       c->d_slotpos.as<int16_t>(), c->d_pidx.as<uint32_t>(), c->d_pvec.as<uint16_t>(),
-                       c->d_rank.as<int32_t>(), c->d_opoff.as<int64_t>(), c->d_ops.as<double>(), c->d_M.as<double>(),
+                       c->d_slotord.as<int16_t>(), c->d_slotseg.as<int32_t>(), c->d_pidx.as<uint32_t>(),
+                       c->d_pvec.as<uint16_t>(), c->d_opoff.as<int64_t>(), c->d_ops.as<double>(), c->d_M.as<double>(),
                        c->d_M.as<double>() + static_cast<size_t>(c->total_cells) * c->LD, c->d_L.as<double>(), c->LD,
                        c->kpad_max);
     prof_end(c);

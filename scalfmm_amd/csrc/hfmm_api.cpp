@@ -154,12 +154,36 @@ static int upload_operators(Context* c) {
     }
     HFMM_TRY(ensure(c, c->d_slotpos, sizeof(int16_t) * sp.size()));
     HFMM_HIP(c, hipMemcpy(c->d_slotpos.ptr, sp.data(), sizeof(int16_t) * sp.size(), hipMemcpyHostToDevice));
+    // slots of every parity ordered by the rank tiles of their operator (stable),
+    // so the M2L kernel runs one specialised loop per rank-tile count
+    std::vector<int16_t> so(8 * kFarSlots * 2);
+    std::vector<int32_t> seg(8 * 5);
+    for (int e = 0; e < 8; ++e) {
+        int i = 0;
+        for (int kt = 1; kt <= 4; ++kt) {
+            seg[e * 5 + kt - 1] = i;
+            for (int j = 0; j < kFarSlots; ++j) {
+                const int pos = sp[e * kFarSlots + j];
+                const int k = op.rank[op.pidx[pos]];
+                if (((k + 15) >> 4) != kt) continue;
+                so[(e * kFarSlots + i) * 2] = static_cast<int16_t>(j);
+                so[(e * kFarSlots + i) * 2 + 1] = static_cast<int16_t>(pos);
+                ++i;
+            }
+        }
+        seg[e * 5 + 4] = i;
+        if (i != kFarSlots) return fail(c, HFMM_ERR_ARG, "operator rank above 64 is not supported");
+    }
+    HFMM_TRY(ensure(c, c->d_slotord, sizeof(int16_t) * so.size()));
+    HFMM_HIP(c, hipMemcpy(c->d_slotord.ptr, so.data(), sizeof(int16_t) * so.size(), hipMemcpyHostToDevice));
+    HFMM_TRY(ensure(c, c->d_slotseg, sizeof(int32_t) * seg.size()));
+    HFMM_HIP(c, hipMemcpy(c->d_slotseg.ptr, seg.data(), sizeof(int32_t) * seg.size(), hipMemcpyHostToDevice));
     return HFMM_OK;
 }
 
 static void free_all(Context* c) {
     DevBuf* bufs[] = {&c->d_Troots, &c->d_child, &c->d_pidx, &c->d_pvec, &c->d_rank, &c->d_opoff, &c->d_ops,
-                      &c->d_slotpos, &c->d_in, &c->d_keys, &c->d_keys2, &c->d_vals, &c->d_vals2, &c->d_hist,
+                      &c->d_slotpos, &c->d_slotord, &c->d_slotseg, &c->d_in, &c->d_keys, &c->d_keys2, &c->d_vals, &c->d_vals2, &c->d_hist,
                       &c->d_scan_part, &c->d_flags, &c->d_scan, &c->d_sorted, &c->d_perm, &c->d_pleaf,
                       &c->d_count, &c->d_cellkey, &c->d_cellkey_tmp, &c->d_childstart, &c->d_leafstart,
                       &c->d_p2p, &c->d_m2l, &c->d_order, &c->d_pcount, &c->d_M, &c->d_L, &c->d_res_sorted,

@@ -48,7 +48,9 @@ struct Context {
     DevBuf d_rank;           // [343] int32 (by canonical index)
     DevBuf d_opoff;          // [343] int64 offset of U_c (doubles) in d_ops
     DevBuf d_ops;            // packed U_c, V_c row-major [nn][kpad]
-    DevBuf d_slotpos;        // [8][189] int16
+    DevBuf d_slotpos;        // [8][189] int16 relative position of each slot
+    DevBuf d_slotord;        // [8][189][2] int16 (slot, position) ordered by rank tiles
+    DevBuf d_slotseg;        // [8][5] int32 start of the 1..4 rank-tile segments (+ end)
 
     // ---- particles
     int64_t n = 0;
