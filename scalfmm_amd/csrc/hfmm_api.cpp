@@ -186,7 +186,7 @@ static void free_all(Context* c) {
                       &c->d_slotpos, &c->d_slotord, &c->d_slotseg, &c->d_in, &c->d_keys, &c->d_keys2, &c->d_vals, &c->d_vals2, &c->d_hist,
                       &c->d_scan_part, &c->d_flags, &c->d_scan, &c->d_sorted, &c->d_perm, &c->d_pleaf,
                       &c->d_count, &c->d_cellkey, &c->d_cellkey_tmp, &c->d_childstart, &c->d_leafstart,
-                      &c->d_p2p, &c->d_m2l, &c->d_order, &c->d_pcount, &c->d_M, &c->d_L, &c->d_res_sorted,
+                      &c->d_p2p, &c->d_m2l, &c->d_order, &c->d_pcount, &c->d_dense, &c->d_M, &c->d_L, &c->d_res_sorted,
                       &c->d_res};
     for (DevBuf* b : bufs) release(*b);
 }

@@ -84,6 +84,7 @@ struct Context {
     int64_t m2l_off[kMaxLevels] = {0};        // offset (in cells) of a level in d_m2l
     DevBuf d_order;                           // int32 parity-sorted cell index per level (same offsets as d_m2l)
     DevBuf d_pcount;                          // int32 [levels][8]
+    DevBuf d_dense;                           // int32 Morton -> cell index tables (levels <= 8)
 
     // ---- expansions and results
     DevBuf d_M, d_L;      // [total_cells][LD]

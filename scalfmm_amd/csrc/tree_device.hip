@@ -51,14 +51,28 @@ __device__ __forceinline__ void demorton3(uint64_t k, int& x, int& y, int& z) {
     x = compact3(k >> 2);
 }
 
-// Lower-bound search of a key in a sorted array; -1 when absent.
-__device__ __forceinline__ int find_key(const uint64_t* __restrict__ keys, int n, uint64_t key) {
-    int lo = 0, hi = n;
-    while (lo < hi) {
-        const int mid = (lo + hi) >> 1;
-        if (keys[mid] < key) lo = mid + 1; else hi = mid;
+// Index of a key among the sorted keys of one level (-1 when absent): a
+// dense Morton -> index table when the level is small enough, otherwise a
+// lower-bound search.
+struct KeyIndex {
+    const uint64_t* keys;
+    int n;
+    const int32_t* dense;  // 8^level entries or null
+    __device__ __forceinline__ int find(uint64_t key) const {
+        if (dense) return dense[key];
+        int lo = 0, hi = n;
+        while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            if (keys[mid] < key) lo = mid + 1; else hi = mid;
+        }
+        return (lo < n && keys[lo] == key) ? lo : -1;
     }
-    return (lo < n && keys[lo] == key) ? lo : -1;
+};
+
+__global__ __launch_bounds__(256) void k_dense_index(int n, const uint64_t* __restrict__ keys,
+                                                    int32_t* __restrict__ dense) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dense[keys[i]] = i;
 }
 
 // ---------------------------------------------------------------------------
@@ -315,12 +329,12 @@ __global__ void k_set_int(int32_t* p, int32_t v) { *p = v; }
 
 // P2P neighbour slots of every leaf: 26 slots in ascending position order,
 // slot -> leaf index or -1 (outside the box or empty).
-__global__ __launch_bounds__(kBlock) void k_p2p_slots(int nleaf, const uint64_t* __restrict__ lkeys, int lim,
-                                                     int32_t* __restrict__ slots) {
+__global__ __launch_bounds__(kBlock) void k_p2p_slots(KeyIndex ix, int lim, int32_t* __restrict__ slots) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int nleaf = ix.n;
     if (i >= nleaf) return;
     int x, y, z;
-    demorton3(lkeys[i], x, y, z);
+    demorton3(ix.keys[i], x, y, z);
     int s = 0;
     for (int dx = -1; dx <= 1; ++dx)
         for (int dy = -1; dy <= 1; ++dy)
@@ -329,7 +343,7 @@ __global__ __launch_bounds__(kBlock) void k_p2p_slots(int nleaf, const uint64_t*
                 const int nx = x + dx, ny = y + dy, nz = z + dz;
                 int f = -1;
                 if (nx >= 0 && nx < lim && ny >= 0 && ny < lim && nz >= 0 && nz < lim)
-                    f = find_key(lkeys, nleaf, morton3(nx, ny, nz));
+                    f = ix.find(morton3(nx, ny, nz));
                 slots[static_cast<int64_t>(i) * kNbrSlots + s] = f;
                 ++s;
             }
@@ -338,12 +352,11 @@ __global__ __launch_bounds__(kBlock) void k_p2p_slots(int nleaf, const uint64_t*
 // M2L slots of every cell of a level: 189 slots in the reference enumeration
 // order (parent neighbour x,y,z from -1 to 1, then cousin 0..7, keeping the
 // non-adjacent ones); slot -> cell index in the level or -1.
-__global__ __launch_bounds__(kBlock) void k_m2l_slots(int ncell, const uint64_t* __restrict__ ckeys, int plim,
-                                                     int32_t* __restrict__ slots) {
+__global__ __launch_bounds__(kBlock) void k_m2l_slots(KeyIndex ix, int plim, int32_t* __restrict__ slots) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= ncell) return;
+    if (i >= ix.n) return;
     int x, y, z;
-    demorton3(ckeys[i], x, y, z);
+    demorton3(ix.keys[i], x, y, z);
     const int px = x >> 1, py = y >> 1, pz = z >> 1;
     const int bx = x & 1, by = y & 1, bz = z & 1;
     int s = 0;
@@ -360,7 +373,7 @@ __global__ __launch_bounds__(kBlock) void k_m2l_slots(int ncell, const uint64_t*
                     const int zd = 2 * dz + (cz & 1) - bz;
                     if (abs(xd) <= 1 && abs(yd) <= 1 && abs(zd) <= 1) continue;
                     slots[static_cast<int64_t>(i) * kFarSlots + s] =
-                        inside ? find_key(ckeys, ncell, (pk << 3) | static_cast<uint64_t>(cz)) : -1;
+                        inside ? ix.find((pk << 3) | static_cast<uint64_t>(cz)) : -1;
                     ++s;
                 }
             }
@@ -495,11 +508,37 @@ int build_tree_device(Context* c) {
         HFMM_HIP(c, hipMemcpyAsync(c->d_cellkey.as<uint64_t>() + c->lvl_off[l], tkeys + static_cast<size_t>(l) * cap,
                                    sizeof(uint64_t) * c->ncell[l], hipMemcpyDeviceToDevice, c->stream));
 
+    // dense Morton -> cell tables for the levels with at most 2^24 cells
+    int64_t dense_total = 0;
+    int64_t dense_off[kMaxLevels];
+    for (int l = 0; l <= L; ++l) {
+        dense_off[l] = -1;
+        if (l >= 2 && 3 * l <= 24) {
+            dense_off[l] = dense_total;
+            dense_total += int64_t(1) << (3 * l);
+        }
+    }
+    HFMM_TRY(ensure(c, c->d_dense, sizeof(int32_t) * static_cast<size_t>(dense_total + 1)));
+    HFMM_HIP(c, hipMemsetAsync(c->d_dense.ptr, 0xFF, sizeof(int32_t) * static_cast<size_t>(dense_total), c->stream));
+    KeyIndex ix[kMaxLevels];
+    for (int l = 2; l <= L; ++l) {
+        ix[l].keys = c->d_cellkey.as<uint64_t>() + c->lvl_off[l];
+        ix[l].n = static_cast<int>(c->ncell[l]);
+        ix[l].dense = dense_off[l] >= 0 ? c->d_dense.as<int32_t>() + dense_off[l] : nullptr;
+        if (ix[l].dense) {
+            prof_begin(c, "dense_index");
+            hipLaunchKernelGGL(k_dense_index, dim3(blocks_for(c->ncell[l])), dim3(kBlock), 0, c->stream, ix[l].n,
+                               ix[l].keys, c->d_dense.as<int32_t>() + dense_off[l]);
+            prof_end(c);
+            HFMM_TRY(check_launch(c, "dense_index"));
+        }
+    }
+
     // 5. P2P slots of the leaves
     HFMM_TRY(ensure(c, c->d_p2p, sizeof(int32_t) * static_cast<size_t>(nl) * kNbrSlots));
     prof_begin(c, "p2p_lists");
-    hipLaunchKernelGGL(k_p2p_slots, dim3(blocks_for(nl)), dim3(kBlock), 0, c->stream, static_cast<int>(nl),
-                       c->d_cellkey.as<uint64_t>() + c->lvl_off[L], 1 << L, c->d_p2p.as<int32_t>());
+    hipLaunchKernelGGL(k_p2p_slots, dim3(blocks_for(nl)), dim3(kBlock), 0, c->stream, ix[L], 1 << L,
+                       c->d_p2p.as<int32_t>());
     prof_end(c);
     HFMM_TRY(check_launch(c, "p2p_lists"));
 
@@ -513,8 +552,8 @@ int build_tree_device(Context* c) {
         c->m2l_off[l] = c->lvl_off[l];
         const uint64_t* ck = c->d_cellkey.as<uint64_t>() + c->lvl_off[l];
         prof_begin(c, "m2l_lists");
-        hipLaunchKernelGGL(k_m2l_slots, dim3(blocks_for(nc)), dim3(kBlock), 0, c->stream, static_cast<int>(nc), ck,
-                           1 << (l - 1), c->d_m2l.as<int32_t>() + static_cast<size_t>(c->m2l_off[l]) * kFarSlots);
+        hipLaunchKernelGGL(k_m2l_slots, dim3(blocks_for(nc)), dim3(kBlock), 0, c->stream, ix[l], 1 << (l - 1),
+                           c->d_m2l.as<int32_t>() + static_cast<size_t>(c->m2l_off[l]) * kFarSlots);
         prof_end(c);
         HFMM_TRY(check_launch(c, "m2l_lists"));
         // parity order: stable sort of (key & 7) carrying the cell index
