@@ -12,39 +12,13 @@
 //        (Src/Kernels/P2P/FP2P.hpp:103-269 conventions)
 #include <hip/hip_runtime.h>
 
+#include "fmm_common.hpp"
 #include "hfmm_internal.hpp"
 
 namespace hfmm {
 namespace {
 
 typedef double d4 __attribute__((ext_vector_type(4)));
-
-__device__ __forceinline__ uint32_t compact3(uint64_t x) {
-    x &= 0x1249249249249249ull;
-    x = (x ^ (x >> 2)) & 0x10c30c30c30c30c3ull;
-    x = (x ^ (x >> 4)) & 0x100f00f00f00f00full;
-    x = (x ^ (x >> 8)) & 0x1f0000ff0000ffull;
-    x = (x ^ (x >> 16)) & 0x1f00000000ffffull;
-    x = (x ^ (x >> 32)) & 0x1fffffull;
-    return static_cast<uint32_t>(x);
-}
-
-struct Box {
-    double corner[3];
-    double leafw;
-};
-
-// Leaf centre (getLeafCellCenter, FAbstractChebKernel.hpp:53-58) and the
-// affine map bounds a = c - w/2, b = c + w/2 (FInterpMapping.hpp:30-37).
-__device__ __forceinline__ void leaf_bounds(const Box& bx, uint64_t key, double a[3], double b[3]) {
-    const int co[3] = {static_cast<int>(compact3(key >> 2)), static_cast<int>(compact3(key >> 1)),
-                       static_cast<int>(compact3(key))};
-    for (int d = 0; d < 3; ++d) {
-        const double c = bx.corner[d] + (static_cast<double>(co[d]) + .5) * bx.leafw;
-        a[d] = c - bx.leafw / 2.;
-        b[d] = c + bx.leafw / 2.;
-    }
-}
 
 // S(xi, root_j) = 1/p + 2/p sum_{o>=1} T_o(xi) T_o(root_j); dS = d/dxi S.
 template <int P, bool GRAD>
@@ -366,18 +340,10 @@ __global__ __launch_bounds__(256) void k_p2p_f64(int64_t n, const int32_t* __res
 // MFMA 16x16x4 f64 operand layout (checked by hfmm_selftest_mfma_f64):
 // A[i][k] in lane i + 16k, B[k][j] in lane j + 16k, and register r of lane l
 // holds D[4r + l/16][l%16].
-constexpr int kM2LWaves = 4;
 constexpr int kM2LCols = 2;       // 16-wide column tiles per workgroup
 constexpr int kM2LTile = 16 * kM2LCols;
 constexpr int kMaxRankTiles = 4;  // kpad <= 64
 
-struct M2LLevels {
-    int nlev;
-    int tile_begin[kMaxLevels + 1];  // prefix of (upper bounds of) tiles per level
-    int64_t cell_off[kMaxLevels];    // first cell of the level in the cell arrays
-    int pcount_off[kMaxLevels];      // parity counts of the level
-    double scale[kMaxLevels];
-};
 
 // Per-workgroup state of an M2L tile.
 template <int P>
@@ -646,6 +612,30 @@ Box make_box(const Context* c) {
     return b;
 }
 
+}  // namespace
+
+M2LLevels make_levels(const Context* c, int tile_targets, int& tiles) {
+    const int L = c->height() - 1;
+    M2LLevels lv{};
+    lv.nlev = 0;
+    tiles = 0;
+    lv.tile_begin[0] = 0;
+    for (int l = 2; l <= L; ++l) {
+        const int i = lv.nlev++;
+        lv.cell_off[i] = c->lvl_off[l];
+        lv.pcount_off[i] = 8 * l;
+        // 2^l cells per axis: scale factor 2 / (width / 2^l) (FInterpMatrixKernel.hpp:124-133)
+        lv.scale[i] = 2. / (c->P.box_width / static_cast<double>(int64_t(1) << l));
+        // segment of 8 x (tiles of the largest parity); the parity counts are
+        // on the device, ceil(ncell/tile) bounds every parity's tile count
+        tiles += 8 * static_cast<int>((c->ncell[l] + tile_targets - 1) / tile_targets);
+        lv.tile_begin[i + 1] = tiles;
+    }
+    return lv;
+}
+
+namespace {
+
 template <int P>
 int p2m_impl(Context* c) {
     const int L = c->height() - 1;
@@ -699,22 +689,8 @@ int l2p_impl(Context* c) {
 
 template <int P>
 int m2l_impl(Context* c) {
-    const int L = c->height() - 1;
-    M2LLevels lv{};
-    lv.nlev = 0;
     int tiles = 0;
-    lv.tile_begin[0] = 0;
-    for (int l = 2; l <= L; ++l) {
-        const int i = lv.nlev++;
-        lv.cell_off[i] = c->lvl_off[l];
-        lv.pcount_off[i] = 8 * l;
-        // 2^l cells per axis: scale factor 2 / (width / 2^l) (FInterpMatrixKernel.hpp:124-133)
-        lv.scale[i] = 2. / (c->P.box_width / static_cast<double>(int64_t(1) << l));
-        // segment of 8 x (tiles of the largest parity); the parity counts are
-        // on the device, ceil(ncell/32) bounds every parity's tile count
-        tiles += 8 * static_cast<int>((c->ncell[l] + kM2LTile - 1) / kM2LTile);
-        lv.tile_begin[i + 1] = tiles;
-    }
+    const M2LLevels lv = make_levels(c, kM2LTile, tiles);
     const unsigned grid = static_cast<unsigned>(tiles);
     const size_t shmem =
         sizeof(double) * kM2LWaves * static_cast<size_t>(c->kpad_max) * kM2LTile + sizeof(uint16_t) * 2 * c->LD;
