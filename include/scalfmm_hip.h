@@ -52,6 +52,12 @@ extern "C" {
 
 /* ---- precision modes ------------------------------------------------------ */
 #define HFMM_PRECISION_FP64 0 /* every kernel in FP64 (the reference's FReal=double) */
+#define HFMM_PRECISION_MIXED 1 /* M2L on bfloat16 matrix cores with three-part operand
+                                  splits and FP32 accumulation, P2P in FP32 with
+                                  compensated sums; P2M/M2M/L2L/L2P and all results in
+                                  FP64.  Relative error about 1e-7 on top of the
+                                  FP64 path's; no counterpart in the reference (its
+                                  FReal=float instantiation is the closest). */
 
 typedef struct hfmm_ctx hfmm_ctx;
 
@@ -162,6 +168,11 @@ int hfmm_reset_kernel_times(hfmm_ctx* ctx);
  * layout the M2L kernel assumes; no counterpart in the reference (a check of
  * the CDNA4 matrix-core lane layout). */
 int hfmm_selftest_mfma_f64(hfmm_ctx* ctx, double* out);
+/* Runs one v_mfma_f32_16x16x32_bf16 on A[i][k] = (7i + 3k) mod 11 - 5,
+ * B[k][j] = (5k + 2j) mod 13 - 6 (exact in bfloat16 and FP32) and writes
+ * D = A B (16 x 16, row-major) through the operand layout the mixed-precision
+ * M2L kernel assumes; no counterpart in the reference. */
+int hfmm_selftest_mfma_bf16(hfmm_ctx* ctx, float* out);
 
 /* Library version string. */
 const char* hfmm_version(void);

@@ -39,6 +39,16 @@ __device__ __forceinline__ void leaf_bounds(const Box& bx, uint64_t key, double 
     }
 }
 
+// Workgroups of the persistent big-leaf P2P kernels (one wave each): the
+// item count is known only on the device, the capacity bounds it.
+inline unsigned p2p_tiled_grid(const Context* c) {
+    const int64_t g = c->p2p_items_cap < 4096 ? c->p2p_items_cap : 4096;
+    return static_cast<unsigned>(g > 0 ? g : 1);
+}
+
+// The box of the context with its leaf width (level height-1).
+Box make_box(const Context* c);
+
 struct M2LLevels {
     int nlev;
     int tile_begin[kMaxLevels + 1];  // prefix of (upper bounds of) tiles per level

@@ -295,6 +295,7 @@ __global__ __launch_bounds__(256) void k_p2p_f64(int64_t n, const int32_t* __res
     const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const int li = pleaf[i];
+    if (lstart[li + 1] - lstart[li] > kP2PBig) return;  // k_p2p_f64_tiled
     const double tx = sp[i], ty = sp[n + i], tz = sp[2 * n + i], tq = sp[3 * n + i];
     double pot = 0., fx = 0., fy = 0., fz = 0.;
     for (int s = -1; s < kNbrSlots; ++s) {
@@ -317,6 +318,72 @@ __global__ __launch_bounds__(256) void k_p2p_f64(int64_t n, const int32_t* __res
     res[n + i] = fx * tq;
     res[2 * n + i] = fy * tq;
     res[3 * n + i] = fz * tq;
+}
+
+// P2P of the big leaves (more than kP2PBig particles): one wave per work item
+// of up to kP2PItem consecutive targets of one leaf, two per lane; the sources
+// of each neighbour leaf pass through LDS 64 at a time and every lane reads
+// them as broadcasts.  Same pair arithmetic and summation order as k_p2p_f64.
+__global__ __launch_bounds__(64) void k_p2p_f64_tiled(const int2* __restrict__ items, const int32_t* __restrict__ nitems,
+                                                     const int32_t* __restrict__ lstart,
+                                                     const int32_t* __restrict__ slots, const double* __restrict__ sp,
+                                                     int64_t n, double* __restrict__ res) {
+    __shared__ double4 src[64];
+    const int lane = threadIdx.x;
+    const int total = *nitems;
+    for (int it = blockIdx.x; it < total; it += gridDim.x) {
+        const int2 item = items[it];
+        const int leaf = item.x;
+        const int tend = min(lstart[leaf + 1], item.y + kP2PItem);
+        int ti[2];
+        bool ok[2];
+        double t[2][3], acc[2][4];
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {
+            ti[u] = item.y + 64 * u + lane;
+            ok[u] = ti[u] < tend;
+            const int tt = ok[u] ? ti[u] : item.y;
+            for (int d = 0; d < 3; ++d) t[u][d] = sp[d * n + tt];
+            for (int k = 0; k < 4; ++k) acc[u][k] = 0.;
+        }
+        for (int s = -1; s < kNbrSlots; ++s) {
+            const int nb = s < 0 ? leaf : slots[static_cast<int64_t>(leaf) * kNbrSlots + s];
+            if (nb < 0) continue;
+            const int j0 = lstart[nb], j1 = lstart[nb + 1];
+            for (int jc = j0; jc < j1; jc += 64) {
+                __syncthreads();  // the previous chunk is consumed
+                if (jc + lane < j1) {
+                    const int j = jc + lane;
+                    src[lane] = make_double4(sp[j], sp[n + j], sp[2 * n + j], sp[3 * n + j]);
+                }
+                __syncthreads();
+                const int m = min(64, j1 - jc);
+                for (int jj = 0; jj < m; ++jj) {
+                    const double4 sj = src[jj];
+#pragma unroll
+                    for (int u = 0; u < 2; ++u) {
+                        if (jc + jj == ti[u]) continue;
+                        const double dx = t[u][0] - sj.x, dy = t[u][1] - sj.y, dz = t[u][2] - sj.z;
+                        const double inv = 1.0 / sqrt(dx * dx + dy * dy + dz * dz);
+                        const double inv3 = inv * inv * inv;
+                        acc[u][0] += inv * sj.w;
+                        acc[u][1] += -inv3 * dx * sj.w;
+                        acc[u][2] += -inv3 * dy * sj.w;
+                        acc[u][3] += -inv3 * dz * sj.w;
+                    }
+                }
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {
+            if (!ok[u]) continue;
+            const double tq = sp[3 * n + ti[u]];
+            res[ti[u]] = acc[u][0];
+            res[n + ti[u]] = acc[u][1] * tq;
+            res[2 * n + ti[u]] = acc[u][2] * tq;
+            res[3 * n + ti[u]] = acc[u][3] * tq;
+        }
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -604,6 +671,8 @@ __global__ __launch_bounds__(256) void k_scatter(int64_t n, const int32_t* __res
 
 inline unsigned blocks256(int64_t n) { return static_cast<unsigned>((n + 255) / 256); }
 
+}  // namespace
+
 Box make_box(const Context* c) {
     Box b;
     const double half = c->P.box_width / 2;
@@ -611,8 +680,6 @@ Box make_box(const Context* c) {
     b.leafw = c->P.box_width / static_cast<double>(1 << (c->height() - 1));
     return b;
 }
-
-}  // namespace
 
 M2LLevels make_levels(const Context* c, int tile_targets, int& tiles) {
     const int L = c->height() - 1;
@@ -752,7 +819,13 @@ int run_p2p(Context* c) {
                        c->d_leafstart.as<int32_t>(), c->d_p2p.as<int32_t>(), c->d_sorted.as<double>(),
                        c->d_res_sorted.as<double>());
     prof_end(c);
-    return check_launch(c, "p2p");
+    HFMM_TRY(check_launch(c, "p2p"));
+    prof_begin(c, "p2p_tiled");
+    hipLaunchKernelGGL(k_p2p_f64_tiled, dim3(p2p_tiled_grid(c)), dim3(64), 0, c->stream, c->d_p2pitems.as<int2>(),
+                       p2p_item_count(c), c->d_leafstart.as<int32_t>(), c->d_p2p.as<int32_t>(),
+                       c->d_sorted.as<double>(), c->n, c->d_res_sorted.as<double>());
+    prof_end(c);
+    return check_launch(c, "p2p_tiled");
 }
 
 int zero_results(Context* c) {

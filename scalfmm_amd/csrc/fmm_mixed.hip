@@ -1,0 +1,705 @@
+// fmm_mixed.hip -- the mixed-precision passes of the Chebyshev 1/r FMM
+// (HFMM_PRECISION_MIXED).
+//
+// M2L: the tile scheme of the FP64 kernel, with every operand split into
+// three bfloat16 parts (x = hi + mid + lo, 24 significant bits) and every
+// product run on v_mfma_f32_16x16x32_bf16 as the six leading partial
+// products (hi*hi, hi*mid, mid*hi, hi*lo, lo*hi, mid*mid) accumulated in
+// FP32.  The node permutations of the symmetric scheme are folded into
+// position-specific operators (one per far-field position, built on the
+// device from the canonical FP64 ones), so both GEMMs stream their A operand
+// contiguously.  The M2L output side works in the Chebyshev modal basis
+// (coefficients of T_a(x) T_b(y) T_c(z) instead of values at the nodes): the
+// FP32 accumulation and the 24-bit operators then err relative to each mode,
+// not relative to the cell's mean potential, so the gradient modes that make
+// the forces keep their accuracy.  A small FP64 kernel turns the modal locals
+// back into node values.  P2M, M2M, L2L and L2P stay FP64.
+// P2P: FP32 pair terms on leaf-local coordinates split in exact hi and small
+// lo parts (see k_local_coords), 1/r by rsqrt, per-target sums compensated
+// (Kahan) once per block of 16 pair terms, results written in FP64.
+//
+// Reference operators: Src/Kernels/Chebyshev/FChebSymKernel.hpp:210-328 (M2L),
+// Src/Kernels/P2P/FP2P.hpp:103-269 (P2P).
+#include <hip/hip_runtime.h>
+
+#include <vector>
+
+#include "fmm_common.hpp"
+#include "hfmm_internal.hpp"
+
+namespace hfmm {
+namespace {
+
+typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
+typedef float f4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ bf8 ld8(const uint16_t* p) { return *reinterpret_cast<const bf8*>(p); }
+
+__device__ __forceinline__ f4 mfma_bf16(bf8 a, bf8 b, f4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+}
+
+// The six leading partial products of (ah + am + al) (bh + bm + bl), smallest first.
+__device__ __forceinline__ f4 mma6(bf8 ah, bf8 am, bf8 al, bf8 bh, bf8 bm, bf8 bl, f4 c) {
+    c = mfma_bf16(ah, bl, c);
+    c = mfma_bf16(al, bh, c);
+    c = mfma_bf16(am, bm, c);
+    c = mfma_bf16(ah, bm, c);
+    c = mfma_bf16(am, bh, c);
+    return mfma_bf16(ah, bh, c);
+}
+
+__device__ __forceinline__ uint16_t bf16_bits(__bf16 v) { return __builtin_bit_cast(uint16_t, v); }
+
+// x = hi + mid + lo, each part rounded to nearest bfloat16.
+__device__ __forceinline__ void split3(float x, __bf16& hi, __bf16& mid, __bf16& lo) {
+    hi = static_cast<__bf16>(x);
+    const float r1 = x - static_cast<float>(hi);
+    mid = static_cast<__bf16>(r1);
+    lo = static_cast<__bf16>(r1 - static_cast<float>(mid));
+}
+
+__device__ __forceinline__ void split3d(double x, uint16_t& h, uint16_t& m, uint16_t& l) {
+    const __bf16 hi = static_cast<__bf16>(static_cast<float>(x));
+    const double r1 = x - static_cast<double>(static_cast<float>(hi));
+    const __bf16 mid = static_cast<__bf16>(static_cast<float>(r1));
+    const double r2 = r1 - static_cast<double>(static_cast<float>(mid));
+    h = bf16_bits(hi);
+    m = bf16_bits(mid);
+    l = bf16_bits(static_cast<__bf16>(static_cast<float>(r2)));
+}
+
+// Position-specific operators from the canonical ones (one workgroup per
+// far-field position), planes hi/mid/lo:
+//   VT_pos[k][n] = V_c[pv[n]][k]                         as [3][kp16][LD]
+//   U_pos[m][k]  = sum_n W3[m][n] U_c[pv[n]][k]          as [3][LD][kp32]
+// with W3 = W (x) W (x) W the nodal -> modal transform of one dimension,
+// W[a][j] = (a ? 2 : 1) / p T_a(x_j) (discrete orthogonality of T at the
+// roots), node / mode index n = n1 + p n2 + p^2 n3.  The padded permutation
+// entries point at the zero row nn of U_c / V_c.
+__global__ __launch_bounds__(256) void k_pack_positions(const int16_t* __restrict__ plist, const int32_t* __restrict__ pkt,
+                                                       const int64_t* __restrict__ posoff, const uint32_t* __restrict__ pidx,
+                                                       const uint16_t* __restrict__ pvec, const int64_t* __restrict__ opoff,
+                                                       const double* __restrict__ ops, const double* __restrict__ Troots,
+                                                       int p, int LD, uint16_t* __restrict__ opm) {
+    __shared__ double sW[16 * 16];
+    const int nn = p * p * p;
+    for (int i = threadIdx.x; i < p * p; i += blockDim.x) sW[i] = (i >= p ? 2. : 1.) / p * Troots[i];
+    __syncthreads();
+    const int pos = plist[blockIdx.x];
+    const int kt = pkt[blockIdx.x];
+    const int kp16 = 16 * kt, kp32 = 32 * ((kt + 1) / 2);
+    const double* Uc = ops + opoff[pidx[pos]];
+    const double* Vc = Uc + static_cast<int64_t>(nn + 1) * kp16;
+    const uint16_t* pv = pvec + static_cast<int64_t>(pos) * LD;
+    uint16_t* vt = opm + posoff[pos];
+    const int64_t nv = static_cast<int64_t>(kp16) * LD;
+    for (int64_t e = threadIdx.x; e < nv; e += blockDim.x) {
+        const int k = static_cast<int>(e / LD), n = static_cast<int>(e % LD);
+        split3d(Vc[static_cast<int64_t>(pv[n]) * kp16 + k], vt[e], vt[nv + e], vt[2 * nv + e]);
+    }
+    uint16_t* u = vt + 3 * nv;
+    const int64_t nu = static_cast<int64_t>(LD) * kp32;
+    for (int64_t e = threadIdx.x; e < nu; e += blockDim.x) {
+        const int m = static_cast<int>(e / kp32), k = static_cast<int>(e % kp32);
+        double v = 0.;
+        if (m < nn && k < kp16) {
+            const int m1 = m % p, m2 = (m / p) % p, m3 = m / (p * p);
+            for (int n3 = 0; n3 < p; ++n3)
+                for (int n2 = 0; n2 < p; ++n2) {
+                    const double w23 = sW[m3 * p + n3] * sW[m2 * p + n2];
+                    const int nb = (n3 * p + n2) * p;
+                    double t = 0.;
+                    for (int n1 = 0; n1 < p; ++n1) t += sW[m1 * p + n1] * Uc[static_cast<int64_t>(pv[nb + n1]) * kp16 + k];
+                    v += w23 * t;
+                }
+        }
+        split3d(v, u[e], u[nu + e], u[2 * nu + e]);
+    }
+}
+
+// Modal -> nodal locals in place, one wave per cell, FP64:
+// f[n] = sum_m T_m1(x_n1) T_m2(x_n2) T_m3(x_n3) a[m], three one-dimensional passes.
+template <int P>
+__global__ __launch_bounds__(256) void k_modal_to_nodal(int64_t ncells, const double* __restrict__ Troots,
+                                                       double* __restrict__ L, int LD) {
+    constexpr int NN = P * P * P;
+    __shared__ double sT[P * P];
+    __shared__ double buf[4][2][NN];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    for (int i = threadIdx.x; i < P * P; i += 256) sT[i] = Troots[i];
+    const int64_t cell = static_cast<int64_t>(blockIdx.x) * 4 + wave;
+    const bool live = cell < ncells;
+    double* Lc = L + (live ? cell : 0) * LD;
+    double* a = buf[wave][0];
+    double* b = buf[wave][1];
+    for (int i = lane; i < NN; i += 64) a[i] = live ? Lc[i] : 0.;
+    __syncthreads();
+    int stride = 1;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        for (int i = lane; i < NN; i += 64) {
+            const int nd = (i / stride) % P;
+            const int base = i - nd * stride;
+            double s = 0.;
+#pragma unroll
+            for (int m = 0; m < P; ++m) s += sT[m * P + nd] * a[base + m * stride];
+            b[i] = s;
+        }
+        __syncthreads();
+        double* t = a;
+        a = b;
+        b = t;
+        stride *= P;
+    }
+    if (live)
+        for (int i = lane; i < NN; i += 64) Lc[i] = a[i];
+}
+
+// Multipoles (FP64, [cells + 1][LD]) -> three bfloat16 planes of the same shape.
+__global__ __launch_bounds__(256) void k_split_multipoles(int64_t nvals, const double* __restrict__ M,
+                                                         uint16_t* __restrict__ Mb) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= nvals) return;
+    split3d(M[i], Mb[i], Mb[nvals + i], Mb[2 * nvals + i]);
+}
+
+// ---------------------------------------------------------------------------
+// M2L.  A workgroup of 4 waves owns COLS column tiles of 16 targets of one
+// parity at one level and walks the 189 slots of that parity, ordered by the
+// rank tiles of their operators.  Per slot:
+//   GEMM1  C (kp16 x 16 COLS) = VT_pos * M_src, the node dimension split in
+//          four over the waves, partial C tiles through LDS;
+//   GEMM2  acc (nn x 16 COLS) += U_pos * C, row tiles spread over the waves,
+//          B = the sum of the partial C tiles, split again into three parts.
+// MFMA 16x16x32 bf16 layout (checked by hfmm_selftest_mfma_bf16): A[i][k] in
+// lane i + 16 (k / 8), element k % 8; B[k][j] in lane j + 16 (k / 8),
+// element k % 8; register r of lane l holds D[4 (l / 16) + r][l % 16].
+template <int P, int COLS>
+struct MixState {
+    static constexpr int NN = P * P * P;
+    static constexpr int NT = (NN + 15) / 16;
+    static constexpr int NTW = (NT + kM2LWaves - 1) / kM2LWaves;
+    f4 acc[NTW][COLS];
+};
+
+template <int P, int COLS, int KT>
+__device__ __forceinline__ void mix_segment(MixState<P, COLS>& st, int i0, int i1, const int16_t* s_slot,
+                                            const int64_t* s_off, const int32_t* lslots, const int (&tgt)[COLS],
+                                            int (&nsrc)[COLS], const uint16_t* __restrict__ Mbl,
+                                            const uint16_t* __restrict__ Mbz, int64_t plane,
+                                            const uint16_t* __restrict__ opm, float* Cp, int LD, int crows, int wave,
+                                            int col, int kq, int kb0, int kb1) {
+    constexpr int W = kM2LWaves;
+    constexpr int NTW = MixState<P, COLS>::NTW;
+    constexpr int TC = 16 * COLS;
+    constexpr int KP16 = 16 * KT;       // GEMM1 rows: rank padded to 16
+    constexpr int KB32 = (KT + 1) / 2;  // GEMM2 K blocks: rank padded to 32
+    constexpr int KP32 = 32 * KB32;
+    for (int i = i0; i < i1; ++i) {
+        const uint16_t* Mrow[COLS];
+        bool any = false;
+#pragma unroll
+        for (int ct = 0; ct < COLS; ++ct) {
+            const int s = nsrc[ct];
+            any |= s >= 0;
+            // a missing source reads the zero multipole (no divergent branches)
+            Mrow[ct] = (s >= 0 ? Mbl + static_cast<int64_t>(s) * LD : Mbz) + 8 * kq;
+            nsrc[ct] = (tgt[ct] >= 0 && i + 1 < kFarSlots)
+                           ? lslots[static_cast<int64_t>(tgt[ct]) * kFarSlots + s_slot[i + 1]]
+                           : -1;
+        }
+        if (__ballot(any) == 0) continue;  // same answer in every wave
+        const uint16_t* VT = opm + s_off[i];
+        const uint16_t* U = VT + 3 * static_cast<int64_t>(KP16) * LD;
+        __syncthreads();  // every wave is past the previous slot's GEMM2 (C is reused)
+        {
+            f4 c1[KT][COLS];
+#pragma unroll
+            for (int rt = 0; rt < KT; ++rt)
+#pragma unroll
+                for (int ct = 0; ct < COLS; ++ct) c1[rt][ct] = f4{0.f, 0.f, 0.f, 0.f};
+            for (int kb = kb0; kb < kb1; ++kb) {
+                bf8 bh[COLS], bm[COLS], bl[COLS];
+#pragma unroll
+                for (int ct = 0; ct < COLS; ++ct) {
+                    const uint16_t* p = Mrow[ct] + 32 * kb;
+                    bh[ct] = ld8(p);
+                    bm[ct] = ld8(p + plane);
+                    bl[ct] = ld8(p + 2 * plane);
+                }
+#pragma unroll
+                for (int rt = 0; rt < KT; ++rt) {
+                    const uint16_t* a = VT + static_cast<int64_t>(16 * rt + col) * LD + 32 * kb + 8 * kq;
+                    const bf8 ah = ld8(a), am = ld8(a + KP16 * LD), al = ld8(a + 2 * KP16 * LD);
+#pragma unroll
+                    for (int ct = 0; ct < COLS; ++ct)
+                        c1[rt][ct] = mma6(ah, am, al, bh[ct], bm[ct], bl[ct], c1[rt][ct]);
+                }
+            }
+            float* Cw = Cp + static_cast<size_t>(wave) * crows * TC;
+#pragma unroll
+            for (int rt = 0; rt < KT; ++rt)
+#pragma unroll
+                for (int ct = 0; ct < COLS; ++ct)
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) Cw[(16 * rt + 4 * kq + r) * TC + 16 * ct + col] = c1[rt][ct][r];
+        }
+        __syncthreads();
+        // Rows 16 KT .. KP32 of C are zero: the segments run with growing KT,
+        // so no earlier slot wrote them, and the LDS was cleared at the start.
+#pragma unroll
+        for (int kb = 0; kb < KB32; ++kb) {
+            bf8 bh[COLS], bm[COLS], bl[COLS];
+#pragma unroll
+            for (int ct = 0; ct < COLS; ++ct)
+#pragma unroll
+                for (int j = 0; j < 8; ++j) {
+                    const int idx = (32 * kb + 8 * kq + j) * TC + 16 * ct + col;
+                    float v = Cp[idx];
+#pragma unroll
+                    for (int w = 1; w < W; ++w) v += Cp[static_cast<size_t>(w) * crows * TC + idx];
+                    __bf16 h, m, l;
+                    split3(v, h, m, l);
+                    bh[ct][j] = h;
+                    bm[ct][j] = m;
+                    bl[ct][j] = l;
+                }
+#pragma unroll
+            for (int t = 0; t < NTW; ++t) {
+                // a wave with fewer real row tiles computes throw-away ones on row 0
+                const int row = 16 * (wave + W * t) + col;
+                const uint16_t* a = U + static_cast<int64_t>(row < LD ? row : 0) * KP32 + 32 * kb + 8 * kq;
+                const bf8 ah = ld8(a), am = ld8(a + static_cast<int64_t>(LD) * KP32),
+                          al = ld8(a + 2 * static_cast<int64_t>(LD) * KP32);
+#pragma unroll
+                for (int ct = 0; ct < COLS; ++ct) st.acc[t][ct] = mma6(ah, am, al, bh[ct], bm[ct], bl[ct], st.acc[t][ct]);
+            }
+        }
+    }
+}
+
+template <int P, int COLS>
+__global__ __launch_bounds__(256) void k_m2l_mixed(M2LLevels lv, const int32_t* __restrict__ slots,
+                                                  const int32_t* __restrict__ order, const int32_t* __restrict__ pcount,
+                                                  const int16_t* __restrict__ slotord, const int32_t* __restrict__ slotseg,
+                                                  const int64_t* __restrict__ posoff, const uint16_t* __restrict__ opm,
+                                                  const uint16_t* __restrict__ Mb, int64_t plane, int64_t zero_cell,
+                                                  double* __restrict__ L, int LD, int crows) {
+    constexpr int NN = P * P * P;
+    constexpr int W = kM2LWaves;
+    constexpr int NT = MixState<P, COLS>::NT;
+    constexpr int NTW = MixState<P, COLS>::NTW;
+    constexpr int TC = 16 * COLS;
+    __shared__ int16_t s_slot[kFarSlots];
+    __shared__ int64_t s_off[kFarSlots];
+    extern __shared__ float fsm[];  // partial C tiles [W][crows][TC]
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int col = lane & 15, kq = lane >> 4;
+    const int KB = LD / 32;
+    const int kb0 = (KB * wave) / W, kb1 = (KB * (wave + 1)) / W;
+    const int ntw = (NT - wave + W - 1) / W;
+
+    const int bid = blockIdx.x;
+    if (bid >= lv.tile_begin[lv.nlev]) return;
+    int li = 0;
+    while (bid >= lv.tile_begin[li + 1]) ++li;
+    const int lb = bid - lv.tile_begin[li];
+    const int e = lb % 8;  // parity: consecutive workgroups land on different XCDs
+    const int ptile = lb / 8;
+    const int32_t* pc = pcount + lv.pcount_off[li];
+    int first = 0;
+    for (int q = 0; q < e; ++q) first += pc[q];
+    const int cnt = min(TC, pc[e] - ptile * TC);
+    if (cnt <= 0) return;
+    first += ptile * TC;
+    for (int i = tid; i < kFarSlots; i += 64 * W) {
+        s_slot[i] = slotord[(e * kFarSlots + i) * 2];
+        s_off[i] = posoff[slotord[(e * kFarSlots + i) * 2 + 1]];
+    }
+    for (int i = tid; i < W * crows * TC; i += 64 * W) fsm[i] = 0.f;
+    int seg[5];
+#pragma unroll
+    for (int q = 0; q < 5; ++q) seg[q] = slotseg[e * 5 + q];
+    const int64_t coff = lv.cell_off[li];
+    const int32_t* lslots = slots + coff * kFarSlots;
+    const uint16_t* Mbl = Mb + coff * LD;
+    const uint16_t* Mbz = Mb + zero_cell * LD;
+    double* Ll = L + coff * LD;
+    int tgt[COLS];
+#pragma unroll
+    for (int ct = 0; ct < COLS; ++ct) {
+        const int cc = 16 * ct + col;
+        tgt[ct] = cc < cnt ? order[coff + first + cc] : -1;
+    }
+    __syncthreads();
+    int nsrc[COLS];
+#pragma unroll
+    for (int ct = 0; ct < COLS; ++ct)
+        nsrc[ct] = tgt[ct] >= 0 ? lslots[static_cast<int64_t>(tgt[ct]) * kFarSlots + s_slot[0]] : -1;
+    MixState<P, COLS> st;
+#pragma unroll
+    for (int t = 0; t < NTW; ++t)
+#pragma unroll
+        for (int ct = 0; ct < COLS; ++ct) st.acc[t][ct] = f4{0.f, 0.f, 0.f, 0.f};
+    mix_segment<P, COLS, 1>(st, seg[0], seg[1], s_slot, s_off, lslots, tgt, nsrc, Mbl, Mbz, plane, opm, fsm, LD,
+                            crows, wave, col, kq, kb0, kb1);
+    mix_segment<P, COLS, 2>(st, seg[1], seg[2], s_slot, s_off, lslots, tgt, nsrc, Mbl, Mbz, plane, opm, fsm, LD,
+                            crows, wave, col, kq, kb0, kb1);
+    mix_segment<P, COLS, 3>(st, seg[2], seg[3], s_slot, s_off, lslots, tgt, nsrc, Mbl, Mbz, plane, opm, fsm, LD,
+                            crows, wave, col, kq, kb0, kb1);
+    mix_segment<P, COLS, 4>(st, seg[3], seg[4], s_slot, s_off, lslots, tgt, nsrc, Mbl, Mbz, plane, opm, fsm, LD,
+                            crows, wave, col, kq, kb0, kb1);
+    const double scale = lv.scale[li];
+#pragma unroll
+    for (int ct = 0; ct < COLS; ++ct) {
+        if (tgt[ct] < 0) continue;
+#pragma unroll
+        for (int t = 0; t < NTW; ++t) {
+            if (t < ntw) {  // real row tiles of this wave
+                const int nt = wave + W * t;
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int row = 16 * nt + 4 * kq + r;
+                    if (row < NN)
+                        Ll[static_cast<int64_t>(tgt[ct]) * LD + row] = scale * static_cast<double>(st.acc[t][ct][r]);
+                }
+            }
+        }
+    }
+}
+
+// Leaf-local coordinates of the sorted particles in leaf widths, as float
+// pairs: hi = the offset from the leaf centre rounded to a multiple of 2^-23,
+// lo = the remainder.  Differences of hi parts across neighbouring leaves
+// (centres one whole unit apart) are then exact in FP32, and a pair's
+// separation dx = (hi_t - c - hi_s) + (lo_t - lo_s) carries one rounding
+// relative to dx itself, however close the pair.  hi.w holds the charge.
+__global__ __launch_bounds__(256) void k_local_coords(int64_t n, const int32_t* __restrict__ pleaf,
+                                                     const uint64_t* __restrict__ lkeys, Box bx,
+                                                     const double* __restrict__ sp, float4* __restrict__ hi,
+                                                     float4* __restrict__ lo) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double a[3], b[3];
+    leaf_bounds(bx, lkeys[pleaf[i]], a, b);
+    float h[3], l[3];
+    for (int d = 0; d < 3; ++d) {
+        const double x = (sp[d * n + i] - (a[d] + b[d]) / 2.) / bx.leafw;
+        const double xh = rint(x * 8388608.) / 8388608.;  // 2^23
+        h[d] = static_cast<float>(xh);
+        l[d] = static_cast<float>(x - xh);
+    }
+    hi[i] = make_float4(h[0], h[1], h[2], static_cast<float>(sp[3 * n + i]));
+    lo[i] = make_float4(l[0], l[1], l[2], 0.f);
+}
+
+// Compensated (Kahan) add of a block partial into (sum, comp).
+__device__ __forceinline__ void kahan_add(float& sum, float& comp, float part) {
+    const float y = part - comp;
+    const float t = sum + y;
+    comp = (t - sum) - y;
+    sum = t;
+}
+
+// One source against one target in leaf units: separation from the exact hi
+// difference plus the lo difference, 1/r by rsqrt.
+__device__ __forceinline__ void pair_f32(const float (&th)[3], const float (&tl)[3], const float4 sh, const float4 sl,
+                                         bool self, float (&part)[4]) {
+    const float dx = (th[0] - sh.x) + (tl[0] - sl.x);
+    const float dy = (th[1] - sh.y) + (tl[1] - sl.y);
+    const float dz = (th[2] - sh.z) + (tl[2] - sl.z);
+    const float r2 = dx * dx + dy * dy + dz * dz;
+    const float inv = self ? 0.f : rsqrtf(r2);
+    const float qi = sh.w * inv;
+    const float qi3 = qi * inv * inv;
+    part[0] += qi;
+    part[1] -= qi3 * dx;
+    part[2] -= qi3 * dy;
+    part[3] -= qi3 * dz;
+}
+
+// Relative position of neighbour slot s (-1 = the leaf itself): slot s is
+// pos = s (s < 13) or s + 1 of the 3x3x3 block, pos = 9 (dx+1) + 3 (dy+1) +
+// (dz+1); the source leaf's centre sits (dx, dy, dz) leaf widths away.
+__device__ __forceinline__ void slot_offset(int s, float (&c)[3]) {
+    const int pos = s < 0 ? 13 : (s < 13 ? s : s + 1);
+    c[0] = static_cast<float>(pos / 9 - 1);
+    c[1] = static_cast<float>((pos / 3) % 3 - 1);
+    c[2] = static_cast<float>(pos % 3 - 1);
+}
+
+constexpr int kKahanBlock = 16;  // pair terms summed plainly before a compensated add
+
+// Mixed P2P of the leaves with at most kP2PBig particles: one thread per
+// target (sorted order); results written in FP64 (potential / leaf width,
+// force * q / leaf width^2).
+__global__ __launch_bounds__(256) void k_p2p_mixed(int64_t n, const int32_t* __restrict__ pleaf,
+                                                  const int32_t* __restrict__ lstart, const int32_t* __restrict__ slots,
+                                                  const float4* __restrict__ hi, const float4* __restrict__ lo,
+                                                  const double* __restrict__ sp, double inv_w,
+                                                  double* __restrict__ res) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int li = pleaf[i];
+    if (lstart[li + 1] - lstart[li] > kP2PBig) return;  // k_p2p_mixed_tiled
+    const float4 t = hi[i], tlo = lo[i];
+    const float tl[3] = {tlo.x, tlo.y, tlo.z};
+    float sum[4] = {0.f, 0.f, 0.f, 0.f}, comp[4] = {0.f, 0.f, 0.f, 0.f};
+    for (int s = -1; s < kNbrSlots; ++s) {
+        const int nb = s < 0 ? li : slots[static_cast<int64_t>(li) * kNbrSlots + s];
+        if (nb < 0) continue;
+        float c[3];
+        slot_offset(s, c);
+        const float th[3] = {t.x - c[0], t.y - c[1], t.z - c[2]};
+        const int j0 = lstart[nb], j1 = lstart[nb + 1];
+        for (int jb = j0; jb < j1; jb += kKahanBlock) {
+            float part[4] = {0.f, 0.f, 0.f, 0.f};
+            const int je = min(j1, jb + kKahanBlock);
+            for (int j = jb; j < je; ++j) pair_f32(th, tl, hi[j], lo[j], j == i, part);
+#pragma unroll
+            for (int k = 0; k < 4; ++k) kahan_add(sum[k], comp[k], part[k]);
+        }
+    }
+    const double qf = sp[3 * n + i] * inv_w * inv_w;
+    res[i] = (static_cast<double>(sum[0]) - static_cast<double>(comp[0])) * inv_w;
+    res[n + i] = (static_cast<double>(sum[1]) - static_cast<double>(comp[1])) * qf;
+    res[2 * n + i] = (static_cast<double>(sum[2]) - static_cast<double>(comp[2])) * qf;
+    res[3 * n + i] = (static_cast<double>(sum[3]) - static_cast<double>(comp[3])) * qf;
+}
+
+// Mixed P2P of the big leaves: one wave per work item of up to kP2PItem
+// consecutive targets of one leaf, two per lane; the sources of each
+// neighbour leaf pass through LDS 64 at a time and are read as broadcasts.
+__global__ __launch_bounds__(64) void k_p2p_mixed_tiled(const int2* __restrict__ items,
+                                                       const int32_t* __restrict__ nitems,
+                                                       const int32_t* __restrict__ lstart,
+                                                       const int32_t* __restrict__ slots, const float4* __restrict__ hi,
+                                                       const float4* __restrict__ lo, const double* __restrict__ sp,
+                                                       int64_t n, double inv_w, double* __restrict__ res) {
+    __shared__ float4 sh[64], sl[64];
+    const int lane = threadIdx.x;
+    const int total = *nitems;
+    for (int it = blockIdx.x; it < total; it += gridDim.x) {
+        const int2 item = items[it];
+        const int leaf = item.x;
+        const int tend = min(lstart[leaf + 1], item.y + kP2PItem);
+        int ti[2];
+        bool ok[2];
+        float t[2][3], tl[2][3], sum[2][4], comp[2][4];
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {
+            ti[u] = item.y + 64 * u + lane;
+            ok[u] = ti[u] < tend;
+            const int tt = ok[u] ? ti[u] : item.y;
+            const float4 a = hi[tt], b = lo[tt];
+            t[u][0] = a.x; t[u][1] = a.y; t[u][2] = a.z;
+            tl[u][0] = b.x; tl[u][1] = b.y; tl[u][2] = b.z;
+#pragma unroll
+            for (int k = 0; k < 4; ++k) sum[u][k] = comp[u][k] = 0.f;
+        }
+        for (int s = -1; s < kNbrSlots; ++s) {
+            const int nb = s < 0 ? leaf : slots[static_cast<int64_t>(leaf) * kNbrSlots + s];
+            if (nb < 0) continue;
+            float c[3];
+            slot_offset(s, c);
+            float th[2][3];
+#pragma unroll
+            for (int u = 0; u < 2; ++u)
+#pragma unroll
+                for (int d = 0; d < 3; ++d) th[u][d] = t[u][d] - c[d];
+            const int j0 = lstart[nb], j1 = lstart[nb + 1];
+            for (int jc = j0; jc < j1; jc += 64) {
+                __syncthreads();  // the previous chunk is consumed
+                if (jc + lane < j1) {
+                    sh[lane] = hi[jc + lane];
+                    sl[lane] = lo[jc + lane];
+                }
+                __syncthreads();
+                const int m = min(64, j1 - jc);
+                for (int jb = 0; jb < m; jb += kKahanBlock) {
+                    float part[2][4] = {{0.f, 0.f, 0.f, 0.f}, {0.f, 0.f, 0.f, 0.f}};
+                    const int je = min(m, jb + kKahanBlock);
+                    for (int jj = jb; jj < je; ++jj) {
+                        const float4 a = sh[jj], b = sl[jj];
+#pragma unroll
+                        for (int u = 0; u < 2; ++u) pair_f32(th[u], tl[u], a, b, jc + jj == ti[u], part[u]);
+                    }
+#pragma unroll
+                    for (int u = 0; u < 2; ++u)
+#pragma unroll
+                        for (int k = 0; k < 4; ++k) kahan_add(sum[u][k], comp[u][k], part[u][k]);
+                }
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {
+            if (!ok[u]) continue;
+            const int i = ti[u];
+            const double qf = sp[3 * n + i] * inv_w * inv_w;
+            res[i] = (static_cast<double>(sum[u][0]) - static_cast<double>(comp[u][0])) * inv_w;
+            res[n + i] = (static_cast<double>(sum[u][1]) - static_cast<double>(comp[u][1])) * qf;
+            res[2 * n + i] = (static_cast<double>(sum[u][2]) - static_cast<double>(comp[u][2])) * qf;
+            res[3 * n + i] = (static_cast<double>(sum[u][3]) - static_cast<double>(comp[u][3])) * qf;
+        }
+    }
+}
+
+// A[i][k] = (7 i + 3 k) mod 11 - 5, B[k][j] = (5 k + 2 j) mod 13 - 6 (exact in
+// bfloat16, exact products and sums in FP32); D = A B written row-major
+// through the layout the M2L kernel assumes.
+__global__ void k_mfma_probe_bf16(float* out) {
+    const int l = threadIdx.x;
+    bf8 a, b;
+    for (int j = 0; j < 8; ++j) {
+        const int k = 8 * (l / 16) + j;
+        a[j] = static_cast<__bf16>(static_cast<float>((7 * (l % 16) + 3 * k) % 11 - 5));
+        b[j] = static_cast<__bf16>(static_cast<float>((5 * k + 2 * (l % 16)) % 13 - 6));
+    }
+    f4 c = {0.f, 0.f, 0.f, 0.f};
+    c = mfma_bf16(a, b, c);
+    for (int r = 0; r < 4; ++r) out[(4 * (l / 16) + r) * 16 + (l % 16)] = c[r];
+}
+
+inline unsigned blocks_of(int64_t n) { return static_cast<unsigned>((n + 255) / 256); }
+
+template <int P, int COLS>
+int m2l_mixed_impl(Context* c) {
+    int tiles = 0;
+    const M2LLevels lv = make_levels(c, 16 * COLS, tiles);
+    const int crows = (c->kpad_max + 31) & ~31;
+    const size_t shmem = sizeof(float) * kM2LWaves * static_cast<size_t>(crows) * 16 * COLS;
+    HFMM_HIP(c, hipFuncSetAttribute(reinterpret_cast<const void*>(&k_m2l_mixed<P, COLS>),
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(shmem)));
+    const int64_t plane = (c->total_cells + 1) * static_cast<int64_t>(c->LD);
+    prof_begin(c, "m2l_mixed");
+    hipLaunchKernelGGL((k_m2l_mixed<P, COLS>), dim3(static_cast<unsigned>(tiles)), dim3(64 * kM2LWaves), shmem,
+                       c->stream, lv, c->d_m2l.as<int32_t>(), c->d_order.as<int32_t>(), c->d_pcount.as<int32_t>(),
+                       c->d_slotord.as<int16_t>(), c->d_slotseg.as<int32_t>(), c->d_posoff.as<int64_t>(),
+                       c->d_opm.as<uint16_t>(), c->d_Mb.as<uint16_t>(), plane, c->total_cells, c->d_L.as<double>(),
+                       c->LD, crows);
+    prof_end(c);
+    HFMM_TRY(check_launch(c, "m2l_mixed"));
+    // the M2L targets: every cell of levels 2..h-1, contiguous in the cell arrays
+    const int64_t first = c->lvl_off[2], ncells = c->lvl_off[c->height()] - first;
+    prof_begin(c, "modal_to_nodal");
+    hipLaunchKernelGGL(k_modal_to_nodal<P>, dim3(static_cast<unsigned>((ncells + 3) / 4)), dim3(256), 0, c->stream,
+                       ncells, c->d_Troots.as<double>(), c->d_L.as<double>() + first * c->LD, c->LD);
+    prof_end(c);
+    return check_launch(c, "modal_to_nodal");
+}
+
+}  // namespace
+
+int prepare_mixed_operators(Context* c) {
+    const ChebOperators& op = c->ops;
+    const int LD = c->LD;
+    if (LD % 32) return fail(c, HFMM_ERR_ARG, "mixed precision needs 32 | padded node count");
+    bool far[343] = {false};
+    for (int e = 0; e < 8; ++e) {
+        FarSlot s[kFarSlots];
+        far_slots(e, s);
+        for (int j = 0; j < kFarSlots; ++j) far[s[j].pos] = true;
+    }
+    std::vector<int16_t> plist;
+    std::vector<int32_t> pkt;
+    std::vector<int64_t> posoff(343, 0);
+    int64_t total = 0;
+    for (int pos = 0; pos < 343; ++pos) {
+        if (!far[pos]) continue;
+        const int k = op.rank[op.pidx[pos]];
+        const int kt = (k + 15) / 16;
+        if (kt < 1 || kt > 4) return fail(c, HFMM_ERR_ARG, "operator rank outside 1..64");
+        plist.push_back(static_cast<int16_t>(pos));
+        pkt.push_back(kt);
+        posoff[pos] = total;
+        total += 3 * static_cast<int64_t>(16 * kt) * LD + 3 * static_cast<int64_t>(LD) * 32 * ((kt + 1) / 2);
+    }
+    DevBuf d_plist, d_pkt;
+    auto run = [&]() -> int {
+        HFMM_TRY(ensure(c, d_plist, sizeof(int16_t) * plist.size()));
+        HFMM_TRY(ensure(c, d_pkt, sizeof(int32_t) * pkt.size()));
+        HFMM_TRY(ensure(c, c->d_posoff, sizeof(int64_t) * posoff.size()));
+        HFMM_TRY(ensure(c, c->d_opm, sizeof(uint16_t) * static_cast<size_t>(total)));
+        HFMM_HIP(c, hipMemcpy(d_plist.ptr, plist.data(), sizeof(int16_t) * plist.size(), hipMemcpyHostToDevice));
+        HFMM_HIP(c, hipMemcpy(d_pkt.ptr, pkt.data(), sizeof(int32_t) * pkt.size(), hipMemcpyHostToDevice));
+        HFMM_HIP(c, hipMemcpy(c->d_posoff.ptr, posoff.data(), sizeof(int64_t) * posoff.size(), hipMemcpyHostToDevice));
+        hipLaunchKernelGGL(k_pack_positions, dim3(static_cast<unsigned>(plist.size())), dim3(256), 0, c->stream,
+                           d_plist.as<int16_t>(), d_pkt.as<int32_t>(), c->d_posoff.as<int64_t>(),
+                           c->d_pidx.as<uint32_t>(), c->d_pvec.as<uint16_t>(), c->d_opoff.as<int64_t>(),
+                           c->d_ops.as<double>(), c->d_Troots.as<double>(), op.p, LD, c->d_opm.as<uint16_t>());
+        HFMM_TRY(check_launch(c, "pack_positions"));
+        HFMM_HIP(c, hipStreamSynchronize(c->stream));
+        return HFMM_OK;
+    };
+    const int st = run();
+    release(d_plist);
+    release(d_pkt);
+    return st;
+}
+
+int run_split_multipoles(Context* c) {
+    const int64_t nvals = (c->total_cells + 1) * static_cast<int64_t>(c->LD);
+    HFMM_TRY(ensure(c, c->d_Mb, sizeof(uint16_t) * 3 * static_cast<size_t>(nvals)));
+    prof_begin(c, "split_multipoles");
+    hipLaunchKernelGGL(k_split_multipoles, dim3(blocks_of(nvals)), dim3(256), 0, c->stream, nvals,
+                       c->d_M.as<double>(), c->d_Mb.as<uint16_t>());
+    prof_end(c);
+    return check_launch(c, "split_multipoles");
+}
+
+int run_m2l_mixed(Context* c) {
+    if (c->kpad_max > 64) return fail(c, HFMM_ERR_ARG, "operator rank above 64 is not supported");
+    switch (c->order()) {
+        case 5: return m2l_mixed_impl<5, 4>(c);
+        case 6: return m2l_mixed_impl<6, 4>(c);
+        case 7: return m2l_mixed_impl<7, 4>(c);
+        case 8: return m2l_mixed_impl<8, 4>(c);
+        default: return fail(c, HFMM_ERR_ARG, "order not compiled (5..8)");
+    }
+}
+
+int run_p2p_mixed(Context* c) {
+    const int L = c->height() - 1;
+    HFMM_TRY(ensure(c, c->d_loc, sizeof(float) * 8 * static_cast<size_t>(c->n)));
+    const Box bx = make_box(c);
+    float4* hi = c->d_loc.as<float4>();
+    float4* lo = hi + c->n;
+    prof_begin(c, "local_coords");
+    hipLaunchKernelGGL(k_local_coords, dim3(blocks_of(c->n)), dim3(256), 0, c->stream, c->n, c->d_pleaf.as<int32_t>(),
+                       c->d_cellkey.as<uint64_t>() + c->lvl_off[L], bx, c->d_sorted.as<double>(), hi, lo);
+    prof_end(c);
+    HFMM_TRY(check_launch(c, "local_coords"));
+    const double inv_w = 1. / bx.leafw;
+    prof_begin(c, "p2p_mixed");
+    hipLaunchKernelGGL(k_p2p_mixed, dim3(blocks_of(c->n)), dim3(256), 0, c->stream, c->n, c->d_pleaf.as<int32_t>(),
+                       c->d_leafstart.as<int32_t>(), c->d_p2p.as<int32_t>(), hi, lo, c->d_sorted.as<double>(), inv_w,
+                       c->d_res_sorted.as<double>());
+    prof_end(c);
+    HFMM_TRY(check_launch(c, "p2p_mixed"));
+    prof_begin(c, "p2p_mixed_tiled");
+    hipLaunchKernelGGL(k_p2p_mixed_tiled, dim3(p2p_tiled_grid(c)), dim3(64), 0, c->stream, c->d_p2pitems.as<int2>(),
+                       p2p_item_count(c), c->d_leafstart.as<int32_t>(), c->d_p2p.as<int32_t>(), hi, lo,
+                       c->d_sorted.as<double>(), c->n, inv_w, c->d_res_sorted.as<double>());
+    prof_end(c);
+    return check_launch(c, "p2p_mixed_tiled");
+}
+
+int selftest_mfma_bf16(Context* c, float* out) {
+    DevBuf b;
+    int st = ensure(c, b, sizeof(float) * 256);
+    if (st == HFMM_OK) {
+        hipLaunchKernelGGL(k_mfma_probe_bf16, dim3(1), dim3(64), 0, c->stream, b.as<float>());
+        st = check_launch(c, "mfma_probe_bf16");
+    }
+    if (st == HFMM_OK &&
+        (hipMemcpyAsync(out, b.ptr, sizeof(float) * 256, hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+         hipStreamSynchronize(c->stream) != hipSuccess))
+        st = fail(c, HFMM_ERR_DEVICE, "probe copy failed");
+    release(b);
+    return st;
+}
+
+}  // namespace hfmm

@@ -183,10 +183,10 @@ static int upload_operators(Context* c) {
 
 static void free_all(Context* c) {
     DevBuf* bufs[] = {&c->d_Troots, &c->d_child, &c->d_pidx, &c->d_pvec, &c->d_rank, &c->d_opoff, &c->d_ops,
-                      &c->d_slotpos, &c->d_slotord, &c->d_slotseg, &c->d_in, &c->d_keys, &c->d_keys2, &c->d_vals, &c->d_vals2, &c->d_hist,
+                      &c->d_slotpos, &c->d_slotord, &c->d_slotseg, &c->d_posoff, &c->d_opm, &c->d_in, &c->d_keys, &c->d_keys2, &c->d_vals, &c->d_vals2, &c->d_hist,
                       &c->d_scan_part, &c->d_flags, &c->d_scan, &c->d_sorted, &c->d_perm, &c->d_pleaf,
                       &c->d_count, &c->d_cellkey, &c->d_cellkey_tmp, &c->d_childstart, &c->d_leafstart,
-                      &c->d_p2p, &c->d_m2l, &c->d_order, &c->d_pcount, &c->d_dense, &c->d_M, &c->d_L, &c->d_res_sorted,
+                      &c->d_p2p, &c->d_p2pitems, &c->d_m2l, &c->d_order, &c->d_pcount, &c->d_dense, &c->d_M, &c->d_L, &c->d_Mb, &c->d_loc, &c->d_res_sorted,
                       &c->d_res};
     for (DevBuf* b : bufs) release(*b);
 }
@@ -214,7 +214,7 @@ int hfmm_create(const hfmm_params* params, hfmm_ctx** out) {
     if (params->order < 5 || params->order > 8) return HFMM_ERR_ARG;
     if (params->tree_height < 3 || params->tree_height > 20) return HFMM_ERR_ARG;
     if (!(params->box_width > 0) || !std::isfinite(params->box_width)) return HFMM_ERR_ARG;
-    if (params->precision != HFMM_PRECISION_FP64) return HFMM_ERR_ARG;
+    if (params->precision != HFMM_PRECISION_FP64 && params->precision != HFMM_PRECISION_MIXED) return HFMM_ERR_ARG;
     hfmm_ctx* h = new (std::nothrow) hfmm_ctx();
     if (!h) return HFMM_ERR_NOMEM;
     Context* c = &h->c;
@@ -227,6 +227,7 @@ int hfmm_create(const hfmm_params* params, hfmm_ctx** out) {
     std::string err;
     int st = hfmm::build_cheb_operators(params->order, params->epsilon, c->ops, err);
     if (st == HFMM_OK) st = hfmm::upload_operators(c);
+    if (st == HFMM_OK && c->P.precision == HFMM_PRECISION_MIXED) st = hfmm::prepare_mixed_operators(c);
     if (st != HFMM_OK) {
         hfmm::free_all(c);
         (void)hipStreamDestroy(c->stream);
@@ -322,15 +323,21 @@ int hfmm_evaluate(hfmm_ctx* h, uint32_t ops) {
     if (ops & HFMM_M2M)
         for (int l = L - 1; l >= 2; --l) HFMM_TRY(hfmm::run_m2m(c, l));
     // transfer pass: M2L writes every local of levels 2..h-1
-    if (ops & HFMM_M2L)
-        HFMM_TRY(hfmm::run_m2l(c));
-    else
+    const bool mixed = c->P.precision == HFMM_PRECISION_MIXED;
+    if (ops & HFMM_M2L) {
+        if (mixed) {
+            HFMM_TRY(hfmm::run_split_multipoles(c));
+            HFMM_TRY(hfmm::run_m2l_mixed(c));
+        } else {
+            HFMM_TRY(hfmm::run_m2l(c));
+        }
+    } else
         HFMM_HIP(c, hipMemsetAsync(c->d_L.ptr, 0, exp_bytes, c->stream));
     // downward pass
     if (ops & HFMM_L2L)
         for (int l = 2; l <= L - 1; ++l) HFMM_TRY(hfmm::run_l2l(c, l));
     // near field initialises the results, the merge adds the far field
-    if (ops & HFMM_P2P) HFMM_TRY(hfmm::run_p2p(c));
+    if (ops & HFMM_P2P) HFMM_TRY(mixed ? hfmm::run_p2p_mixed(c) : hfmm::run_p2p(c));
     else HFMM_TRY(hfmm::zero_results(c));
     if (ops & HFMM_L2P) HFMM_TRY(hfmm::run_l2p(c));
     HFMM_TRY(hfmm::scatter_results(c));
@@ -550,6 +557,12 @@ int hfmm_selftest_mfma_f64(hfmm_ctx* h, double* out) {
     CTX_CHECK(h);
     if (!out) return HFMM_ERR_ARG;
     return hfmm::selftest_mfma_f64(&h->c, out);
+}
+
+int hfmm_selftest_mfma_bf16(hfmm_ctx* h, float* out) {
+    CTX_CHECK(h);
+    if (!out) return HFMM_ERR_ARG;
+    return hfmm::selftest_mfma_bf16(&h->c, out);
 }
 
 }  // extern "C"

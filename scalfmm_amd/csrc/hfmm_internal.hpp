@@ -17,6 +17,8 @@ constexpr int kMaxLevels = 21;
 constexpr int kNbrSlots = 26;   // P2P neighbour slots per leaf
 constexpr int kFarSlots = 189;  // M2L slots per cell
 constexpr int kTileTargets = 16;  // targets per M2L tile (the MFMA N dimension)
+constexpr int kP2PBig = 64;    // leaves with more particles run P2P through LDS-staged sources
+constexpr int kP2PItem = 128;  // targets per P2P work item of a big leaf (two per lane of a wave)
 
 // Grow-only device allocation.
 struct DevBuf {
@@ -51,6 +53,8 @@ struct Context {
     DevBuf d_slotpos;        // [8][189] int16 relative position of each slot
     DevBuf d_slotord;        // [8][189][2] int16 (slot, position) ordered by rank tiles
     DevBuf d_slotseg;        // [8][5] int32 start of the 1..4 rank-tile segments (+ end)
+    DevBuf d_posoff;         // [343] int64 offset of a position's mixed operators in d_opm
+    DevBuf d_opm;            // mixed precision: position-specific operators, bf16 hi/mid/lo planes
 
     // ---- particles
     int64_t n = 0;
@@ -80,6 +84,8 @@ struct Context {
     int64_t cs_off[kMaxLevels] = {0};
     DevBuf d_leafstart;                       // int32 [nleaf+1]
     DevBuf d_p2p;                             // int32 [nleaf][26]
+    DevBuf d_p2pitems;                        // int2 (leaf, first target) work items of the big leaves
+    int64_t p2p_items_cap = 0;                // upper bound of the item count (the count itself is on the device)
     DevBuf d_m2l;                             // int32 [cells of levels 2..h-1][189]
     int64_t m2l_off[kMaxLevels] = {0};        // offset (in cells) of a level in d_m2l
     DevBuf d_order;                           // int32 parity-sorted cell index per level (same offsets as d_m2l)
@@ -88,6 +94,8 @@ struct Context {
 
     // ---- expansions and results
     DevBuf d_M, d_L;      // [total_cells][LD]
+    DevBuf d_Mb;          // mixed precision: multipoles as bf16 hi/mid/lo planes [3][total_cells + 1][LD]
+    DevBuf d_loc;         // mixed precision: leaf-local float4 (x, y, z, q) of the sorted particles
     DevBuf d_res_sorted;  // pot, fx, fy, fz (sorted order)
     DevBuf d_res;         // pot, fx, fy, fz (input order)
 
@@ -126,8 +134,10 @@ int check_launch(Context* c, const char* name);
 
 // Tree construction (tree_device.hip)
 int build_tree_device(Context* c);
+// Device counter holding the number of P2P work items of the big leaves.
+inline int32_t* p2p_item_count(Context* c) { return reinterpret_cast<int32_t*>(c->d_count.as<int64_t>() + 12); }
 
-// Passes (fmm_device.hip)
+// FP64 passes (fmm_device.hip)
 int run_p2m(Context* c);
 int run_m2m(Context* c, int level);  // parents at `level`, children at level+1
 int run_m2l(Context* c);  // every level 2..h-1 in one launch
@@ -137,5 +147,12 @@ int run_p2p(Context* c);
 int zero_results(Context* c);
 int scatter_results(Context* c);
 int selftest_mfma_f64(Context* c, double* out);
+
+// Mixed-precision passes (fmm_mixed.hip)
+int prepare_mixed_operators(Context* c);  // position-specific bf16 operators
+int run_split_multipoles(Context* c);     // d_M -> d_Mb
+int run_m2l_mixed(Context* c);
+int run_p2p_mixed(Context* c);
+int selftest_mfma_bf16(Context* c, float* out);
 
 }  // namespace hfmm

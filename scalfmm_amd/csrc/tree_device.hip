@@ -400,6 +400,26 @@ inline unsigned blocks_for(int64_t n) { return static_cast<unsigned>((n + kBlock
 
 // Run-length groups of sorted keys >> shift: writes group keys and starts,
 // returns the count on the host.
+// P2P work items of the big leaves: ceil(count / kP2PItem) items of
+// consecutive targets per leaf with more than kP2PBig particles.
+__global__ __launch_bounds__(kBlock) void k_p2p_item_counts(int nleaf, const int32_t* __restrict__ lstart,
+                                                           int32_t* __restrict__ cnt) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nleaf) return;
+    const int m = lstart[i + 1] - lstart[i];
+    cnt[i] = m > kP2PBig ? (m + kP2PItem - 1) / kP2PItem : 0;
+}
+
+__global__ __launch_bounds__(kBlock) void k_p2p_items(int nleaf, const int32_t* __restrict__ lstart,
+                                                     const int32_t* __restrict__ off, int2* __restrict__ items) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nleaf) return;
+    const int m = lstart[i + 1] - lstart[i];
+    if (m <= kP2PBig) return;
+    const int k = (m + kP2PItem - 1) / kP2PItem;
+    for (int q = 0; q < k; ++q) items[off[i] + q] = make_int2(i, lstart[i] + q * kP2PItem);
+}
+
 int make_groups(Context* c, const uint64_t* keys, int64_t n, int shift, uint64_t* gkey, int32_t* gstart,
                 int32_t* group_of, int64_t& count, const char* tag) {
     HFMM_TRY(ensure(c, c->d_flags, sizeof(int32_t) * static_cast<size_t>(n + 1)));
@@ -541,6 +561,25 @@ int build_tree_device(Context* c) {
                        c->d_p2p.as<int32_t>());
     prof_end(c);
     HFMM_TRY(check_launch(c, "p2p_lists"));
+    {
+        // items of the big leaves; their count stays on the device (slot 12 of d_count)
+        c->p2p_items_cap = n / kP2PItem + n / (kP2PBig + 1) + 1;
+        HFMM_TRY(ensure(c, c->d_p2pitems, sizeof(int2) * static_cast<size_t>(c->p2p_items_cap)));
+        HFMM_TRY(ensure(c, c->d_flags, sizeof(int32_t) * static_cast<size_t>(nl + 1)));
+        HFMM_TRY(ensure(c, c->d_scan, sizeof(int32_t) * static_cast<size_t>(nl + 1)));
+        prof_begin(c, "p2p_items");
+        hipLaunchKernelGGL(k_p2p_item_counts, dim3(blocks_for(nl)), dim3(kBlock), 0, c->stream, static_cast<int>(nl),
+                           c->d_leafstart.as<int32_t>(), c->d_flags.as<int32_t>());
+        prof_end(c);
+        HFMM_TRY(check_launch(c, "p2p_items"));
+        HFMM_TRY(exclusive_scan<int32_t>(c, c->d_flags.as<int32_t>(), c->d_scan.as<int32_t>(), nl, p2p_item_count(c),
+                                         "p2p_items"));
+        prof_begin(c, "p2p_items");
+        hipLaunchKernelGGL(k_p2p_items, dim3(blocks_for(nl)), dim3(kBlock), 0, c->stream, static_cast<int>(nl),
+                           c->d_leafstart.as<int32_t>(), c->d_scan.as<int32_t>(), c->d_p2pitems.as<int2>());
+        prof_end(c);
+        HFMM_TRY(check_launch(c, "p2p_items"));
+    }
 
     // 6. M2L slots and parity-sorted target order of levels 2..h-1
     HFMM_TRY(ensure(c, c->d_m2l, sizeof(int32_t) * static_cast<size_t>(off) * kFarSlots));

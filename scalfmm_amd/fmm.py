@@ -206,18 +206,25 @@ class Engine:
         self._check(self._L.hfmm_selftest_mfma_f64(self.h, _dp(out)))
         return out.reshape(16, 16)
 
+    def selftest_mfma_bf16(self):
+        out = np.empty(256, dtype=np.float32)
+        self._check(self._L.hfmm_selftest_mfma_bf16(self.h, out.ctypes.data_as(C.POINTER(C.c_float))))
+        return out.reshape(16, 16)
+
 
 class FChebSymKernel:
     """Kernel parameters of FChebSymKernel<double, ..., FInterpMatrixKernelR, ORDER>."""
 
-    def __init__(self, tree_height, box_width, box_center, order, epsilon=None, device=0):
+    def __init__(self, tree_height, box_width, box_center, order, epsilon=None, device=0,
+                 precision=_abi.PRECISION_FP64):
         self.tree_height = int(tree_height)
         self.box_width = float(box_width)
         self.box_center = tuple(float(c) for c in box_center)
         self.order = int(order)
         self.epsilon = float(epsilon) if epsilon else 0.0
         self.device = device
-        self.engine = Engine(tree_height, box_width, box_center, order, self.epsilon, device)
+        self.precision = int(precision)
+        self.engine = Engine(tree_height, box_width, box_center, order, self.epsilon, device, self.precision)
 
 
 class FGroupTree:

@@ -1,0 +1,161 @@
+"""GPU parity of the mixed-precision form (HFMM_PRECISION_MIXED) through the C-ABI.
+
+The mixed form changes only the arithmetic of M2L (bfloat16 matrix cores on
+three-part operand splits, FP32 accumulation) and P2P (FP32 pair terms,
+compensated per-target sums); the tree, the lists and the other passes are the
+FP64 ones.  Stated tolerances:
+  - against the FP64 oracle on the same inputs: relative L2 <= 1e-6 for the
+    potential and <= 3e-6 for each force component (24-bit operands: a few
+    units of 2^-24 per interaction, accumulated over ~190 M2L terms);
+  - against FP64 direct sums: the FP64 form's own error plus that bound, and
+    <= 1e-6 for the potential at order 8 (the mixed form's accuracy target).
+"""
+import sys
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+POT_TOL = 1e-6
+FORCE_TOL = 3e-6
+
+
+def _say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def _rel(a, b):
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+def _mixed(h, p, width=1.0, center=(0.5, 0.5, 0.5)):
+    from scalfmm_amd import PRECISION_MIXED, Engine
+    return Engine(h, width, center, p, 0.0, 0, PRECISION_MIXED)
+
+
+def _check(got, ref, pot_tol=POT_TOL, force_tol=FORCE_TOL):
+    errs = [_rel(a, b) for a, b in zip(got, ref[:4])]
+    _say(f"  rel L2 vs FP64: pot {errs[0]:.3e} fx {errs[1]:.3e} fy {errs[2]:.3e} fz {errs[3]:.3e}")
+    assert errs[0] <= pot_tol, errs
+    assert max(errs[1:]) <= force_tol, errs
+    return errs
+
+
+def test_mfma_bf16_layout():
+    e = _mixed(3, 5)
+    D = e.selftest_mfma_bf16()
+    i = np.arange(16)[:, None]
+    k = np.arange(32)[None, :]
+    A = (7 * i + 3 * k) % 11 - 5
+    B = (5 * np.arange(32)[:, None] + 2 * np.arange(16)[None, :]) % 13 - 6
+    assert np.array_equal(D, (A @ B).astype(np.float32))
+
+
+@pytest.mark.parametrize("ops_name", ["p2p", "far", "all"])
+def test_mixed_fixture_order7(oracle, fixture20k, ops_name):
+    from scalfmm_amd import FFmmFarField, FFmmNearAndFarFields, FFmmP2P
+    ops = {"p2p": FFmmP2P, "far": FFmmFarField, "all": FFmmNearAndFarFields}[ops_name]
+    f = fixture20k
+    d = f["data"]
+    x, y, z, q = (d[:, k].copy() for k in range(4))
+    _say(f"mixed fixture20k h5 p7 ops={ops}")
+    e = _mixed(5, 7)
+    e.set_particles(x, y, z, q)
+    e.build_tree()
+    e.evaluate(ops)
+    ref = oracle.fmm(x, y, z, q, f["center"], f["width"], 5, 7, ops=ops)
+    _check(e.results(), ref)
+
+
+def test_mixed_fixture_vs_direct(oracle, fixture20k):
+    """The fixture's FP64 direct results: the mixed error stays within the
+    FP64 form's error plus the mixed bound."""
+    from test_oracle_fmm import PUBLISHED
+    f = fixture20k
+    d = f["data"]
+    e = _mixed(5, 7)
+    e.set_particles(*(d[:, k].copy() for k in range(4)))
+    e.build_tree()
+    e.evaluate()
+    for name, col, got in zip(("pot", "fx", "fy", "fz"), range(4, 8), e.results()):
+        acc = oracle.accuracy(d[:, col], got)
+        bound = PUBLISHED[name][0] + (POT_TOL if name == "pot" else FORCE_TOL)
+        _say(f"  {name}: rel L2 vs direct {acc['rel_l2']:.3e} (FP64 form {PUBLISHED[name][0]:.3e})")
+        assert acc["rel_l2"] <= bound, (name, acc)
+
+
+@pytest.mark.parametrize("order,h,n", [(5, 5, 20000), (6, 4, 8000), (8, 4, 6000), (8, 5, 40000)])
+def test_mixed_uniform_orders(oracle, order, h, n):
+    from scalfmm_amd.distributions import uniform_cube
+    x, y, z, q = uniform_cube(n, 200 + order)
+    _say(f"mixed uniform n={n} h={h} p={order}")
+    e = _mixed(h, order)
+    e.set_particles(x, y, z, q)
+    e.build_tree()
+    e.evaluate()
+    ref = oracle.fmm(x, y, z, q, (0.5, 0.5, 0.5), 1.0, h, order)
+    _check(e.results(), ref)
+
+
+@pytest.fixture(scope="module")
+def plummer100k(oracle):
+    """100k Plummer particles, height 6: core leaves hold thousands of particles."""
+    from scalfmm_amd.distributions import plummer
+    n, h, p = 100_000, 6, 8
+    x, y, z, q = plummer(n, 11)
+    return {"n": n, "h": h, "p": p, "xyzq": (x, y, z, q)}
+
+
+@pytest.mark.parametrize("ops_name", ["p2p", "far", "all"])
+def test_mixed_plummer(oracle, plummer100k, ops_name):
+    from scalfmm_amd import FFmmFarField, FFmmNearAndFarFields, FFmmP2P
+    ops = {"p2p": FFmmP2P, "far": FFmmFarField, "all": FFmmNearAndFarFields}[ops_name]
+    d = plummer100k
+    x, y, z, q = d["xyzq"]
+    _say(f"mixed plummer n={d['n']} h={d['h']} p={d['p']} ops={ops}")
+    e = _mixed(d["h"], d["p"])
+    e.set_particles(x, y, z, q)
+    e.build_tree()
+    e.evaluate(ops)
+    got = e.results()
+    ref = oracle.fmm(x, y, z, q, (0.5, 0.5, 0.5), 1.0, d["h"], d["p"], ops=ops)
+    _check(got, ref)
+    if ops_name == "all":
+        t = np.sort(np.random.default_rng(3).choice(d["n"], 500, replace=False))
+        dp = oracle.direct(x, y, z, q, targets=t)[0]
+        err = _rel(got[0][t], dp)
+        _say(f"  pot rel L2 vs direct on {len(t)} targets: {err:.3e} (FP64 form {_rel(ref[0][t], dp):.3e})")
+        assert err <= 1e-6
+
+
+def test_mixed_repeat_is_deterministic():
+    from scalfmm_amd.distributions import uniform_cube
+    x, y, z, q = uniform_cube(30000, 9)
+    e = _mixed(5, 6)
+    e.set_particles(x, y, z, q)
+    e.build_tree()
+    e.evaluate()
+    a = e.results()
+    e.build_tree()
+    e.evaluate()
+    b = e.results()
+    for u, v in zip(a, b):
+        assert np.array_equal(u, v)
+
+
+def test_mixed_matches_fp64_engine_lists():
+    """Same tree and lists as the FP64 form (the precision mode changes arithmetic only)."""
+    from scalfmm_amd import Engine
+    from scalfmm_amd.distributions import plummer
+    x, y, z, q = plummer(20000, 4)
+    a = _mixed(5, 5)
+    b = Engine(5, 1.0, (0.5, 0.5, 0.5), 5)
+    for e in (a, b):
+        e.set_particles(x, y, z, q)
+        e.build_tree()
+    assert np.array_equal(a.sorted_keys(), b.sorted_keys())
+    for level in range(2, 5):
+        for u, v in zip(a.m2l_list(level), b.m2l_list(level)):
+            assert np.array_equal(u, v)

@@ -143,6 +143,29 @@ def test_fmm_fixture_published_errors(oracle, fixture20k):
         assert abs(acc["rel_l2"] / PUBLISHED[name][0] - 1) < 0.01, (name, acc)
 
 
+@pytest.mark.parametrize("ops_name", ["p2p", "all"])
+def test_fmm_plummer_big_leaves(oracle, ops_name):
+    """Clustered input whose core leaves hold hundreds to thousands of particles
+    (the LDS-staged P2P path) next to sparse outer leaves."""
+    from scalfmm_amd import FFmmNearAndFarFields, FFmmP2P
+    from scalfmm_amd.distributions import plummer
+    ops = {"p2p": FFmmP2P, "all": FFmmNearAndFarFields}[ops_name]
+    n, h, p = 60000, 5, 5
+    x, y, z, q = plummer(n, 21)
+    _say(f"plummer n={n} h={h} p={p} ops={ops}")
+    e = _engine(h, p)
+    e.set_particles(x, y, z, q)
+    e.build_tree()
+    counts = e.leaf_ranges()[1]
+    _say(f"  largest leaf {counts.max()}, leaves above 64: {(counts > 64).sum()} of {len(counts)}")
+    assert counts.max() > 8 * 64
+    e.evaluate(ops)
+    got = e.results()
+    ref = oracle.fmm(x, y, z, q, (0.5, 0.5, 0.5), 1.0, h, p, ops=ops)
+    for a, b in zip(got, ref[:4]):
+        assert _rel(a, b) <= FP64_TOL
+
+
 @pytest.mark.parametrize("order,h,n", [(5, 5, 20000), (6, 4, 8000), (8, 4, 6000), (7, 6, 60000)])
 def test_fmm_uniform_orders(oracle, order, h, n):
     from scalfmm_amd.distributions import uniform_cube
