@@ -1,18 +1,27 @@
-"""Benchmark of the FP64 Chebyshev 1/r FMM on one MI355X.
+"""Benchmark of the Chebyshev 1/r FMM on one MI355X.
 
 A step is one pass of the hot path over the resident particles: build the
 octree on the device (keys, sort, leaves, levels, P2P and M2L lists) and
 evaluate P2M/M2M/M2L/L2L/L2P/P2P, with the particles already in HBM.
 
-Default workload: 1,000,000 particles uniform in the unit cube (drand48 seed
-2), charges U[-1,1], height 7, order 7, epsilon 1e-7, FP64.
+Default workload (the accuracy-targeted configuration): 10,000,000 particles
+from a seeded Plummer sphere (scale 1, truncated at radius 10, rescaled into
+the unit cube), charges U[-1,1], height 8, order 8.  `value` is the
+mixed-precision form's throughput; the plain FP64 form runs on the same
+particles beside it (`fp64_gpu`, `mixed_vs_fp64`), and both are checked
+against an FP64 direct sum on 10,000 sampled targets (`accuracy_vs_direct`,
+computed by the library's FP64 direct-sum kernel, outside the timed region).
 
-Prints one JSON line (rank 0).  `roofline` is the M2L kernel (the dominant
-one): algorithmic flops 4 p^3 k_c - k_c - p^3 per interaction (the reference's
-count, Src/Kernels/Chebyshev/FChebFlopsSymKernel.hpp:64-65) over the HIP-event
-time of its launches in the timed region, against the FP64 MFMA peak.
-`cpu_baseline` times the FP64 OpenMP oracle on a 100,000-particle sample with
-the same particles per leaf (height 6) on all host cores.
+`roofline` is the kernel with the largest share of the step, with its
+algorithmic work over the HIP-event time of its launches in the timed region:
+  - M2L: the reference's flop count 4 p^3 k - k - p^3 per interaction
+    (Src/Kernels/Chebyshev/FChebFlopsSymKernel.hpp:62-63); the mixed form
+    executes it as six bfloat16 partial products, so its algorithmic bf16 work
+    is six times that count, against the bf16 dense MFMA peak;
+  - P2P: 34 flops per target-source pair (FChebFlopsSymKernel.hpp:64-65)
+    against the FP32 (mixed) or FP64 vector peak.
+`cpu_baseline` times the FP64 OpenMP oracle on a bounded sample of the same
+distribution with the same particles per leaf, on all host cores.
 """
 from __future__ import annotations
 
@@ -28,7 +37,16 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 METRIC = "Mparticles/s, 1/r Cheb FMM @ rel err 1e-6 (mixed vs FP64 GPU); %HBM/MFMA peak"
-FP64_MFMA_PEAK_TFLOPS = 78.6
+PEAK = {  # MI355X published peaks
+    "fp64_mfma": 78.6, "bf16_mfma": 2500.0, "fp32_vector": 157.3, "fp64_vector": 78.6, "hbm_gbs": 8000.0,
+}
+WORKLOADS = {
+    # name: (distribution, n, height, order, cpu sample (n, height))
+    "plummer10M-h8-p8": ("plummer", 10_000_000, 8, 8, (100_000, 6)),
+    "plummer10M-h8-p5": ("plummer", 10_000_000, 8, 5, (100_000, 6)),
+    "uniform1M-h7-p7": ("uniform", 1_000_000, 7, 7, (100_000, 6)),
+}
+SEED = {"plummer": 7, "uniform": 2}
 
 
 def _canonical(i, j, k):
@@ -39,6 +57,7 @@ def _canonical(i, j, k):
 
 
 def algorithmic_m2l_flops(engine, height, order):
+    """The reference's M2L flop count summed over this tree's interaction lists."""
     p3 = order ** 3
     rank_of = {}
     for i in range(-3, 4):
@@ -64,43 +83,42 @@ def algorithmic_m2l_flops(engine, height, order):
     return total, count
 
 
-def direct_sample(x, y, z, q, targets):
-    """FP64 direct sum on a few targets (numpy, chunked over sources)."""
-    pot = np.zeros(len(targets))
-    fx = np.zeros(len(targets))
-    for a, t in enumerate(targets):
-        dx = x[t] - x
-        dy = y[t] - y
-        dz = z[t] - z
-        r2 = dx * dx + dy * dy + dz * dz
-        r2[t] = np.inf
-        inv = 1.0 / np.sqrt(r2)
-        pot[a] = np.dot(q, inv)
-        fx[a] = -q[t] * np.dot(q, inv ** 3 * dx)
-    return pot, fx
+def p2p_pairs(engine):
+    """Target-source pairs of the near field (self pairs excluded), split into
+    the leaves with at most 64 particles and the bigger ones."""
+    _, cnt = engine.leaf_ranges()
+    off, nb, _ = engine.p2p_list()
+    owner = np.repeat(np.arange(len(cnt)), np.diff(off))
+    nbr_sum = np.bincount(owner, weights=cnt[nb].astype(np.float64), minlength=len(cnt))
+    per_leaf = cnt * (cnt - 1) + cnt * nbr_sum
+    big = cnt > 64
+    return float(per_leaf[~big].sum()), float(per_leaf[big].sum())
 
 
-def cpu_baseline(seconds_budget=30.0):
+def make_particles(dist, n):
+    from scalfmm_amd.distributions import plummer, uniform_cube
+    return (plummer if dist == "plummer" else uniform_cube)(n, SEED[dist])
+
+
+def cpu_baseline(dist, n, h, p):
     from oracle import pyoracle
-    from scalfmm_amd.distributions import uniform_cube
-    n, h, p = 100_000, 6, 7
-    x, y, z, q = uniform_cube(n, 2)
+    x, y, z, q = make_particles(dist, n)
     cheb = pyoracle.Cheb(p)
     cores = os.cpu_count() or 1
     t0 = time.perf_counter()
     st = pyoracle.fmm(x, y, z, q, (0.5, 0.5, 0.5), 1.0, h, p, cheb=cheb, nthreads=cores)[4]
     wall = time.perf_counter() - t0
     return {"value": n / st.t_total / 1e6, "unit": "Mparticles/s", "cores": cores, "kind": "port",
-            "sample": f"oracle FP64 OpenMP FMM (tree + all passes, operators prebuilt), {n} uniform particles, "
-                      f"height {h}, order {p} (same particles per leaf as the 1M/height-7 workload); "
+            "sample": f"oracle FP64 OpenMP FMM (tree + all passes, operators prebuilt), {n} {dist} particles, "
+                      f"height {h}, order {p} (about the same particles per leaf as the full workload); "
                       f"{st.t_total:.2f} s, M2L {st.t_m2l:.2f} s, P2P {st.t_p2p:.2f} s, wall {wall:.2f} s"}
 
 
-def profiled_traffic(n, h, p):
-    """HBM bytes per M2L launch from the committed rocprofv3 summary of this
-    workload (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, see the summary), or None."""
-    path = os.path.join(ROOT, "profiles", f"m2l_fp64_uniform{n // 1_000_000}M_h{h}_p{p}.md")
-    if n % 1_000_000 or not os.path.exists(path):
+def profiled_traffic(kernel, workload):
+    """HBM bytes per launch of `kernel` from its committed rocprofv3 summary
+    (FETCH_SIZE and WRITE_SIZE scaled by a copy-kernel calibration), or None."""
+    path = os.path.join(ROOT, "profiles", f"{kernel}_{workload}.md")
+    if not os.path.exists(path):
         return None
     for line in open(path):
         if line.startswith("traffic_bytes_per_launch:"):
@@ -121,16 +139,74 @@ def copy_calibration(mbytes=512, reps=3):
     print(json.dumps({"copy_calibration_bytes_read": n * 8, "copy_calibration_bytes_written": n * 8, "reps": reps}))
 
 
+def run_form(parts, h, p, precision, steps, warmup):
+    """Warm up, then time `steps` tree builds + evaluations; kernel times from
+    HIP events on the engine's stream over the same region."""
+    from scalfmm_amd import Engine
+    x, y, z, q = parts
+    e = Engine(h, 1.0, (0.5, 0.5, 0.5), p, 0.0, 0, precision)
+    e.set_particles(x, y, z, q)  # host -> HBM once, outside the timed region
+    for _ in range(warmup):
+        e.build_tree()
+        e.evaluate()
+    e.synchronize()
+    e.set_profiling(True)
+    e.reset_kernel_times()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        e.build_tree()
+        e.evaluate()
+    e.synchronize()
+    t1 = time.perf_counter()
+    times = e.kernel_times()
+    e.set_profiling(False)
+    ms = (t1 - t0) * 1e3 / steps
+    return e, ms, {k: v[0] / steps for k, v in sorted(times.items())}, {k: v[1] / steps for k, v in times.items()}
+
+
+def roofline(engine, kernel_ms, launches, h, p, mixed, workload):
+    """Roofline of the kernel with the largest time share."""
+    m2l_name = "m2l_mixed" if mixed else "m2l"
+    p2p_small, p2p_big = ("p2p_mixed", "p2p_mixed_tiled") if mixed else ("p2p", "p2p_tiled")
+    top = max(kernel_ms, key=kernel_ms.get)
+    if top == m2l_name:
+        flops, n_int = algorithmic_m2l_flops(engine, h, p)
+        work = 6 * flops if mixed else flops
+        peak = PEAK["bf16_mfma"] if mixed else PEAK["fp64_mfma"]
+        kname, bound = ("k_m2l_mixed", "mfma") if mixed else ("k_m2l_f64", "mfma")
+        extra = {"m2l_interactions": n_int, "fp64_equivalent_flops_per_step": flops,
+                 "work": "6 bf16 partial products per product" if mixed else "FP64 products"}
+    elif top in (p2p_small, p2p_big):
+        small, big = p2p_pairs(engine)
+        pairs = big if top == p2p_big else small
+        work = 34.0 * pairs
+        peak = PEAK["fp32_vector"] if mixed else PEAK["fp64_vector"]
+        kname = {"p2p_mixed": "k_p2p_mixed", "p2p_mixed_tiled": "k_p2p_mixed_tiled", "p2p": "k_p2p_f64",
+                 "p2p_tiled": "k_p2p_f64_tiled"}[top]
+        bound = "valu"
+        extra = {"pairs": pairs, "work": "34 flops per target-source pair"}
+    else:
+        return {"kernel": top, "note": "no roofline model for this kernel"}
+    ms = kernel_ms[top]
+    achieved = work / (ms / 1e3) / 1e12
+    out = {"bound": bound, "achieved": achieved, "peak": peak, "unit": "TFLOP/s", "frac": achieved / peak,
+           "traffic": profiled_traffic(kname, workload), "kernel": kname, "kernel_ms_per_step": ms,
+           "launches_per_step": launches.get(top), "algorithmic_flops_per_launch": work / max(launches.get(top, 1), 1)}
+    out.update(extra)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--n", type=int, default=1_000_000)
-    ap.add_argument("--height", type=int, default=7)
-    ap.add_argument("--order", type=int, default=7)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", default="plummer10M-h8-p8", choices=sorted(WORKLOADS))
+    ap.add_argument("--precision", default="mixed", choices=["mixed", "fp64"])
+    ap.add_argument("--no-fp64-compare", action="store_true", help="skip the FP64 form beside the mixed one")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-accuracy", action="store_true")
+    ap.add_argument("--targets", type=int, default=10_000, help="sampled targets of the direct-sum check")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a 512 MiB elementwise copy (counter calibration)")
     args = ap.parse_args()
@@ -138,64 +214,69 @@ def main():
         copy_calibration()
         return
     if args.gpus != 1:
-        print(json.dumps({"error": "this engine runs on one GPU; multi-GPU sharding is a design note only "
-                                   "(see DESIGN.md), so --gpus must be 1"}))
+        print(json.dumps({"error": "this engine runs on one GPU; multi-GPU sharding is a design note only, "
+                                   "so --gpus must be 1"}))
         sys.exit(2)
 
-    from scalfmm_amd import Engine
-    from scalfmm_amd.distributions import uniform_cube
+    from scalfmm_amd import PRECISION_FP64, PRECISION_MIXED
 
-    n, h, p = args.n, args.height, args.order
-    x, y, z, q = uniform_cube(n, 2)
-    e = Engine(h, 1.0, (0.5, 0.5, 0.5), p)
-    e.set_particles(x, y, z, q)  # host -> HBM once, outside the timed region
-    for _ in range(args.warmup):
-        e.build_tree()
-        e.evaluate()
-    e.synchronize()
-    e.set_profiling(True)
-    e.reset_kernel_times()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        e.build_tree()
-        e.evaluate()
-    e.synchronize()
-    t1 = time.perf_counter()
-    times = e.kernel_times()
-    e.set_profiling(False)
-    ms_per_step = (t1 - t0) * 1e3 / args.steps
-    value = n / (ms_per_step / 1e3) / 1e6
+    dist, n, h, p, (cn, ch) = WORKLOADS[args.workload]
+    print(f"[bench] generating {n} {dist} particles", file=sys.stderr, flush=True)
+    parts = make_particles(dist, n)
+    mixed = args.precision == "mixed"
+    print(f"[bench] {args.precision} form: {args.warmup} warmup + {args.steps} timed steps", file=sys.stderr,
+          flush=True)
+    e, ms, kms, launches = run_form(parts, h, p, PRECISION_MIXED if mixed else PRECISION_FP64, args.steps,
+                                    args.warmup)
+    value = n / (ms / 1e3) / 1e6
+    rl = roofline(e, kms, launches, h, p, mixed, args.workload)
+    got = e.results()
+    direct = None
+    if not args.no_accuracy:
+        print(f"[bench] FP64 direct sum on {args.targets} sampled targets", file=sys.stderr, flush=True)
+        t = np.sort(np.random.default_rng(5).choice(n, min(args.targets, n), replace=False))
+        direct = (t,) + e.direct(t)[:2]
+    e.close()
 
-    flops, n_m2l = algorithmic_m2l_flops(e, h, p)
-    m2l_ms, m2l_launches = times.get("m2l", (float("nan"), 0))
-    m2l_ms_step = m2l_ms / args.steps
-    achieved = flops / (m2l_ms_step / 1e3) / 1e12 if m2l_ms_step > 0 else float("nan")
+    fp64 = None
+    got64 = None
+    if mixed and not args.no_fp64_compare:
+        print("[bench] fp64 form on the same particles", file=sys.stderr, flush=True)
+        e64, ms64, kms64, launches64 = run_form(parts, h, p, PRECISION_FP64, args.steps, args.warmup)
+        fp64 = {"value": n / (ms64 / 1e3) / 1e6, "ms_per_step": ms64, "kernel_ms_per_step": kms64,
+                "roofline": roofline(e64, kms64, launches64, h, p, False, args.workload)}
+        got64 = e64.results()
+        e64.close()
 
     acc = None
-    if not args.no_accuracy:
-        pot, fx, _, _ = e.results()
-        rng = np.random.default_rng(5)
-        t = np.sort(rng.choice(n, 200, replace=False))
-        rp, rfx = direct_sample(x, y, z, q, t)
-        acc = {"targets": len(t), "pot_rel_l2": float(np.linalg.norm(pot[t] - rp) / np.linalg.norm(rp)),
-               "fx_rel_l2": float(np.linalg.norm(fx[t] - rfx) / np.linalg.norm(rfx))}
+    if direct is not None:
+        t, rp, rfx = direct
+
+        def rel(a, b):
+            return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+        acc = {"targets": len(t), args.precision: {"pot_rel_l2": rel(got[0][t], rp), "fx_rel_l2": rel(got[1][t], rfx)}}
+        if got64 is not None:
+            acc["fp64"] = {"pot_rel_l2": rel(got64[0][t], rp), "fx_rel_l2": rel(got64[1][t], rfx)}
+            acc["mixed_vs_fp64_pot_rel_l2"] = rel(got[0], got64[0])
+            acc["mixed_vs_fp64_fx_rel_l2"] = rel(got[1], got64[1])
 
     out = {
         "metric": METRIC, "value": value, "unit": "Mparticles/s", "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
-        "config": {"workload": f"uniform-{n}-h{h}-p{p}-fp64", "particles": n, "height": h, "order": p,
-                   "epsilon": 10.0 ** -p, "precision": "fp64", "distribution": "uniform unit cube, drand48 seed 2"},
-        "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP64_MFMA_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / FP64_MFMA_PEAK_TFLOPS, "traffic": profiled_traffic(n, h, p),
-                     "traffic_source": "profiles/ rocprofv3 FETCH_SIZE/WRITE_SIZE summary", "kernel": "k_m2l_f64",
-                     "algorithmic_flops_per_step": flops, "m2l_interactions": n_m2l,
-                     "m2l_ms_per_step": m2l_ms_step, "m2l_launches_per_step": m2l_launches / args.steps},
-        "kernel_ms_per_step": {k: v[0] / args.steps for k, v in sorted(times.items())},
+        "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "bf16x3-mfma/f32 (M2L), f32 (P2P), f64 (rest)" if mixed else "f64",
+        "data": "synthetic",
+        "config": {"workload": args.workload, "particles": n, "distribution": dist,
+                   "seed": SEED[dist], "height": h, "order": p, "epsilon": 10.0 ** -p, "precision": args.precision},
+        "roofline": rl,
+        "kernel_ms_per_step": kms,
+        "fp64_gpu": fp64,
+        "mixed_vs_fp64": (value / fp64["value"]) if fp64 else None,
         "accuracy_vs_direct": acc,
     }
     if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline()
+        print(f"[bench] cpu baseline ({cn} particles, height {ch})", file=sys.stderr, flush=True)
+        out["cpu_baseline"] = cpu_baseline(dist, cn, ch, p)
     print(json.dumps(out))
 
 

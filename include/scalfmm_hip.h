@@ -118,6 +118,15 @@ int hfmm_get_results(hfmm_ctx* ctx, double* pot, double* fx, double* fy, double*
 int hfmm_get_results_device(hfmm_ctx* ctx, const double** pot, const double** fx, const double** fy,
                             const double** fz);
 
+/* FP64 direct sum over all particles of the context for the given targets
+ * (input-order indices, host array): potential and force as the FMM reports
+ * them, the O(N^2) reference the reference's test programs compute with
+ * FP2P::NonMutualParticles (Tests/Kernels/DirectAlgorithm.cpp:203-215).
+ * Needs hfmm_set_particles*; results are host arrays of ntargets entries
+ * (NULL skips one). */
+int hfmm_direct(hfmm_ctx* ctx, int64_t ntargets, const int64_t* targets, double* pot, double* fx, double* fy,
+                double* fz);
+
 /* ---- inspection (used by the parity tests) -------------------------------- */
 typedef struct hfmm_tree_info {
     int64_t n_particles;

@@ -29,7 +29,7 @@ EXPORTS = (
     "hfmm_get_tree_info", "hfmm_get_sorted_keys", "hfmm_get_cell_keys", "hfmm_get_leaf_ranges",
     "hfmm_get_p2p_list", "hfmm_get_m2l_list", "hfmm_get_operator", "hfmm_get_expansions",
     "hfmm_set_profiling", "hfmm_get_kernel_times", "hfmm_reset_kernel_times", "hfmm_selftest_mfma_f64",
-    "hfmm_selftest_mfma_bf16",
+    "hfmm_selftest_mfma_bf16", "hfmm_direct",
     "hfmm_version",
 )
 
@@ -91,6 +91,7 @@ def lib(path: str | None = None):
     L.hfmm_reset_kernel_times.argtypes = [vp]
     L.hfmm_selftest_mfma_f64.argtypes = [vp, dp]
     L.hfmm_selftest_mfma_bf16.argtypes = [vp, C.POINTER(C.c_float)]
+    L.hfmm_direct.argtypes = [vp, C.c_int64, C.POINTER(C.c_int64), dp, dp, dp, dp]
     L.hfmm_version.restype = C.c_char_p
     for name in EXPORTS:
         getattr(L, name).restype = getattr(L, name).restype if name in ("hfmm_last_error", "hfmm_version") \

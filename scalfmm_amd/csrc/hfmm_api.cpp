@@ -559,6 +559,18 @@ int hfmm_selftest_mfma_f64(hfmm_ctx* h, double* out) {
     return hfmm::selftest_mfma_f64(&h->c, out);
 }
 
+int hfmm_direct(hfmm_ctx* h, int64_t ntargets, const int64_t* targets, double* pot, double* fx, double* fy,
+                double* fz) {
+    CTX_CHECK(h);
+    Context* c = &h->c;
+    if (c->n <= 0) return hfmm::fail(c, HFMM_ERR_STATE, "direct before set_particles");
+    if (ntargets <= 0 || !targets) return hfmm::fail(c, HFMM_ERR_ARG, "direct: bad targets");
+    for (int64_t i = 0; i < ntargets; ++i)
+        if (targets[i] < 0 || targets[i] >= c->n) return hfmm::fail(c, HFMM_ERR_ARG, "direct: target out of range");
+    HFMM_HIP(c, hipSetDevice(c->device));
+    return hfmm::direct_sum(c, ntargets, targets, pot, fx, fy, fz);
+}
+
 int hfmm_selftest_mfma_bf16(hfmm_ctx* h, float* out) {
     CTX_CHECK(h);
     if (!out) return HFMM_ERR_ARG;

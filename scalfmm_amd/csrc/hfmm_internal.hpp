@@ -155,4 +155,7 @@ int run_m2l_mixed(Context* c);
 int run_p2p_mixed(Context* c);
 int selftest_mfma_bf16(Context* c, float* out);
 
+// FP64 direct sum for sampled targets (direct.hip)
+int direct_sum(Context* c, int64_t nt, const int64_t* targets, double* pot, double* fx, double* fy, double* fz);
+
 }  // namespace hfmm

@@ -206,6 +206,13 @@ class Engine:
         self._check(self._L.hfmm_selftest_mfma_f64(self.h, _dp(out)))
         return out.reshape(16, 16)
 
+    def direct(self, targets):
+        """FP64 direct sum on the device for the given input-order targets."""
+        t = np.ascontiguousarray(targets, dtype=np.int64)
+        out = [np.empty(len(t)) for _ in range(4)]
+        self._check(self._L.hfmm_direct(self.h, len(t), _ip64(t), *(_dp(a) for a in out)))
+        return tuple(out)
+
     def selftest_mfma_bf16(self):
         out = np.empty(256, dtype=np.float32)
         self._check(self._L.hfmm_selftest_mfma_bf16(self.h, out.ctypes.data_as(C.POINTER(C.c_float))))

@@ -39,6 +39,21 @@ def _clustered(n, seed):
     return pts[:, 0].copy(), pts[:, 1].copy(), pts[:, 2].copy(), rng.uniform(-1, 1, n)
 
 
+def test_direct_matches_fixture_and_oracle(oracle, fixture20k):
+    """The device FP64 direct sum against the fixture's stored direct results
+    and the oracle's direct sum."""
+    d = fixture20k["data"]
+    x, y, z, q = (d[:, k].copy() for k in range(4))
+    e = _engine(5, 5)
+    e.set_particles(x, y, z, q)
+    t = np.array([0, 1, 17, 5000, 12345, 19999, 7, 8, 9])
+    got = e.direct(t)
+    ref = oracle.direct(x, y, z, q, targets=t)
+    for k, (a, b) in enumerate(zip(got, ref)):
+        assert _rel(a, b) <= 1e-13, k
+        assert _rel(a, d[t, 4 + k]) <= 1e-12, k
+
+
 def test_mfma_f64_layout():
     e = _engine(3, 5)
     D = e.selftest_mfma_f64()
