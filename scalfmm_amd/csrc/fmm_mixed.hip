@@ -14,8 +14,8 @@
 // not relative to the cell's mean potential, so the gradient modes that make
 // the forces keep their accuracy.  A small FP64 kernel turns the modal locals
 // back into node values.  P2M, M2M, L2L and L2P stay FP64.
-// P2P: FP32 pair terms on leaf-local coordinates split in exact hi and small
-// lo parts (see k_local_coords), 1/r by rsqrt, per-target sums compensated
+// P2P: FP32 pair terms on leaf-local fixed-point coordinates (exact integer
+// separations, see k_local_coords), 1/r by rsqrt, per-target sums compensated
 // (Kahan) once per block of 16 pair terms, results written in FP64.
 //
 // Reference operators: Src/Kernels/Chebyshev/FChebSymKernel.hpp:210-328 (M2L),
@@ -371,29 +371,30 @@ __global__ __launch_bounds__(256) void k_m2l_mixed(M2LLevels lv, const int32_t* 
     }
 }
 
-// Leaf-local coordinates of the sorted particles in leaf widths, as float
-// pairs: hi = the offset from the leaf centre rounded to a multiple of 2^-23,
-// lo = the remainder.  Differences of hi parts across neighbouring leaves
-// (centres one whole unit apart) are then exact in FP32, and a pair's
-// separation dx = (hi_t - c - hi_s) + (lo_t - lo_s) carries one rounding
-// relative to dx itself, however close the pair.  hi.w holds the charge.
+// Leaf-local fixed-point coordinates of the sorted particles: the offset from
+// the leaf centre in units of 2^-30 leaf widths, clamped to |value| < 2^29
+// (one unit off at most, on the leaf faces), charge as FP32 bits in .w.  A
+// separation (t - c) - s, c = the source leaf's centre offset (0 or +-2^30
+// units), is exact in int32 (|.| <= 2^31 - 2) and rounds once when converted
+// to FP32, relative to the separation itself; the coordinate resolution is
+// 2^-31 leaf widths.
+constexpr double kUnitsPerLeaf = 1073741824.;  // 2^30
+constexpr int kHalfLeaf = 1 << 29;
+
 __global__ __launch_bounds__(256) void k_local_coords(int64_t n, const int32_t* __restrict__ pleaf,
                                                      const uint64_t* __restrict__ lkeys, Box bx,
-                                                     const double* __restrict__ sp, float4* __restrict__ hi,
-                                                     float4* __restrict__ lo) {
+                                                     const double* __restrict__ sp, int4* __restrict__ loc) {
     const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= n) return;
     double a[3], b[3];
     leaf_bounds(bx, lkeys[pleaf[i]], a, b);
-    float h[3], l[3];
+    int v[3];
     for (int d = 0; d < 3; ++d) {
-        const double x = (sp[d * n + i] - (a[d] + b[d]) / 2.) / bx.leafw;
-        const double xh = rint(x * 8388608.) / 8388608.;  // 2^23
-        h[d] = static_cast<float>(xh);
-        l[d] = static_cast<float>(x - xh);
+        const double x = (sp[d * n + i] - (a[d] + b[d]) / 2.) / bx.leafw;  // in [-1/2, 1/2]
+        const double u = rint(x * kUnitsPerLeaf);
+        v[d] = static_cast<int>(fmin(fmax(u, -(kHalfLeaf - 1.)), kHalfLeaf - 1.));
     }
-    hi[i] = make_float4(h[0], h[1], h[2], static_cast<float>(sp[3 * n + i]));
-    lo[i] = make_float4(l[0], l[1], l[2], 0.f);
+    loc[i] = make_int4(v[0], v[1], v[2], __float_as_int(static_cast<float>(sp[3 * n + i])));
 }
 
 // Compensated (Kahan) add of a block partial into (sum, comp).
@@ -404,146 +405,168 @@ __device__ __forceinline__ void kahan_add(float& sum, float& comp, float part) {
     sum = t;
 }
 
-// One source against one target in leaf units: separation from the exact hi
-// difference plus the lo difference, 1/r by rsqrt.
-__device__ __forceinline__ void pair_f32(const float (&th)[3], const float (&tl)[3], const float4 sh, const float4 sl,
-                                         bool self, float (&part)[4]) {
-    const float dx = (th[0] - sh.x) + (tl[0] - sl.x);
-    const float dy = (th[1] - sh.y) + (tl[1] - sl.y);
-    const float dz = (th[2] - sh.z) + (tl[2] - sl.z);
+// One source against one target; t = the target's coordinates minus the
+// source leaf's centre offset.  With MASK the pair is skipped when `skip`
+// holds (the pair of a target with itself, sources past a leaf's end).
+template <bool MASK>
+__device__ __forceinline__ void pair_f32(const int (&t)[3], int4 s, bool skip, float (&part)[4]) {
+    const float dx = static_cast<float>(t[0] - s.x);
+    const float dy = static_cast<float>(t[1] - s.y);
+    const float dz = static_cast<float>(t[2] - s.z);
     const float r2 = dx * dx + dy * dy + dz * dz;
-    const float inv = self ? 0.f : rsqrtf(r2);
-    const float qi = sh.w * inv;
-    const float qi3 = qi * inv * inv;
+    // v_rsq_f32 directly: r2 is 0 (only for a skipped pair) or at least
+    // 1 unit^2, never denormal
+    float inv = __builtin_amdgcn_rsqf(r2);
+    if (MASK) inv = skip ? 0.f : inv;
+    const float qi = __int_as_float(s.w) * inv;
+    const float qi3 = qi * (inv * inv);
     part[0] += qi;
     part[1] -= qi3 * dx;
     part[2] -= qi3 * dy;
     part[3] -= qi3 * dz;
 }
 
-// Relative position of neighbour slot s (-1 = the leaf itself): slot s is
-// pos = s (s < 13) or s + 1 of the 3x3x3 block, pos = 9 (dx+1) + 3 (dy+1) +
-// (dz+1); the source leaf's centre sits (dx, dy, dz) leaf widths away.
-__device__ __forceinline__ void slot_offset(int s, float (&c)[3]) {
+// Centre offset of neighbour slot s (-1 = the leaf itself) in coordinate
+// units: slot s is pos = s (s < 13) or s + 1 of the 3x3x3 block,
+// pos = 9 (dx+1) + 3 (dy+1) + (dz+1).
+__device__ __forceinline__ void slot_offset(int s, int (&c)[3]) {
     const int pos = s < 0 ? 13 : (s < 13 ? s : s + 1);
-    c[0] = static_cast<float>(pos / 9 - 1);
-    c[1] = static_cast<float>((pos / 3) % 3 - 1);
-    c[2] = static_cast<float>(pos % 3 - 1);
+    c[0] = (pos / 9 - 1) * (2 * kHalfLeaf);
+    c[1] = ((pos / 3) % 3 - 1) * (2 * kHalfLeaf);
+    c[2] = (pos % 3 - 1) * (2 * kHalfLeaf);
 }
 
 constexpr int kKahanBlock = 16;  // pair terms summed plainly before a compensated add
 
-// Mixed P2P of the leaves with at most kP2PBig particles: one thread per
-// target (sorted order); results written in FP64 (potential / leaf width,
-// force * q / leaf width^2).
-__global__ __launch_bounds__(256) void k_p2p_mixed(int64_t n, const int32_t* __restrict__ pleaf,
-                                                  const int32_t* __restrict__ lstart, const int32_t* __restrict__ slots,
-                                                  const float4* __restrict__ hi, const float4* __restrict__ lo,
-                                                  const double* __restrict__ sp, double inv_w,
-                                                  double* __restrict__ res) {
-    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const int li = pleaf[i];
-    if (lstart[li + 1] - lstart[li] > kP2PBig) return;  // k_p2p_mixed_tiled
-    const float4 t = hi[i], tlo = lo[i];
-    const float tl[3] = {tlo.x, tlo.y, tlo.z};
-    float sum[4] = {0.f, 0.f, 0.f, 0.f}, comp[4] = {0.f, 0.f, 0.f, 0.f};
-    for (int s = -1; s < kNbrSlots; ++s) {
-        const int nb = s < 0 ? li : slots[static_cast<int64_t>(li) * kNbrSlots + s];
-        if (nb < 0) continue;
-        float c[3];
-        slot_offset(s, c);
-        const float th[3] = {t.x - c[0], t.y - c[1], t.z - c[2]};
-        const int j0 = lstart[nb], j1 = lstart[nb + 1];
-        for (int jb = j0; jb < j1; jb += kKahanBlock) {
-            float part[4] = {0.f, 0.f, 0.f, 0.f};
-            const int je = min(j1, jb + kKahanBlock);
-            for (int j = jb; j < je; ++j) pair_f32(th, tl, hi[j], lo[j], j == i, part);
-#pragma unroll
-            for (int k = 0; k < 4; ++k) kahan_add(sum[k], comp[k], part[k]);
-        }
-    }
-    const double qf = sp[3 * n + i] * inv_w * inv_w;
-    res[i] = (static_cast<double>(sum[0]) - static_cast<double>(comp[0])) * inv_w;
+// Scales of the unit-coordinate sums: potential / (leaf width 2^-30), force /
+// (leaf width 2^-30)^2, times the target charge for the force.
+__device__ __forceinline__ void write_p2p(int64_t n, int64_t i, const float (&sum)[4], const float (&comp)[4],
+                                          double q, double inv_u, double* __restrict__ res) {
+    const double qf = q * inv_u * inv_u;
+    res[i] = (static_cast<double>(sum[0]) - static_cast<double>(comp[0])) * inv_u;
     res[n + i] = (static_cast<double>(sum[1]) - static_cast<double>(comp[1])) * qf;
     res[2 * n + i] = (static_cast<double>(sum[2]) - static_cast<double>(comp[2])) * qf;
     res[3 * n + i] = (static_cast<double>(sum[3]) - static_cast<double>(comp[3])) * qf;
 }
 
-// Mixed P2P of the big leaves: one wave per work item of up to kP2PItem
-// consecutive targets of one leaf, two per lane; the sources of each
-// neighbour leaf pass through LDS 64 at a time and are read as broadcasts.
-__global__ __launch_bounds__(64) void k_p2p_mixed_tiled(const int2* __restrict__ items,
-                                                       const int32_t* __restrict__ nitems,
-                                                       const int32_t* __restrict__ lstart,
-                                                       const int32_t* __restrict__ slots, const float4* __restrict__ hi,
-                                                       const float4* __restrict__ lo, const double* __restrict__ sp,
-                                                       int64_t n, double inv_w, double* __restrict__ res) {
-    __shared__ float4 sh[64], sl[64];
-    const int lane = threadIdx.x;
-    const int total = *nitems;
-    for (int it = blockIdx.x; it < total; it += gridDim.x) {
-        const int2 item = items[it];
-        const int leaf = item.x;
-        const int tend = min(lstart[leaf + 1], item.y + kP2PItem);
-        int ti[2];
-        bool ok[2];
-        float t[2][3], tl[2][3], sum[2][4], comp[2][4];
+// Mixed P2P of the leaves with at most kP2PBig particles: one thread per
+// target (sorted order).
+__global__ __launch_bounds__(256) void k_p2p_mixed(int64_t n, const int32_t* __restrict__ pleaf,
+                                                  const int32_t* __restrict__ lstart, const int32_t* __restrict__ slots,
+                                                  const int4* __restrict__ loc, const double* __restrict__ sp,
+                                                  double inv_u, double* __restrict__ res) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int li = pleaf[i];
+    if (lstart[li + 1] - lstart[li] > kP2PBig) return;  // k_p2p_mixed_big
+    const int4 t = loc[i];
+    float sum[4] = {0.f, 0.f, 0.f, 0.f}, comp[4] = {0.f, 0.f, 0.f, 0.f};
+    for (int s = -1; s < kNbrSlots; ++s) {
+        const int nb = s < 0 ? li : slots[static_cast<int64_t>(li) * kNbrSlots + s];
+        if (nb < 0) continue;
+        int c[3];
+        slot_offset(s, c);
+        const int tc[3] = {t.x - c[0], t.y - c[1], t.z - c[2]};
+        const int j0 = lstart[nb], j1 = lstart[nb + 1];
+        for (int jb = j0; jb < j1; jb += kKahanBlock) {
+            float part[4] = {0.f, 0.f, 0.f, 0.f};
+            const int je = min(j1, jb + kKahanBlock);
+            for (int j = jb; j < je; ++j) pair_f32<true>(tc, loc[j], j == i, part);
 #pragma unroll
-        for (int u = 0; u < 2; ++u) {
-            ti[u] = item.y + 64 * u + lane;
-            ok[u] = ti[u] < tend;
-            const int tt = ok[u] ? ti[u] : item.y;
-            const float4 a = hi[tt], b = lo[tt];
-            t[u][0] = a.x; t[u][1] = a.y; t[u][2] = a.z;
-            tl[u][0] = b.x; tl[u][1] = b.y; tl[u][2] = b.z;
+            for (int k = 0; k < 4; ++k) kahan_add(sum[k], comp[k], part[k]);
+        }
+    }
+    write_p2p(n, i, sum, comp, sp[3 * n + i], inv_u, res);
+}
+
+// Mixed P2P of the big leaves.  Each wave takes work items of up to
+// kP2PItem = 64 TPL consecutive targets of one leaf (TPL per lane) and walks
+// the sources of the 27 neighbour leaves in blocks of 16; every source address
+// is wave-uniform, so the sources arrive through scalar loads into SGPRs and
+// feed the pair arithmetic as scalar operands (no LDS, no barriers).  A
+// block running past the leaf's end skips the extra sources (the particle
+// array has kKahanBlock padding entries, so the reads stay in bounds).
+template <int TPL, bool SELF>
+__device__ __forceinline__ void p2p_leaf_sources(const int4* __restrict__ loc, int j0, int j1, const int (&tc)[TPL][3],
+                                                 const int (&ti)[TPL], float (&sum)[TPL][4], float (&comp)[TPL][4]) {
+    int jb = j0;
+    for (; jb + kKahanBlock <= j1; jb += kKahanBlock) {
+        float part[TPL][4] = {};
+#pragma unroll
+        for (int jj = 0; jj < kKahanBlock; ++jj) {
+            const int4 s = loc[jb + jj];
+#pragma unroll
+            for (int u = 0; u < TPL; ++u) pair_f32<SELF>(tc[u], s, jb + jj == ti[u], part[u]);
+        }
+#pragma unroll
+        for (int u = 0; u < TPL; ++u)
+#pragma unroll
+            for (int k = 0; k < 4; ++k) kahan_add(sum[u][k], comp[u][k], part[u][k]);
+    }
+    if (jb < j1) {
+        float part[TPL][4] = {};
+#pragma unroll
+        for (int jj = 0; jj < kKahanBlock; ++jj) {
+            const int4 s = loc[jb + jj];
+            const bool past = jb + jj >= j1;
+#pragma unroll
+            for (int u = 0; u < TPL; ++u) pair_f32<true>(tc[u], s, past || (SELF && jb + jj == ti[u]), part[u]);
+        }
+#pragma unroll
+        for (int u = 0; u < TPL; ++u)
+#pragma unroll
+            for (int k = 0; k < 4; ++k) kahan_add(sum[u][k], comp[u][k], part[u][k]);
+    }
+}
+
+template <int TPL>
+__global__ __launch_bounds__(256) void k_p2p_mixed_big(const int2* __restrict__ items,
+                                                      const int32_t* __restrict__ nitems,
+                                                      const int32_t* __restrict__ lstart,
+                                                      const int32_t* __restrict__ slots, const int4* __restrict__ loc,
+                                                      const double* __restrict__ sp, int64_t n, double inv_u,
+                                                      double* __restrict__ res) {
+    static_assert(64 * TPL == kP2PItem, "work item = one wave x TPL targets");
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int total = *nitems;
+    for (int it = blockIdx.x * 4 + wave; it < total; it += gridDim.x * 4) {
+        const int2 item = items[it];
+        const int leaf = __builtin_amdgcn_readfirstlane(item.x);
+        const int first = __builtin_amdgcn_readfirstlane(item.y);
+        const int tend = min(lstart[leaf + 1], first + kP2PItem);
+        int ti[TPL];
+        int4 t[TPL];
+        float sum[TPL][4], comp[TPL][4];
+#pragma unroll
+        for (int u = 0; u < TPL; ++u) {
+            ti[u] = first + 64 * u + lane;
+            t[u] = loc[ti[u] < tend ? ti[u] : first];
 #pragma unroll
             for (int k = 0; k < 4; ++k) sum[u][k] = comp[u][k] = 0.f;
         }
         for (int s = -1; s < kNbrSlots; ++s) {
-            const int nb = s < 0 ? leaf : slots[static_cast<int64_t>(leaf) * kNbrSlots + s];
+            const int nb = __builtin_amdgcn_readfirstlane(s < 0 ? leaf : slots[static_cast<int64_t>(leaf) * kNbrSlots + s]);
             if (nb < 0) continue;
-            float c[3];
+            int c[3];
             slot_offset(s, c);
-            float th[2][3];
+            int tc[TPL][3];
 #pragma unroll
-            for (int u = 0; u < 2; ++u)
-#pragma unroll
-                for (int d = 0; d < 3; ++d) th[u][d] = t[u][d] - c[d];
-            const int j0 = lstart[nb], j1 = lstart[nb + 1];
-            for (int jc = j0; jc < j1; jc += 64) {
-                __syncthreads();  // the previous chunk is consumed
-                if (jc + lane < j1) {
-                    sh[lane] = hi[jc + lane];
-                    sl[lane] = lo[jc + lane];
-                }
-                __syncthreads();
-                const int m = min(64, j1 - jc);
-                for (int jb = 0; jb < m; jb += kKahanBlock) {
-                    float part[2][4] = {{0.f, 0.f, 0.f, 0.f}, {0.f, 0.f, 0.f, 0.f}};
-                    const int je = min(m, jb + kKahanBlock);
-                    for (int jj = jb; jj < je; ++jj) {
-                        const float4 a = sh[jj], b = sl[jj];
-#pragma unroll
-                        for (int u = 0; u < 2; ++u) pair_f32(th[u], tl[u], a, b, jc + jj == ti[u], part[u]);
-                    }
-#pragma unroll
-                    for (int u = 0; u < 2; ++u)
-#pragma unroll
-                        for (int k = 0; k < 4; ++k) kahan_add(sum[u][k], comp[u][k], part[u][k]);
-                }
+            for (int u = 0; u < TPL; ++u) {
+                tc[u][0] = t[u].x - c[0];
+                tc[u][1] = t[u].y - c[1];
+                tc[u][2] = t[u].z - c[2];
             }
+            const int j0 = __builtin_amdgcn_readfirstlane(lstart[nb]);
+            const int j1 = __builtin_amdgcn_readfirstlane(lstart[nb + 1]);
+            if (s < 0)
+                p2p_leaf_sources<TPL, true>(loc, j0, j1, tc, ti, sum, comp);
+            else
+                p2p_leaf_sources<TPL, false>(loc, j0, j1, tc, ti, sum, comp);
         }
 #pragma unroll
-        for (int u = 0; u < 2; ++u) {
-            if (!ok[u]) continue;
-            const int i = ti[u];
-            const double qf = sp[3 * n + i] * inv_w * inv_w;
-            res[i] = (static_cast<double>(sum[u][0]) - static_cast<double>(comp[u][0])) * inv_w;
-            res[n + i] = (static_cast<double>(sum[u][1]) - static_cast<double>(comp[u][1])) * qf;
-            res[2 * n + i] = (static_cast<double>(sum[u][2]) - static_cast<double>(comp[u][2])) * qf;
-            res[3 * n + i] = (static_cast<double>(sum[u][3]) - static_cast<double>(comp[u][3])) * qf;
-        }
+        for (int u = 0; u < TPL; ++u)
+            if (ti[u] < tend) write_p2p(n, ti[u], sum[u], comp[u], sp[3 * n + ti[u]], inv_u, res);
     }
 }
 
@@ -663,28 +686,30 @@ int run_m2l_mixed(Context* c) {
 
 int run_p2p_mixed(Context* c) {
     const int L = c->height() - 1;
-    HFMM_TRY(ensure(c, c->d_loc, sizeof(float) * 8 * static_cast<size_t>(c->n)));
+    // kKahanBlock padding entries after the particles: read by the masked tail blocks
+    HFMM_TRY(ensure(c, c->d_loc, sizeof(int4) * static_cast<size_t>(c->n + kKahanBlock)));
     const Box bx = make_box(c);
-    float4* hi = c->d_loc.as<float4>();
-    float4* lo = hi + c->n;
+    int4* loc = c->d_loc.as<int4>();
+    HFMM_HIP(c, hipMemsetAsync(loc + c->n, 0, sizeof(int4) * kKahanBlock, c->stream));
     prof_begin(c, "local_coords");
     hipLaunchKernelGGL(k_local_coords, dim3(blocks_of(c->n)), dim3(256), 0, c->stream, c->n, c->d_pleaf.as<int32_t>(),
-                       c->d_cellkey.as<uint64_t>() + c->lvl_off[L], bx, c->d_sorted.as<double>(), hi, lo);
+                       c->d_cellkey.as<uint64_t>() + c->lvl_off[L], bx, c->d_sorted.as<double>(), loc);
     prof_end(c);
     HFMM_TRY(check_launch(c, "local_coords"));
-    const double inv_w = 1. / bx.leafw;
+    const double inv_u = kUnitsPerLeaf / bx.leafw;
     prof_begin(c, "p2p_mixed");
     hipLaunchKernelGGL(k_p2p_mixed, dim3(blocks_of(c->n)), dim3(256), 0, c->stream, c->n, c->d_pleaf.as<int32_t>(),
-                       c->d_leafstart.as<int32_t>(), c->d_p2p.as<int32_t>(), hi, lo, c->d_sorted.as<double>(), inv_w,
+                       c->d_leafstart.as<int32_t>(), c->d_p2p.as<int32_t>(), loc, c->d_sorted.as<double>(), inv_u,
                        c->d_res_sorted.as<double>());
     prof_end(c);
     HFMM_TRY(check_launch(c, "p2p_mixed"));
-    prof_begin(c, "p2p_mixed_tiled");
-    hipLaunchKernelGGL(k_p2p_mixed_tiled, dim3(p2p_tiled_grid(c)), dim3(64), 0, c->stream, c->d_p2pitems.as<int2>(),
-                       p2p_item_count(c), c->d_leafstart.as<int32_t>(), c->d_p2p.as<int32_t>(), hi, lo,
-                       c->d_sorted.as<double>(), c->n, inv_w, c->d_res_sorted.as<double>());
+    const unsigned grid = (p2p_tiled_grid(c) + 3) / 4;
+    prof_begin(c, "p2p_mixed_big");
+    hipLaunchKernelGGL(k_p2p_mixed_big<kP2PItem / 64>, dim3(grid), dim3(256), 0, c->stream, c->d_p2pitems.as<int2>(),
+                       p2p_item_count(c), c->d_leafstart.as<int32_t>(), c->d_p2p.as<int32_t>(), loc,
+                       c->d_sorted.as<double>(), c->n, inv_u, c->d_res_sorted.as<double>());
     prof_end(c);
-    return check_launch(c, "p2p_mixed_tiled");
+    return check_launch(c, "p2p_mixed_big");
 }
 
 int selftest_mfma_bf16(Context* c, float* out) {
