@@ -167,7 +167,7 @@ def run_form(parts, h, p, precision, steps, warmup):
 def roofline(engine, kernel_ms, launches, h, p, mixed, workload):
     """Roofline of the kernel with the largest time share."""
     m2l_name = "m2l_mixed" if mixed else "m2l"
-    p2p_small, p2p_big = ("p2p_mixed", "p2p_mixed_big") if mixed else ("p2p", "p2p_tiled")
+    p2p_small, p2p_big = ("p2p_mixed", "p2p_mixed_big") if mixed else ("p2p", "p2p_big")
     top = max(kernel_ms, key=kernel_ms.get)
     if top == m2l_name:
         flops, n_int = algorithmic_m2l_flops(engine, h, p)
@@ -182,7 +182,7 @@ def roofline(engine, kernel_ms, launches, h, p, mixed, workload):
         work = 34.0 * pairs
         peak = PEAK["fp32_vector"] if mixed else PEAK["fp64_vector"]
         kname = {"p2p_mixed": "k_p2p_mixed", "p2p_mixed_big": "k_p2p_mixed_big", "p2p": "k_p2p_f64",
-                 "p2p_tiled": "k_p2p_f64_tiled"}[top]
+                 "p2p_big": "k_p2p_f64_big"}[top]
         bound = "valu"
         extra = {"pairs": pairs, "work": "34 flops per target-source pair"}
     else:

@@ -295,7 +295,7 @@ __global__ __launch_bounds__(256) void k_p2p_f64(int64_t n, const int32_t* __res
     const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const int li = pleaf[i];
-    if (lstart[li + 1] - lstart[li] > kP2PBig) return;  // k_p2p_f64_tiled
+    if (lstart[li + 1] - lstart[li] > kP2PBig) return;  // k_p2p_f64_big
     const double tx = sp[i], ty = sp[n + i], tz = sp[2 * n + i], tq = sp[3 * n + i];
     double pot = 0., fx = 0., fy = 0., fz = 0.;
     for (int s = -1; s < kNbrSlots; ++s) {
@@ -320,63 +320,98 @@ __global__ __launch_bounds__(256) void k_p2p_f64(int64_t n, const int32_t* __res
     res[3 * n + i] = fz * tq;
 }
 
-// P2P of the big leaves (more than kP2PBig particles): one wave per work item
-// of up to kP2PItem consecutive targets of one leaf, two per lane; the sources
-// of each neighbour leaf pass through LDS 64 at a time and every lane reads
-// them as broadcasts.  Same pair arithmetic and summation order as k_p2p_f64.
-__global__ __launch_bounds__(64) void k_p2p_f64_tiled(const int2* __restrict__ items, const int32_t* __restrict__ nitems,
-                                                     const int32_t* __restrict__ lstart,
-                                                     const int32_t* __restrict__ slots, const double* __restrict__ sp,
-                                                     int64_t n, double* __restrict__ res) {
-    __shared__ double4 src[64];
-    const int lane = threadIdx.x;
+// P2P of the big leaves (more than kP2PBig particles).  Each wave takes work
+// items of kP2PItem consecutive targets of one leaf (two per lane) and walks
+// the 27 neighbour leaves in blocks of 8 sources.  Every source address is
+// wave-uniform, so the sources arrive through scalar loads into SGPRs and
+// feed the FP64 arithmetic as scalar operands (no LDS, no barriers).  1/r is
+// v_rsq_f64 refined by two Newton steps, y <- y (3 - r^2 y^2) / 2.  A block
+// running past the leaf's end masks the extra sources (the sorted arrays are
+// padded, so the reads stay in bounds).  Sources are summed in the same order
+// as k_p2p_f64.
+constexpr int kP2PBlock64 = 8;
+
+__device__ __forceinline__ double rsqrt_f64(double r2) {
+    double y = __builtin_amdgcn_rsq(r2);
+    y = y * fma(-0.5 * r2, y * y, 1.5);
+    y = y * fma(-0.5 * r2, y * y, 1.5);
+    return y;
+}
+
+template <bool MASK>
+__device__ __forceinline__ void pair_f64(const double (&t)[3], double sx, double sy, double sz, double sq, bool skip,
+                                         double (&acc)[4]) {
+    const double dx = t[0] - sx, dy = t[1] - sy, dz = t[2] - sz;
+    double inv = rsqrt_f64(dx * dx + dy * dy + dz * dz);
+    if (MASK) inv = skip ? 0. : inv;
+    const double qi = sq * inv;
+    const double qi3 = qi * inv * inv;
+    acc[0] += qi;
+    acc[1] -= qi3 * dx;
+    acc[2] -= qi3 * dy;
+    acc[3] -= qi3 * dz;
+}
+
+template <bool SELF>
+__device__ __forceinline__ void p2p_f64_leaf(const double* __restrict__ sp, int64_t n, int j0, int j1,
+                                             const double (&t)[2][3], const int (&ti)[2], double (&acc)[2][4]) {
+    int jb = j0;
+    for (; jb + kP2PBlock64 <= j1; jb += kP2PBlock64) {
+#pragma unroll
+        for (int jj = 0; jj < kP2PBlock64; ++jj) {
+            const int j = jb + jj;
+            const double sx = sp[j], sy = sp[n + j], sz = sp[2 * n + j], sq = sp[3 * n + j];
+#pragma unroll
+            for (int u = 0; u < 2; ++u) pair_f64<SELF>(t[u], sx, sy, sz, sq, j == ti[u], acc[u]);
+        }
+    }
+    if (jb < j1) {
+#pragma unroll
+        for (int jj = 0; jj < kP2PBlock64; ++jj) {
+            const int j = jb + jj;
+            const double sx = sp[j], sy = sp[n + j], sz = sp[2 * n + j], sq = sp[3 * n + j];
+            const bool past = j >= j1;
+#pragma unroll
+            for (int u = 0; u < 2; ++u) pair_f64<true>(t[u], sx, sy, sz, sq, past || (SELF && j == ti[u]), acc[u]);
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void k_p2p_f64_big(const int2* __restrict__ items, const int32_t* __restrict__ nitems,
+                                                    const int32_t* __restrict__ lstart,
+                                                    const int32_t* __restrict__ slots, const double* __restrict__ sp,
+                                                    int64_t n, double* __restrict__ res) {
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int total = *nitems;
-    for (int it = blockIdx.x; it < total; it += gridDim.x) {
+    for (int it = blockIdx.x * 4 + wave; it < total; it += gridDim.x * 4) {
         const int2 item = items[it];
-        const int leaf = item.x;
-        const int tend = min(lstart[leaf + 1], item.y + kP2PItem);
+        const int leaf = __builtin_amdgcn_readfirstlane(item.x);
+        const int first = __builtin_amdgcn_readfirstlane(item.y);
+        const int tend = min(lstart[leaf + 1], first + kP2PItem);
         int ti[2];
-        bool ok[2];
         double t[2][3], acc[2][4];
 #pragma unroll
         for (int u = 0; u < 2; ++u) {
-            ti[u] = item.y + 64 * u + lane;
-            ok[u] = ti[u] < tend;
-            const int tt = ok[u] ? ti[u] : item.y;
+            ti[u] = first + 64 * u + lane;
+            const int tt = ti[u] < tend ? ti[u] : first;
             for (int d = 0; d < 3; ++d) t[u][d] = sp[d * n + tt];
             for (int k = 0; k < 4; ++k) acc[u][k] = 0.;
         }
         for (int s = -1; s < kNbrSlots; ++s) {
-            const int nb = s < 0 ? leaf : slots[static_cast<int64_t>(leaf) * kNbrSlots + s];
+            const int nb =
+                __builtin_amdgcn_readfirstlane(s < 0 ? leaf : slots[static_cast<int64_t>(leaf) * kNbrSlots + s]);
             if (nb < 0) continue;
-            const int j0 = lstart[nb], j1 = lstart[nb + 1];
-            for (int jc = j0; jc < j1; jc += 64) {
-                __syncthreads();  // the previous chunk is consumed
-                if (jc + lane < j1) {
-                    const int j = jc + lane;
-                    src[lane] = make_double4(sp[j], sp[n + j], sp[2 * n + j], sp[3 * n + j]);
-                }
-                __syncthreads();
-                const int m = min(64, j1 - jc);
-                for (int jj = 0; jj < m; ++jj) {
-                    const double4 sj = src[jj];
-#pragma unroll
-                    for (int u = 0; u < 2; ++u) {
-                        if (jc + jj == ti[u]) continue;
-                        const double dx = t[u][0] - sj.x, dy = t[u][1] - sj.y, dz = t[u][2] - sj.z;
-                        const double inv = 1.0 / sqrt(dx * dx + dy * dy + dz * dz);
-                        const double inv3 = inv * inv * inv;
-                        acc[u][0] += inv * sj.w;
-                        acc[u][1] += -inv3 * dx * sj.w;
-                        acc[u][2] += -inv3 * dy * sj.w;
-                        acc[u][3] += -inv3 * dz * sj.w;
-                    }
-                }
-            }
+            const int j0 = __builtin_amdgcn_readfirstlane(lstart[nb]);
+            const int j1 = __builtin_amdgcn_readfirstlane(lstart[nb + 1]);
+            if (s < 0)
+                p2p_f64_leaf<true>(sp, n, j0, j1, t, ti, acc);
+            else
+                p2p_f64_leaf<false>(sp, n, j0, j1, t, ti, acc);
         }
 #pragma unroll
         for (int u = 0; u < 2; ++u) {
-            if (!ok[u]) continue;
+            if (ti[u] >= tend) continue;
             const double tq = sp[3 * n + ti[u]];
             res[ti[u]] = acc[u][0];
             res[n + ti[u]] = acc[u][1] * tq;
@@ -820,12 +855,12 @@ int run_p2p(Context* c) {
                        c->d_res_sorted.as<double>());
     prof_end(c);
     HFMM_TRY(check_launch(c, "p2p"));
-    prof_begin(c, "p2p_tiled");
-    hipLaunchKernelGGL(k_p2p_f64_tiled, dim3(p2p_tiled_grid(c)), dim3(64), 0, c->stream, c->d_p2pitems.as<int2>(),
-                       p2p_item_count(c), c->d_leafstart.as<int32_t>(), c->d_p2p.as<int32_t>(),
-                       c->d_sorted.as<double>(), c->n, c->d_res_sorted.as<double>());
+    prof_begin(c, "p2p_big");
+    hipLaunchKernelGGL(k_p2p_f64_big, dim3((p2p_tiled_grid(c) + 3) / 4), dim3(256), 0, c->stream,
+                       c->d_p2pitems.as<int2>(), p2p_item_count(c), c->d_leafstart.as<int32_t>(),
+                       c->d_p2p.as<int32_t>(), c->d_sorted.as<double>(), c->n, c->d_res_sorted.as<double>());
     prof_end(c);
-    return check_launch(c, "p2p_tiled");
+    return check_launch(c, "p2p_big");
 }
 
 int zero_results(Context* c) {

@@ -19,6 +19,7 @@ constexpr int kFarSlots = 189;  // M2L slots per cell
 constexpr int kTileTargets = 16;  // targets per M2L tile (the MFMA N dimension)
 constexpr int kP2PBig = 64;    // leaves with more particles run P2P through LDS-staged sources
 constexpr int kP2PItem = 128;  // targets per P2P work item of a big leaf (two per lane of a wave)
+constexpr int kSortedPad = 16; // padding doubles after the sorted particle SoA
 
 // Grow-only device allocation.
 struct DevBuf {

@@ -480,7 +480,9 @@ int build_tree_device(Context* c) {
     HFMM_TRY(radix_sort(c, keys, vals, c->d_keys2.as<uint64_t>(), c->d_vals2.as<int32_t>(), n, 3 * L));
 
     // 3. sorted particle SoA
-    HFMM_TRY(ensure(c, c->d_sorted, sizeof(double) * 4 * n));
+    // kSortedPad doubles after the four SoA arrays: the big-leaf P2P reads whole
+    // blocks of sources past a leaf's end (and masks them)
+    HFMM_TRY(ensure(c, c->d_sorted, sizeof(double) * (4 * n + kSortedPad)));
     HFMM_TRY(ensure(c, c->d_perm, sizeof(int32_t) * n));
     HFMM_HIP(c, hipMemcpyAsync(c->d_perm.ptr, vals, sizeof(int32_t) * n, hipMemcpyDeviceToDevice, c->stream));
     prof_begin(c, "gather");
