@@ -57,11 +57,12 @@ class HipLibraryMissing(RuntimeError):
 
 
 def lib(path: str | None = None):
-    """Loads the HIP engine (no fallback)."""
+    """Loads the HIP engine (no fallback): `path`, else $SCALFMM_HIP_LIB, else
+    the in-tree build."""
     global _lib
     if _lib is not None:
         return _lib
-    p = path or LIB_PATH
+    p = path or os.environ.get("SCALFMM_HIP_LIB") or LIB_PATH
     if not os.path.exists(p):
         raise HipLibraryMissing(f"HIP engine not built: {p} is missing (run __graft_entry__.build())")
     L = C.CDLL(p)
