@@ -183,14 +183,29 @@ struct MixState {
     f4 acc[NTW][COLS];
 };
 
+// Runs the slots [i0, i1) of the tile's (rank-ordered) slot list, all with
+// KT rank tiles.  Per slot:
+//   GEMM1   each wave multiplies its quarter of the node dimension into a
+//           partial C tile, stored as [wave][target][row] floats;
+//   barrier
+//   reduce  wave w sums the four partials of column tile w once, splits the
+//           sums into three bf16 parts and stores them as B fragments;
+//   barrier
+//   GEMM2   every wave reads the B fragments (16-byte LDS loads) for its row
+//           tiles of U.
+// The next slot's GEMM1 may overwrite the partials as soon as the second
+// barrier is passed (every wave has finished reducing), and the next
+// reduction writes the B fragments only after the next first barrier (every
+// wave has finished its GEMM2).
 template <int P, int COLS, int KT>
 __device__ __forceinline__ void mix_segment(MixState<P, COLS>& st, int i0, int i1, const int16_t* s_slot,
                                             const int64_t* s_off, const int32_t* lslots, const int (&tgt)[COLS],
                                             int (&nsrc)[COLS], const uint16_t* __restrict__ Mbl,
                                             const uint16_t* __restrict__ Mbz, int64_t plane,
-                                            const uint16_t* __restrict__ opm, float* Cp, int LD, int crows, int wave,
-                                            int col, int kq, int kb0, int kb1) {
+                                            const uint16_t* __restrict__ opm, float* Cp, bf8* Bs, int LD, int CR,
+                                            int wave, int lane, int col, int kq, int kb0, int kb1) {
     constexpr int W = kM2LWaves;
+    static_assert(COLS == W, "one column tile of C per wave in the reduction");
     constexpr int NTW = MixState<P, COLS>::NTW;
     constexpr int TC = 16 * COLS;
     constexpr int KP16 = 16 * KT;       // GEMM1 rows: rank padded to 16
@@ -212,7 +227,6 @@ __device__ __forceinline__ void mix_segment(MixState<P, COLS>& st, int i0, int i
         if (__ballot(any) == 0) continue;  // same answer in every wave
         const uint16_t* VT = opm + s_off[i];
         const uint16_t* U = VT + 3 * static_cast<int64_t>(KP16) * LD;
-        __syncthreads();  // every wave is past the previous slot's GEMM2 (C is reused)
         {
             f4 c1[KT][COLS];
 #pragma unroll
@@ -237,34 +251,52 @@ __device__ __forceinline__ void mix_segment(MixState<P, COLS>& st, int i0, int i
                         c1[rt][ct] = mma6(ah, am, al, bh[ct], bm[ct], bl[ct], c1[rt][ct]);
                 }
             }
-            float* Cw = Cp + static_cast<size_t>(wave) * crows * TC;
+            // partial C as [wave][target][row]: lane (col, kq) holds rows
+            // 16 rt + 4 kq .. +3 of target 16 ct + col
+            float* Cw = Cp + static_cast<size_t>(wave) * TC * CR;
 #pragma unroll
             for (int rt = 0; rt < KT; ++rt)
 #pragma unroll
                 for (int ct = 0; ct < COLS; ++ct)
-#pragma unroll
-                    for (int r = 0; r < 4; ++r) Cw[(16 * rt + 4 * kq + r) * TC + 16 * ct + col] = c1[rt][ct][r];
+                    *reinterpret_cast<f4*>(Cw + (16 * ct + col) * CR + 16 * rt + 4 * kq) = c1[rt][ct];
         }
         __syncthreads();
         // Rows 16 KT .. KP32 of C are zero: the segments run with growing KT,
         // so no earlier slot wrote them, and the LDS was cleared at the start.
 #pragma unroll
         for (int kb = 0; kb < KB32; ++kb) {
+            const float* q = Cp + (16 * wave + col) * CR + 32 * kb + 8 * kq;
+            f4 v0 = *reinterpret_cast<const f4*>(q), v1 = *reinterpret_cast<const f4*>(q + 4);
+#pragma unroll
+            for (int w = 1; w < W; ++w) {
+                v0 += *reinterpret_cast<const f4*>(q + w * TC * CR);
+                v1 += *reinterpret_cast<const f4*>(q + w * TC * CR + 4);
+            }
+            bf8 h8, m8, l8;
+#pragma unroll
+            for (int e = 0; e < 8; ++e) {
+                __bf16 h, m, l;
+                split3(e < 4 ? v0[e] : v1[e - 4], h, m, l);
+                h8[e] = h;
+                m8[e] = m;
+                l8[e] = l;
+            }
+            bf8* b = Bs + ((wave * 2 + kb) * 3) * 64 + lane;
+            b[0] = h8;
+            b[64] = m8;
+            b[128] = l8;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int kb = 0; kb < KB32; ++kb) {
             bf8 bh[COLS], bm[COLS], bl[COLS];
 #pragma unroll
-            for (int ct = 0; ct < COLS; ++ct)
-#pragma unroll
-                for (int j = 0; j < 8; ++j) {
-                    const int idx = (32 * kb + 8 * kq + j) * TC + 16 * ct + col;
-                    float v = Cp[idx];
-#pragma unroll
-                    for (int w = 1; w < W; ++w) v += Cp[static_cast<size_t>(w) * crows * TC + idx];
-                    __bf16 h, m, l;
-                    split3(v, h, m, l);
-                    bh[ct][j] = h;
-                    bm[ct][j] = m;
-                    bl[ct][j] = l;
-                }
+            for (int ct = 0; ct < COLS; ++ct) {
+                const bf8* b = Bs + ((ct * 2 + kb) * 3) * 64 + lane;
+                bh[ct] = b[0];
+                bm[ct] = b[64];
+                bl[ct] = b[128];
+            }
 #pragma unroll
             for (int t = 0; t < NTW; ++t) {
                 // a wave with fewer real row tiles computes throw-away ones on row 0
@@ -285,7 +317,7 @@ __global__ __launch_bounds__(256) void k_m2l_mixed(M2LLevels lv, const int32_t* 
                                                   const int16_t* __restrict__ slotord, const int32_t* __restrict__ slotseg,
                                                   const int64_t* __restrict__ posoff, const uint16_t* __restrict__ opm,
                                                   const uint16_t* __restrict__ Mb, int64_t plane, int64_t zero_cell,
-                                                  double* __restrict__ L, int LD, int crows) {
+                                                  double* __restrict__ L, int LD, int CR) {
     constexpr int NN = P * P * P;
     constexpr int W = kM2LWaves;
     constexpr int NT = MixState<P, COLS>::NT;
@@ -293,7 +325,10 @@ __global__ __launch_bounds__(256) void k_m2l_mixed(M2LLevels lv, const int32_t* 
     constexpr int TC = 16 * COLS;
     __shared__ int16_t s_slot[kFarSlots];
     __shared__ int64_t s_off[kFarSlots];
-    extern __shared__ float fsm[];  // partial C tiles [W][crows][TC]
+    // partial C tiles [W][TC][CR] floats (CR = rows padded by 4 against bank
+    // conflicts), then the B fragments [COLS][2][3][64 lanes] x 16 bytes
+    extern __shared__ float fsm[];
+    bf8* Bs = reinterpret_cast<bf8*>(fsm + W * TC * CR);
     const int tid = threadIdx.x;
     const int lane = tid & 63;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
@@ -319,7 +354,7 @@ __global__ __launch_bounds__(256) void k_m2l_mixed(M2LLevels lv, const int32_t* 
         s_slot[i] = slotord[(e * kFarSlots + i) * 2];
         s_off[i] = posoff[slotord[(e * kFarSlots + i) * 2 + 1]];
     }
-    for (int i = tid; i < W * crows * TC; i += 64 * W) fsm[i] = 0.f;
+    for (int i = tid; i < W * TC * CR; i += 64 * W) fsm[i] = 0.f;
     int seg[5];
 #pragma unroll
     for (int q = 0; q < 5; ++q) seg[q] = slotseg[e * 5 + q];
@@ -344,14 +379,14 @@ __global__ __launch_bounds__(256) void k_m2l_mixed(M2LLevels lv, const int32_t* 
     for (int t = 0; t < NTW; ++t)
 #pragma unroll
         for (int ct = 0; ct < COLS; ++ct) st.acc[t][ct] = f4{0.f, 0.f, 0.f, 0.f};
-    mix_segment<P, COLS, 1>(st, seg[0], seg[1], s_slot, s_off, lslots, tgt, nsrc, Mbl, Mbz, plane, opm, fsm, LD,
-                            crows, wave, col, kq, kb0, kb1);
-    mix_segment<P, COLS, 2>(st, seg[1], seg[2], s_slot, s_off, lslots, tgt, nsrc, Mbl, Mbz, plane, opm, fsm, LD,
-                            crows, wave, col, kq, kb0, kb1);
-    mix_segment<P, COLS, 3>(st, seg[2], seg[3], s_slot, s_off, lslots, tgt, nsrc, Mbl, Mbz, plane, opm, fsm, LD,
-                            crows, wave, col, kq, kb0, kb1);
-    mix_segment<P, COLS, 4>(st, seg[3], seg[4], s_slot, s_off, lslots, tgt, nsrc, Mbl, Mbz, plane, opm, fsm, LD,
-                            crows, wave, col, kq, kb0, kb1);
+    mix_segment<P, COLS, 1>(st, seg[0], seg[1], s_slot, s_off, lslots, tgt, nsrc, Mbl, Mbz, plane, opm, fsm, Bs,
+                            LD, CR, wave, lane, col, kq, kb0, kb1);
+    mix_segment<P, COLS, 2>(st, seg[1], seg[2], s_slot, s_off, lslots, tgt, nsrc, Mbl, Mbz, plane, opm, fsm, Bs,
+                            LD, CR, wave, lane, col, kq, kb0, kb1);
+    mix_segment<P, COLS, 3>(st, seg[2], seg[3], s_slot, s_off, lslots, tgt, nsrc, Mbl, Mbz, plane, opm, fsm, Bs,
+                            LD, CR, wave, lane, col, kq, kb0, kb1);
+    mix_segment<P, COLS, 4>(st, seg[3], seg[4], s_slot, s_off, lslots, tgt, nsrc, Mbl, Mbz, plane, opm, fsm, Bs,
+                            LD, CR, wave, lane, col, kq, kb0, kb1);
     const double scale = lv.scale[li];
 #pragma unroll
     for (int ct = 0; ct < COLS; ++ct) {
@@ -592,8 +627,8 @@ template <int P, int COLS>
 int m2l_mixed_impl(Context* c) {
     int tiles = 0;
     const M2LLevels lv = make_levels(c, 16 * COLS, tiles);
-    const int crows = (c->kpad_max + 31) & ~31;
-    const size_t shmem = sizeof(float) * kM2LWaves * static_cast<size_t>(crows) * 16 * COLS;
+    const int CR = ((c->kpad_max + 31) & ~31) + 4;
+    const size_t shmem = sizeof(float) * kM2LWaves * static_cast<size_t>(CR) * 16 * COLS + 16 * COLS * 2 * 3 * 64;
     HFMM_HIP(c, hipFuncSetAttribute(reinterpret_cast<const void*>(&k_m2l_mixed<P, COLS>),
                                     hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(shmem)));
     const int64_t plane = (c->total_cells + 1) * static_cast<int64_t>(c->LD);
@@ -602,7 +637,7 @@ int m2l_mixed_impl(Context* c) {
                        c->stream, lv, c->d_m2l.as<int32_t>(), c->d_order.as<int32_t>(), c->d_pcount.as<int32_t>(),
                        c->d_slotord.as<int16_t>(), c->d_slotseg.as<int32_t>(), c->d_posoff.as<int64_t>(),
                        c->d_opm.as<uint16_t>(), c->d_Mb.as<uint16_t>(), plane, c->total_cells, c->d_L.as<double>(),
-                       c->LD, crows);
+                       c->LD, CR);
     prof_end(c);
     HFMM_TRY(check_launch(c, "m2l_mixed"));
     // the M2L targets: every cell of levels 2..h-1, contiguous in the cell arrays
