@@ -183,6 +183,32 @@ struct MixState {
     f4 acc[NTW][COLS];
 };
 
+// Multipole fragments (three planes, COLS column tiles) of node block kb.
+template <int COLS>
+__device__ __forceinline__ void load_mfrag(bf8 (&b)[3][COLS], const uint16_t* const (&Mrow)[COLS], int64_t plane,
+                                           int kb) {
+#pragma unroll
+    for (int ct = 0; ct < COLS; ++ct) {
+        const uint16_t* p = Mrow[ct] + 32 * kb;
+        b[0][ct] = ld8(p);
+        b[1][ct] = ld8(p + plane);
+        b[2][ct] = ld8(p + 2 * plane);
+    }
+}
+
+// GEMM1 on node block kb: the KT rank rows of VT against the multipole fragments.
+template <int COLS, int KT, int KP16>
+__device__ __forceinline__ void g1_block(f4 (&c1)[KT][COLS], const bf8 (&b)[3][COLS], const uint16_t* VT, int LD,
+                                         int col, int kq, int kb) {
+#pragma unroll
+    for (int rt = 0; rt < KT; ++rt) {
+        const uint16_t* a = VT + static_cast<int64_t>(16 * rt + col) * LD + 32 * kb + 8 * kq;
+        const bf8 ah = ld8(a), am = ld8(a + KP16 * LD), al = ld8(a + 2 * KP16 * LD);
+#pragma unroll
+        for (int ct = 0; ct < COLS; ++ct) c1[rt][ct] = mma6(ah, am, al, b[0][ct], b[1][ct], b[2][ct], c1[rt][ct]);
+    }
+}
+
 // Runs the slots [i0, i1) of the tile's (rank-ordered) slot list, all with
 // KT rank tiles.  Per slot:
 //   GEMM1   each wave multiplies its quarter of the node dimension into a
@@ -233,22 +259,16 @@ __device__ __forceinline__ void mix_segment(MixState<P, COLS>& st, int i0, int i
             for (int rt = 0; rt < KT; ++rt)
 #pragma unroll
                 for (int ct = 0; ct < COLS; ++ct) c1[rt][ct] = f4{0.f, 0.f, 0.f, 0.f};
-            for (int kb = kb0; kb < kb1; ++kb) {
-                bf8 bh[COLS], bm[COLS], bl[COLS];
-#pragma unroll
-                for (int ct = 0; ct < COLS; ++ct) {
-                    const uint16_t* p = Mrow[ct] + 32 * kb;
-                    bh[ct] = ld8(p);
-                    bm[ct] = ld8(p + plane);
-                    bl[ct] = ld8(p + 2 * plane);
-                }
-#pragma unroll
-                for (int rt = 0; rt < KT; ++rt) {
-                    const uint16_t* a = VT + static_cast<int64_t>(16 * rt + col) * LD + 32 * kb + 8 * kq;
-                    const bf8 ah = ld8(a), am = ld8(a + KP16 * LD), al = ld8(a + 2 * KP16 * LD);
-#pragma unroll
-                    for (int ct = 0; ct < COLS; ++ct)
-                        c1[rt][ct] = mma6(ah, am, al, bh[ct], bm[ct], bl[ct], c1[rt][ct]);
+            // the multipole fragments of block kb + 1 are requested before the
+            // MFMAs of block kb (their latency overlaps the matrix work)
+            bf8 b3[2][3][COLS];
+            if (kb0 < kb1) load_mfrag<COLS>(b3[0], Mrow, plane, kb0);
+            for (int kb = kb0; kb < kb1; kb += 2) {
+                if (kb + 1 < kb1) load_mfrag<COLS>(b3[1], Mrow, plane, kb + 1);
+                g1_block<COLS, KT, KP16>(c1, b3[0], VT, LD, col, kq, kb);
+                if (kb + 1 < kb1) {
+                    if (kb + 2 < kb1) load_mfrag<COLS>(b3[0], Mrow, plane, kb + 2);
+                    g1_block<COLS, KT, KP16>(c1, b3[1], VT, LD, col, kq, kb + 1);
                 }
             }
             // partial C as [wave][target][row]: lane (col, kq) holds rows
