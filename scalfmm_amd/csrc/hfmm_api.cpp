@@ -154,20 +154,31 @@ static int upload_operators(Context* c) {
     }
     HFMM_TRY(ensure(c, c->d_slotpos, sizeof(int16_t) * sp.size()));
     HFMM_HIP(c, hipMemcpy(c->d_slotpos.ptr, sp.data(), sizeof(int16_t) * sp.size(), hipMemcpyHostToDevice));
-    // slots of every parity ordered by the rank tiles of their operator (stable),
-    // so the M2L kernel runs one specialised loop per rank-tile count
+    // slots of every parity ordered by the rank tiles of their operator, so the
+    // M2L kernels run one specialised loop per rank-tile count.  Inside a
+    // rank-tile group the slots are ordered by the parity class of their
+    // offset, then by the offset: a tile's targets are one cell of every other
+    // parent (a lattice of stride 2), so consecutive slots whose offsets differ
+    // by 2 cells read largely the same source expansions from the cache.
     std::vector<int16_t> so(8 * kFarSlots * 2);
     std::vector<int32_t> seg(8 * 5);
     for (int e = 0; e < 8; ++e) {
         int i = 0;
         for (int kt = 1; kt <= 4; ++kt) {
             seg[e * 5 + kt - 1] = i;
+            std::vector<std::pair<int, int>> group;  // (order key, slot)
             for (int j = 0; j < kFarSlots; ++j) {
                 const int pos = sp[e * kFarSlots + j];
                 const int k = op.rank[op.pidx[pos]];
                 if (((k + 15) >> 4) != kt) continue;
-                so[(e * kFarSlots + i) * 2] = static_cast<int16_t>(j);
-                so[(e * kFarSlots + i) * 2 + 1] = static_cast<int16_t>(pos);
+                const int dx = pos / 49, dy = (pos / 7) % 7, dz = pos % 7;  // offsets + 3
+                const int cls = ((dx + 1) & 1) * 4 + ((dy + 1) & 1) * 2 + ((dz + 1) & 1);
+                group.emplace_back(cls * 343 + pos, j);
+            }
+            std::sort(group.begin(), group.end());
+            for (const auto& g : group) {
+                so[(e * kFarSlots + i) * 2] = static_cast<int16_t>(g.second);
+                so[(e * kFarSlots + i) * 2 + 1] = sp[e * kFarSlots + g.second];
                 ++i;
             }
         }
