@@ -196,6 +196,40 @@ __device__ __forceinline__ void load_mfrag(bf8 (&b)[3][COLS], const uint16_t* co
     }
 }
 
+// VT rows of rank tile rt, node block kb, three planes.
+template <int KP16>
+__device__ __forceinline__ void load_vfrag(bf8 (&a)[3], const uint16_t* VT, int LD, int rt, int col, int kq, int kb) {
+    const uint16_t* p = VT + static_cast<int64_t>(16 * rt + col) * LD + 32 * kb + 8 * kq;
+    a[0] = ld8(p);
+    a[1] = ld8(p + KP16 * LD);
+    a[2] = ld8(p + 2 * KP16 * LD);
+}
+
+// GEMM1 on node block kb with the VT rows rolling one rank tile ahead: va[0]
+// holds rank tile 0 of this block on entry and rank tile 0 of block kb + 1 on
+// exit (when `more`).
+template <int COLS, int KT, int KP16>
+__device__ __forceinline__ void g1_step(f4 (&c1)[KT][COLS], const bf8 (&b)[3][COLS], bf8 (&va)[2][3],
+                                        const uint16_t* VT, int LD, int col, int kq, int kb, bool more) {
+#pragma unroll
+    for (int rt = 0; rt < KT; ++rt) {
+        const int cur = rt & 1;
+        if (rt + 1 < KT)
+            load_vfrag<KP16>(va[cur ^ 1], VT, LD, rt + 1, col, kq, kb);
+        else if (more)
+            load_vfrag<KP16>(va[cur ^ 1], VT, LD, 0, col, kq, kb + 1);
+#pragma unroll
+        for (int ct = 0; ct < COLS; ++ct)
+            c1[rt][ct] = mma6(va[cur][0], va[cur][1], va[cur][2], b[0][ct], b[1][ct], b[2][ct], c1[rt][ct]);
+        __builtin_amdgcn_sched_barrier(0);
+    }
+    if (KT & 1) {  // the next block's rank tile 0 landed in va[1]
+        va[0][0] = va[1][0];
+        va[0][1] = va[1][1];
+        va[0][2] = va[1][2];
+    }
+}
+
 // GEMM1 on node block kb: the KT rank rows of VT against the multipole fragments.
 template <int COLS, int KT, int KP16>
 __device__ __forceinline__ void g1_block(f4 (&c1)[KT][COLS], const bf8 (&b)[3][COLS], const uint16_t* VT, int LD,
@@ -259,16 +293,23 @@ __device__ __forceinline__ void mix_segment(MixState<P, COLS>& st, int i0, int i
             for (int rt = 0; rt < KT; ++rt)
 #pragma unroll
                 for (int ct = 0; ct < COLS; ++ct) c1[rt][ct] = f4{0.f, 0.f, 0.f, 0.f};
-            // the multipole fragments of block kb + 1 are requested before the
-            // MFMAs of block kb (their latency overlaps the matrix work)
+            // Software pipeline over (node block, rank tile): the multipole
+            // fragments of block kb + 1 and the VT rows of the next rank tile
+            // are requested before the MFMAs that precede their use, so the
+            // L2 latency overlaps the matrix work.
             bf8 b3[2][3][COLS];
-            if (kb0 < kb1) load_mfrag<COLS>(b3[0], Mrow, plane, kb0);
+            bf8 va[2][3];
+            if (kb0 < kb1) {
+                load_mfrag<COLS>(b3[0], Mrow, plane, kb0);
+                load_vfrag<KP16>(va[0], VT, LD, 0, col, kq, kb0);
+            }
             for (int kb = kb0; kb < kb1; kb += 2) {
-                if (kb + 1 < kb1) load_mfrag<COLS>(b3[1], Mrow, plane, kb + 1);
-                g1_block<COLS, KT, KP16>(c1, b3[0], VT, LD, col, kq, kb);
-                if (kb + 1 < kb1) {
+                const bool more = kb + 1 < kb1;
+                if (more) load_mfrag<COLS>(b3[1], Mrow, plane, kb + 1);
+                g1_step<COLS, KT, KP16>(c1, b3[0], va, VT, LD, col, kq, kb, more);
+                if (more) {
                     if (kb + 2 < kb1) load_mfrag<COLS>(b3[0], Mrow, plane, kb + 2);
-                    g1_block<COLS, KT, KP16>(c1, b3[1], VT, LD, col, kq, kb + 1);
+                    g1_step<COLS, KT, KP16>(c1, b3[1], va, VT, LD, col, kq, kb + 1, kb + 2 < kb1);
                 }
             }
             // partial C as [wave][target][row]: lane (col, kq) holds rows
