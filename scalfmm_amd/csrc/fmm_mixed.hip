@@ -181,6 +181,7 @@ struct MixState {
     static constexpr int NT = (NN + 15) / 16;
     static constexpr int NTW = (NT + kM2LWaves - 1) / kM2LWaves;
     f4 acc[NTW][COLS];
+    f4 comp[COLS];  // Kahan compensation of the first row tile (the lowest modes)
 };
 
 // Multipole fragments (three planes, COLS column tiles) of node block kb.
@@ -348,6 +349,13 @@ __device__ __forceinline__ void mix_segment(MixState<P, COLS>& st, int i0, int i
             b[128] = l8;
         }
         __syncthreads();
+        // The first row tile of every wave (modes 0..63: the mean potential and
+        // the in-plane gradients, the largest coefficients) is accumulated per
+        // slot from zero and added with compensation; the other tiles
+        // accumulate directly.
+        f4 fresh[COLS];
+#pragma unroll
+        for (int ct = 0; ct < COLS; ++ct) fresh[ct] = f4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
         for (int kb = 0; kb < KB32; ++kb) {
             bf8 bh[COLS], bm[COLS], bl[COLS];
@@ -366,8 +374,20 @@ __device__ __forceinline__ void mix_segment(MixState<P, COLS>& st, int i0, int i
                 const bf8 ah = ld8(a), am = ld8(a + static_cast<int64_t>(LD) * KP32),
                           al = ld8(a + 2 * static_cast<int64_t>(LD) * KP32);
 #pragma unroll
-                for (int ct = 0; ct < COLS; ++ct) st.acc[t][ct] = mma6(ah, am, al, bh[ct], bm[ct], bl[ct], st.acc[t][ct]);
+                for (int ct = 0; ct < COLS; ++ct) {
+                    if (t == 0)
+                        fresh[ct] = mma6(ah, am, al, bh[ct], bm[ct], bl[ct], fresh[ct]);
+                    else
+                        st.acc[t][ct] = mma6(ah, am, al, bh[ct], bm[ct], bl[ct], st.acc[t][ct]);
+                }
             }
+        }
+#pragma unroll
+        for (int ct = 0; ct < COLS; ++ct) {
+            const f4 y = fresh[ct] - st.comp[ct];
+            const f4 tt = st.acc[0][ct] + y;
+            st.comp[ct] = (tt - st.acc[0][ct]) - y;
+            st.acc[0][ct] = tt;
         }
     }
 }
@@ -440,6 +460,8 @@ __global__ __launch_bounds__(256) void k_m2l_mixed(M2LLevels lv, const int32_t* 
     for (int t = 0; t < NTW; ++t)
 #pragma unroll
         for (int ct = 0; ct < COLS; ++ct) st.acc[t][ct] = f4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int ct = 0; ct < COLS; ++ct) st.comp[ct] = f4{0.f, 0.f, 0.f, 0.f};
     mix_segment<P, COLS, 1>(st, seg[0], seg[1], s_slot, s_off, lslots, tgt, nsrc, Mbl, Mbz, plane, opm, fsm, Bs,
                             LD, CR, wave, lane, col, kq, kb0, kb1);
     mix_segment<P, COLS, 2>(st, seg[1], seg[2], s_slot, s_off, lslots, tgt, nsrc, Mbl, Mbz, plane, opm, fsm, Bs,
@@ -460,7 +482,9 @@ __global__ __launch_bounds__(256) void k_m2l_mixed(M2LLevels lv, const int32_t* 
                 for (int r = 0; r < 4; ++r) {
                     const int row = 16 * nt + 4 * kq + r;
                     if (row < NN)
-                        Ll[static_cast<int64_t>(tgt[ct]) * LD + row] = scale * static_cast<double>(st.acc[t][ct][r]);
+                        Ll[static_cast<int64_t>(tgt[ct]) * LD + row] =
+                            scale * (t == 0 ? static_cast<double>(st.acc[0][ct][r]) - static_cast<double>(st.comp[ct][r])
+                                            : static_cast<double>(st.acc[t][ct][r]));
                 }
             }
         }
