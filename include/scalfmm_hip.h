@@ -94,6 +94,14 @@ int hfmm_set_particles(hfmm_ctx* ctx, int64_t n, const double* x, const double* 
 int hfmm_set_particles_device(hfmm_ctx* ctx, int64_t n, const double* x, const double* y, const double* z,
                               const double* q);
 
+/* New charges for the particles of the built tree (input order; host array, or
+ * a device pointer that must stay valid until the next evaluation).  The tree
+ * depends only on the positions, so evaluating several charge vectors on one
+ * tree skips the rebuild; the reference's NVALS right-hand sides
+ * (FChebSymKernel.hpp:45-488, template parameter NVALS) serve the same use. */
+int hfmm_set_charges(hfmm_ctx* ctx, const double* q);
+int hfmm_set_charges_device(hfmm_ctx* ctx, const double* q);
+
 /* Builds the octree on the device: leaf coordinates and Morton keys
  * (FCoordinateComputer.hpp:42-54, FTreeCoordinate.hpp:91-116), the sort by key
  * (FGroupTree.hpp:215-219), leaves and parent levels (FGroupTree.hpp:224-360),
@@ -126,6 +134,15 @@ int hfmm_get_results_device(hfmm_ctx* ctx, const double** pot, const double** fx
  * (NULL skips one). */
 int hfmm_direct(hfmm_ctx* ctx, int64_t ntargets, const int64_t* targets, double* pot, double* fx, double* fy,
                 double* fz);
+
+/* The field grad(phi) at every particle (input order): the force without the
+ * target charge, F = q grad(phi) (FChebSymKernel.hpp:432-486 accumulate q times
+ * it).  Needed by the adjoint of the potential: for L(phi), dL/dx_j =
+ * g_j grad(phi)(x_j) + q_j grad(psi)(x_j) with g = dL/dphi and psi the
+ * potential of charges g.  Host arrays (NULL skips one), or device pointers
+ * owned by the context. */
+int hfmm_get_field(hfmm_ctx* ctx, double* ex, double* ey, double* ez);
+int hfmm_get_field_device(hfmm_ctx* ctx, const double** ex, const double** ey, const double** ez);
 
 /* ---- inspection (used by the parity tests) -------------------------------- */
 typedef struct hfmm_tree_info {

@@ -30,6 +30,7 @@ EXPORTS = (
     "hfmm_get_p2p_list", "hfmm_get_m2l_list", "hfmm_get_operator", "hfmm_get_expansions",
     "hfmm_set_profiling", "hfmm_get_kernel_times", "hfmm_reset_kernel_times", "hfmm_selftest_mfma_f64",
     "hfmm_selftest_mfma_bf16", "hfmm_direct",
+    "hfmm_get_field", "hfmm_get_field_device", "hfmm_set_charges", "hfmm_set_charges_device",
     "hfmm_version",
 )
 
@@ -93,6 +94,10 @@ def lib(path: str | None = None):
     L.hfmm_selftest_mfma_f64.argtypes = [vp, dp]
     L.hfmm_selftest_mfma_bf16.argtypes = [vp, C.POINTER(C.c_float)]
     L.hfmm_direct.argtypes = [vp, C.c_int64, C.POINTER(C.c_int64), dp, dp, dp, dp]
+    L.hfmm_get_field.argtypes = [vp, dp, dp, dp]
+    L.hfmm_set_charges.argtypes = [vp, dp]
+    L.hfmm_set_charges_device.argtypes = [vp, vp]
+    L.hfmm_get_field_device.argtypes = [vp, C.POINTER(vp), C.POINTER(vp), C.POINTER(vp)]
     L.hfmm_version.restype = C.c_char_p
     for name in EXPORTS:
         getattr(L, name).restype = getattr(L, name).restype if name in ("hfmm_last_error", "hfmm_version") \

@@ -278,11 +278,10 @@ __global__ __launch_bounds__(64) void k_l2p(int nleaf, const uint64_t* __restric
             gy += ty * sz;
             gz += tp * dsz;
         }
-        const double q = sp[3 * n + e];
         res[e] += ph;
-        res[n + e] += gx * (2. / (b[0] - a[0])) * q;
-        res[2 * n + e] += gy * (2. / (b[1] - a[1])) * q;
-        res[3 * n + e] += gz * (2. / (b[2] - a[2])) * q;
+        res[n + e] += gx * (2. / (b[0] - a[0]));
+        res[2 * n + e] += gy * (2. / (b[1] - a[1]));
+        res[3 * n + e] += gz * (2. / (b[2] - a[2]));
     }
 }
 
@@ -296,7 +295,7 @@ __global__ __launch_bounds__(256) void k_p2p_f64(int64_t n, const int32_t* __res
     if (i >= n) return;
     const int li = pleaf[i];
     if (lstart[li + 1] - lstart[li] > kP2PBig) return;  // k_p2p_f64_big
-    const double tx = sp[i], ty = sp[n + i], tz = sp[2 * n + i], tq = sp[3 * n + i];
+    const double tx = sp[i], ty = sp[n + i], tz = sp[2 * n + i];
     double pot = 0., fx = 0., fy = 0., fz = 0.;
     for (int s = -1; s < kNbrSlots; ++s) {
         const int nb = s < 0 ? li : slots[static_cast<int64_t>(li) * kNbrSlots + s];
@@ -315,9 +314,9 @@ __global__ __launch_bounds__(256) void k_p2p_f64(int64_t n, const int32_t* __res
         }
     }
     res[i] = pot;
-    res[n + i] = fx * tq;
-    res[2 * n + i] = fy * tq;
-    res[3 * n + i] = fz * tq;
+    res[n + i] = fx;
+    res[2 * n + i] = fy;
+    res[3 * n + i] = fz;
 }
 
 // P2P of the big leaves (more than kP2PBig particles).  Each wave takes work
@@ -412,11 +411,10 @@ __global__ __launch_bounds__(256) void k_p2p_f64_big(const int2* __restrict__ it
 #pragma unroll
         for (int u = 0; u < 2; ++u) {
             if (ti[u] >= tend) continue;
-            const double tq = sp[3 * n + ti[u]];
             res[ti[u]] = acc[u][0];
-            res[n + ti[u]] = acc[u][1] * tq;
-            res[2 * n + ti[u]] = acc[u][2] * tq;
-            res[3 * n + ti[u]] = acc[u][3] * tq;
+            res[n + ti[u]] = acc[u][1];
+            res[2 * n + ti[u]] = acc[u][2];
+            res[3 * n + ti[u]] = acc[u][3];
         }
     }
 }
@@ -693,15 +691,22 @@ __global__ void k_mfma_probe(double* out) {
     for (int i = 0; i < 4; ++i) out[(4 * i + l / 16) * 16 + (l % 16)] = c[i];
 }
 
+// Sorted -> input order.  The passes accumulate the potential and the field
+// grad(phi); the force is q grad(phi) (the reference's convention,
+// FP2P.hpp:80-101: F_t += q_t q_s grad_t(1/r)), the field is kept as well.
 __global__ __launch_bounds__(256) void k_scatter(int64_t n, const int32_t* __restrict__ perm,
-                                                const double* __restrict__ rs, double* __restrict__ r) {
+                                                const double* __restrict__ sp, const double* __restrict__ rs,
+                                                double* __restrict__ r, double* __restrict__ field) {
     const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const int64_t o = perm[i];
+    const double q = sp[3 * n + i];
     r[o] = rs[i];
-    r[n + o] = rs[n + i];
-    r[2 * n + o] = rs[2 * n + i];
-    r[3 * n + o] = rs[3 * n + i];
+    for (int d = 1; d < 4; ++d) {
+        const double g = rs[d * n + i];
+        r[d * n + o] = g * q;
+        field[(d - 1) * n + o] = g;
+    }
 }
 
 inline unsigned blocks256(int64_t n) { return static_cast<unsigned>((n + 255) / 256); }
@@ -871,7 +876,8 @@ int zero_results(Context* c) {
 int scatter_results(Context* c) {
     prof_begin(c, "scatter");
     hipLaunchKernelGGL(k_scatter, dim3(blocks256(c->n)), dim3(256), 0, c->stream, c->n, c->d_perm.as<int32_t>(),
-                       c->d_res_sorted.as<double>(), c->d_res.as<double>());
+                       c->d_sorted.as<double>(), c->d_res_sorted.as<double>(), c->d_res.as<double>(),
+                       c->d_field.as<double>());
     prof_end(c);
     return check_launch(c, "scatter");
 }

@@ -558,11 +558,11 @@ __device__ __forceinline__ void slot_offset(int s, int (&c)[3]) {
 
 constexpr int kKahanBlock = 16;  // pair terms summed plainly before a compensated add
 
-// Scales of the unit-coordinate sums: potential / (leaf width 2^-30), force /
-// (leaf width 2^-30)^2, times the target charge for the force.
+// Scales of the unit-coordinate sums: potential / (leaf width 2^-30), field /
+// (leaf width 2^-30)^2 (the charge is applied when the results are scattered).
 __device__ __forceinline__ void write_p2p(int64_t n, int64_t i, const float (&sum)[4], const float (&comp)[4],
-                                          double q, double inv_u, double* __restrict__ res) {
-    const double qf = q * inv_u * inv_u;
+                                          double inv_u, double* __restrict__ res) {
+    const double qf = inv_u * inv_u;
     res[i] = (static_cast<double>(sum[0]) - static_cast<double>(comp[0])) * inv_u;
     res[n + i] = (static_cast<double>(sum[1]) - static_cast<double>(comp[1])) * qf;
     res[2 * n + i] = (static_cast<double>(sum[2]) - static_cast<double>(comp[2])) * qf;
@@ -596,7 +596,7 @@ __global__ __launch_bounds__(256) void k_p2p_mixed(int64_t n, const int32_t* __r
             for (int k = 0; k < 4; ++k) kahan_add(sum[k], comp[k], part[k]);
         }
     }
-    write_p2p(n, i, sum, comp, sp[3 * n + i], inv_u, res);
+    write_p2p(n, i, sum, comp, inv_u, res);
 }
 
 // Mixed P2P of the big leaves.  Each wave takes work items of up to
@@ -686,7 +686,7 @@ __global__ __launch_bounds__(256) void k_p2p_mixed_big(const int2* __restrict__ 
         }
 #pragma unroll
         for (int u = 0; u < TPL; ++u)
-            if (ti[u] < tend) write_p2p(n, ti[u], sum[u], comp[u], sp[3 * n + ti[u]], inv_u, res);
+            if (ti[u] < tend) write_p2p(n, ti[u], sum[u], comp[u], inv_u, res);
     }
 }
 

@@ -195,10 +195,10 @@ static int upload_operators(Context* c) {
 static void free_all(Context* c) {
     DevBuf* bufs[] = {&c->d_Troots, &c->d_child, &c->d_pidx, &c->d_pvec, &c->d_rank, &c->d_opoff, &c->d_ops,
                       &c->d_slotpos, &c->d_slotord, &c->d_slotseg, &c->d_posoff, &c->d_opm, &c->d_in, &c->d_keys, &c->d_keys2, &c->d_vals, &c->d_vals2, &c->d_hist,
-                      &c->d_scan_part, &c->d_flags, &c->d_scan, &c->d_sorted, &c->d_perm, &c->d_pleaf,
+                      &c->d_scan_part, &c->d_qin, &c->d_flags, &c->d_scan, &c->d_sorted, &c->d_perm, &c->d_pleaf,
                       &c->d_count, &c->d_cellkey, &c->d_cellkey_tmp, &c->d_childstart, &c->d_leafstart,
                       &c->d_p2p, &c->d_p2pitems, &c->d_m2l, &c->d_order, &c->d_pcount, &c->d_dense, &c->d_M, &c->d_L, &c->d_Mb, &c->d_loc, &c->d_res_sorted,
-                      &c->d_res};
+                      &c->d_res, &c->d_field};
     for (DevBuf* b : bufs) release(*b);
 }
 
@@ -289,6 +289,31 @@ int hfmm_set_particles(hfmm_ctx* h, int64_t n, const double* x, const double* y,
     return HFMM_OK;
 }
 
+int hfmm_set_charges(hfmm_ctx* h, const double* q) {
+    CTX_CHECK(h);
+    Context* c = &h->c;
+    if (!q) return hfmm::fail(c, HFMM_ERR_ARG, "set_charges: bad arguments");
+    if (!c->tree_built) return hfmm::fail(c, HFMM_ERR_STATE, "set_charges before build_tree");
+    HFMM_HIP(c, hipSetDevice(c->device));
+    // the charges live in the owned input copy (device-set particles get one now)
+    HFMM_TRY(hfmm::ensure(c, c->d_qin, sizeof(double) * c->n));
+    HFMM_HIP(c, hipMemcpyAsync(c->d_qin.ptr, q, sizeof(double) * c->n, hipMemcpyHostToDevice, c->stream));
+    c->q = c->d_qin.as<double>();
+    c->evaluated = false;
+    return hfmm::update_charges_device(c);
+}
+
+int hfmm_set_charges_device(hfmm_ctx* h, const double* q) {
+    CTX_CHECK(h);
+    Context* c = &h->c;
+    if (!q) return hfmm::fail(c, HFMM_ERR_ARG, "set_charges_device: bad arguments");
+    if (!c->tree_built) return hfmm::fail(c, HFMM_ERR_STATE, "set_charges before build_tree");
+    HFMM_HIP(c, hipSetDevice(c->device));
+    c->q = q;
+    c->evaluated = false;
+    return hfmm::update_charges_device(c);
+}
+
 int hfmm_set_particles_device(hfmm_ctx* h, int64_t n, const double* x, const double* y, const double* z,
                               const double* q) {
     CTX_CHECK(h);
@@ -328,6 +353,7 @@ int hfmm_evaluate(hfmm_ctx* h, uint32_t ops) {
     HFMM_TRY(hfmm::ensure(c, c->d_L, exp_bytes));
     HFMM_TRY(hfmm::ensure(c, c->d_res_sorted, sizeof(double) * 4 * c->n));
     HFMM_TRY(hfmm::ensure(c, c->d_res, sizeof(double) * 4 * c->n));
+    HFMM_TRY(hfmm::ensure(c, c->d_field, sizeof(double) * 3 * c->n));
     // upward pass
     if (ops & HFMM_P2M) HFMM_TRY(hfmm::run_p2m(c));
     else HFMM_HIP(c, hipMemsetAsync(c->d_M.ptr, 0, exp_bytes, c->stream));
@@ -374,6 +400,31 @@ int hfmm_get_results(hfmm_ctx* h, double* pot, double* fx, double* fy, double* f
             HFMM_HIP(c, hipMemcpyAsync(outs[k], r + static_cast<size_t>(k) * c->n, sizeof(double) * c->n,
                                        hipMemcpyDeviceToHost, c->stream));
     HFMM_HIP(c, hipStreamSynchronize(c->stream));
+    return HFMM_OK;
+}
+
+int hfmm_get_field(hfmm_ctx* h, double* ex, double* ey, double* ez) {
+    CTX_CHECK(h);
+    Context* c = &h->c;
+    if (!c->evaluated) return hfmm::fail(c, HFMM_ERR_STATE, "get_field before evaluate");
+    const double* f = c->d_field.as<double>();
+    double* outs[3] = {ex, ey, ez};
+    for (int k = 0; k < 3; ++k)
+        if (outs[k])
+            HFMM_HIP(c, hipMemcpyAsync(outs[k], f + static_cast<size_t>(k) * c->n, sizeof(double) * c->n,
+                                       hipMemcpyDeviceToHost, c->stream));
+    HFMM_HIP(c, hipStreamSynchronize(c->stream));
+    return HFMM_OK;
+}
+
+int hfmm_get_field_device(hfmm_ctx* h, const double** ex, const double** ey, const double** ez) {
+    CTX_CHECK(h);
+    Context* c = &h->c;
+    if (!c->evaluated) return hfmm::fail(c, HFMM_ERR_STATE, "get_field before evaluate");
+    const double* f = c->d_field.as<double>();
+    if (ex) *ex = f;
+    if (ey) *ey = f + c->n;
+    if (ez) *ez = f + 2 * c->n;
     return HFMM_OK;
 }
 

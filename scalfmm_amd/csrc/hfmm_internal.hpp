@@ -64,6 +64,7 @@ struct Context {
     const double* z = nullptr;
     const double* q = nullptr;
     DevBuf d_in;  // owned copy when set from host: 4*n doubles
+    DevBuf d_qin; // charges set by hfmm_set_charges (host variant)
 
     // ---- tree
     bool tree_built = false;
@@ -97,8 +98,9 @@ struct Context {
     DevBuf d_M, d_L;      // [total_cells][LD]
     DevBuf d_Mb;          // mixed precision: multipoles as bf16 hi/mid/lo planes [3][total_cells + 1][LD]
     DevBuf d_loc;         // mixed precision: leaf-local fixed-point int4 (x, y, z, q bits) of the sorted particles
-    DevBuf d_res_sorted;  // pot, fx, fy, fz (sorted order)
+    DevBuf d_res_sorted;  // pot, grad(phi) x, y, z (sorted order; the charge is applied by the scatter)
     DevBuf d_res;         // pot, fx, fy, fz (input order)
+    DevBuf d_field;       // grad(phi) x, y, z (input order): the force without the target charge
 
     // ---- profiling
     bool profiling = false;
@@ -135,6 +137,8 @@ int check_launch(Context* c, const char* name);
 
 // Tree construction (tree_device.hip)
 int build_tree_device(Context* c);
+// New charges (c->q, input order) on the built tree: refreshes the sorted charges.
+int update_charges_device(Context* c);
 // Device counter holding the number of P2P work items of the big leaves.
 inline int32_t* p2p_item_count(Context* c) { return reinterpret_cast<int32_t*>(c->d_count.as<int64_t>() + 12); }
 

@@ -301,6 +301,14 @@ __global__ __launch_bounds__(kBlock) void k_gather_particles(int64_t n, const in
     out[3 * n + i] = q[o];
 }
 
+// New charges on an existing tree: sorted q[i] = q_in[perm[i]].
+__global__ __launch_bounds__(kBlock) void k_gather_charges(int64_t n, const int32_t* __restrict__ perm,
+                                                          const double* __restrict__ q, double* __restrict__ sq) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    sq[i] = q[perm[i]];
+}
+
 // flags[i] = 1 where (keys[i] >> shift) starts a new group.
 __global__ __launch_bounds__(kBlock) void k_head_flags(int64_t n, const uint64_t* __restrict__ keys, int shift,
                                                       int32_t* __restrict__ flags) {
@@ -445,6 +453,14 @@ int make_groups(Context* c, const uint64_t* keys, int64_t n, int shift, uint64_t
 }
 
 }  // namespace
+
+int update_charges_device(Context* c) {
+    prof_begin(c, "gather_charges");
+    hipLaunchKernelGGL(k_gather_charges, dim3(blocks_for(c->n)), dim3(kBlock), 0, c->stream, c->n,
+                       c->d_perm.as<int32_t>(), c->q, c->d_sorted.as<double>() + 3 * c->n);
+    prof_end(c);
+    return check_launch(c, "gather_charges");
+}
 
 int build_tree_device(Context* c) {
     const int64_t n = c->n;

@@ -68,6 +68,7 @@ class Engine:
         self.h = h
         self.params = p
         self.n = 0
+        self.tree_stamp = 0  # counts tree builds (the autograd wrapper checks it)
         self.height = int(tree_height)
         self.order = int(order)
         self.nnodes = self.order ** 3
@@ -100,7 +101,15 @@ class Engine:
         self._check(self._L.hfmm_set_particles_device(self.h, int(n), x_ptr, y_ptr, z_ptr, q_ptr))
         self.n = int(n)
 
+    def set_charges(self, q):
+        """New charges on the built tree (input order); the tree is kept."""
+        q = np.ascontiguousarray(q, dtype=np.float64)
+        if len(q) != self.n:
+            raise ValueError("one charge per particle")
+        self._check(self._L.hfmm_set_charges(self.h, _dp(q)))
+
     def build_tree(self):
+        self.tree_stamp += 1
         self._check(self._L.hfmm_build_tree(self.h))
 
     def evaluate(self, ops=NEAR_AND_FAR):
@@ -115,6 +124,12 @@ class Engine:
         return tuple(out)
 
     # ---- inspection -----------------------------------------------------------
+    def field(self):
+        """grad(phi) at every particle (input order): the force without the target charge."""
+        out = [np.empty(self.n) for _ in range(3)]
+        self._check(self._L.hfmm_get_field(self.h, *(_dp(a) for a in out)))
+        return tuple(out)
+
     def tree_info(self):
         ti = _abi.TreeInfo()
         self._check(self._L.hfmm_get_tree_info(self.h, C.byref(ti)))
