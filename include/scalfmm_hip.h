@@ -88,6 +88,22 @@ const char* hfmm_last_error(const hfmm_ctx* ctx);
  * constructor (Tests/GroupTree/testBlockedChebyshev.cpp:98-114). */
 int hfmm_set_particles(hfmm_ctx* ctx, int64_t n, const double* x, const double* y, const double* z,
                        const double* q);
+/* Several independent particle sets in one context, each with nrhs charge
+ * vectors.  Set s holds particles set_offsets[s] .. set_offsets[s+1]-1 of the
+ * arrays (set_offsets[0] = 0, set_offsets[nsets] = n, non-decreasing; NULL when
+ * nsets = 1) and lives in its own copy of the context's box with the context's
+ * tree height: no interaction crosses two sets.  q holds nrhs vectors of n
+ * charges (RHS-major).  With nrhs > 1, hfmm_get_results and hfmm_get_field
+ * write nrhs blocks of n values per output (RHS-major) and the device pointers
+ * of hfmm_get_results_device lead RHS r's arrays by 4 n r doubles.  The
+ * reference's counterparts are one FGroupTree per set and the NVALS template
+ * parameter of FChebSymKernel (FChebSymKernel.hpp:45-488, RHS stride 2 p^3 in
+ * FChebCellPOD.hpp:85-100); the sets share one device tree whose keys carry
+ * the set index above the set-local Morton key.  nsets * 8^(tree_height-1)
+ * must stay below 8^20. */
+#define HFMM_MAX_RHS 64
+int hfmm_set_particle_sets(hfmm_ctx* ctx, int32_t nsets, const int64_t* set_offsets, int32_t nrhs, int64_t n,
+                           const double* x, const double* y, const double* z, const double* q);
 /* Same with arrays already resident in device memory of the context's device;
  * the context reads them during hfmm_build_tree and hfmm_evaluate, so they must
  * stay valid until the next hfmm_set_particles*. */
@@ -199,6 +215,40 @@ int hfmm_selftest_mfma_f64(hfmm_ctx* ctx, double* out);
  * D = A B (16 x 16, row-major) through the operand layout the mixed-precision
  * M2L kernel assumes; no counterpart in the reference. */
 int hfmm_selftest_mfma_bf16(hfmm_ctx* ctx, float* out);
+
+/* ---- batches of particle sets ----------------------------------------------- */
+/* Many independent particle sets of uneven sizes, each with nrhs charge
+ * vectors, evaluated by one launch of a HIP graph.  Sets of equal tree height
+ * share one device tree (hfmm_set_particle_sets); the per-height trees share
+ * one stream, and the first hfmm_batch_evaluate after a build runs the passes
+ * and captures them, every RHS of every set, into one graph that later calls
+ * launch.  No counterpart in the reference (one FGroupTree and one algorithm
+ * run per set there). */
+typedef struct hfmm_batch hfmm_batch;
+/* Default height of a set of n particles: the smallest h in [3, 20] with
+ * n <= 32 * 8^(h-1) (at most 32 particles per leaf on average). */
+int32_t hfmm_batch_default_height(int64_t n);
+/* params: order, box (each set's own box), epsilon, precision and device;
+ * params->tree_height is ignored.  set_offsets: nsets+1 entries as in
+ * hfmm_set_particle_sets; set_heights: nsets heights in [3, 20], or NULL for
+ * hfmm_batch_default_height of each set's size. */
+int hfmm_batch_create(const hfmm_params* params, int32_t nsets, const int64_t* set_offsets,
+                      const int32_t* set_heights, int32_t nrhs, hfmm_batch** out);
+int hfmm_batch_destroy(hfmm_batch* b);
+const char* hfmm_batch_last_error(const hfmm_batch* b);
+/* Host arrays of all sets, concatenated in set order; q holds nrhs vectors of
+ * n charges (RHS-major). */
+int hfmm_batch_set_particles(hfmm_batch* b, const double* x, const double* y, const double* z, const double* q);
+int hfmm_batch_build(hfmm_batch* b);
+/* Evaluates every set and RHS (ops as in hfmm_evaluate).  The first call after
+ * a build, or with other ops, runs the passes eagerly and captures them; later
+ * calls launch the captured graph.  Returns without waiting. */
+int hfmm_batch_evaluate(hfmm_batch* b, uint32_t ops);
+int hfmm_batch_synchronize(hfmm_batch* b);
+/* nrhs blocks of n values per output (RHS-major, input order); NULL skips one. */
+int hfmm_batch_get_results(hfmm_batch* b, double* pot, double* fx, double* fy, double* fz);
+/* Number of per-height trees and whether the next evaluate launches a graph. */
+int hfmm_batch_info(const hfmm_batch* b, int32_t* ntrees, int32_t* graph_ready);
 
 /* Library version string. */
 const char* hfmm_version(void);

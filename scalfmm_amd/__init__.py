@@ -5,3 +5,4 @@ from .fmm import (  # noqa: F401
     Engine, FChebSymKernel, FFmmFarField, FFmmL2L, FFmmL2P, FFmmM2L, FFmmM2M, FFmmNearAndFarFields,
     FFmmNearField, FFmmP2M, FFmmP2P, FGroupTaskAlgorithm, FGroupTree, FmmError,
 )
+from .batch import BatchEngine, default_height  # noqa: F401

@@ -31,8 +31,13 @@ EXPORTS = (
     "hfmm_set_profiling", "hfmm_get_kernel_times", "hfmm_reset_kernel_times", "hfmm_selftest_mfma_f64",
     "hfmm_selftest_mfma_bf16", "hfmm_direct",
     "hfmm_get_field", "hfmm_get_field_device", "hfmm_set_charges", "hfmm_set_charges_device",
-    "hfmm_version",
+    "hfmm_version", "hfmm_set_particle_sets",
+    "hfmm_batch_default_height", "hfmm_batch_create", "hfmm_batch_destroy", "hfmm_batch_last_error",
+    "hfmm_batch_set_particles", "hfmm_batch_build", "hfmm_batch_evaluate", "hfmm_batch_synchronize",
+    "hfmm_batch_get_results", "hfmm_batch_info",
 )
+MAX_RHS = 64
+_NON_STATUS = ("hfmm_last_error", "hfmm_version", "hfmm_batch_last_error", "hfmm_batch_default_height")
 
 
 class Params(C.Structure):
@@ -99,8 +104,21 @@ def lib(path: str | None = None):
     L.hfmm_set_charges_device.argtypes = [vp, vp]
     L.hfmm_get_field_device.argtypes = [vp, C.POINTER(vp), C.POINTER(vp), C.POINTER(vp)]
     L.hfmm_version.restype = C.c_char_p
+    L.hfmm_set_particle_sets.argtypes = [vp, i32, C.POINTER(i64), i32, i64, dp, dp, dp, dp]
+    L.hfmm_batch_default_height.restype = i32
+    L.hfmm_batch_default_height.argtypes = [i64]
+    L.hfmm_batch_create.argtypes = [C.POINTER(Params), i32, C.POINTER(i64), C.POINTER(i32), i32, C.POINTER(vp)]
+    L.hfmm_batch_destroy.argtypes = [vp]
+    L.hfmm_batch_last_error.restype = C.c_char_p
+    L.hfmm_batch_last_error.argtypes = [vp]
+    L.hfmm_batch_set_particles.argtypes = [vp, dp, dp, dp, dp]
+    L.hfmm_batch_build.argtypes = [vp]
+    L.hfmm_batch_evaluate.argtypes = [vp, C.c_uint32]
+    L.hfmm_batch_synchronize.argtypes = [vp]
+    L.hfmm_batch_get_results.argtypes = [vp, dp, dp, dp, dp]
+    L.hfmm_batch_info.argtypes = [vp, C.POINTER(i32), C.POINTER(i32)]
     for name in EXPORTS:
-        getattr(L, name).restype = getattr(L, name).restype if name in ("hfmm_last_error", "hfmm_version") \
-            else C.c_int
+        if name not in _NON_STATUS:
+            getattr(L, name).restype = C.c_int
     _lib = L
     return L

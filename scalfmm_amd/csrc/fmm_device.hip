@@ -12,6 +12,8 @@
 //        (Src/Kernels/P2P/FP2P.hpp:103-269 conventions)
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "fmm_common.hpp"
 #include "hfmm_internal.hpp"
 
@@ -711,13 +713,22 @@ __global__ __launch_bounds__(256) void k_scatter(int64_t n, const int32_t* __res
 
 inline unsigned blocks256(int64_t n) { return static_cast<unsigned>((n + 255) / 256); }
 
+// Grid-stride fill with zeros (a kernel node, where hipMemsetAsync would be a
+// memset node in a captured graph).
+__global__ __launch_bounds__(256) void k_zero_words(uint32_t* __restrict__ p, int64_t words) {
+    const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
+    for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < words; i += stride) p[i] = 0u;
+}
+
 }  // namespace
 
 Box make_box(const Context* c) {
     Box b;
     const double half = c->P.box_width / 2;
     for (int d = 0; d < 3; ++d) b.corner[d] = c->P.box_center[d] + (-half);
-    b.leafw = c->P.box_width / static_cast<double>(1 << (c->height() - 1));
+    const int Lloc = c->P.tree_height - 1;
+    b.leafw = c->P.box_width / static_cast<double>(1 << Lloc);
+    b.cmask = (1u << Lloc) - 1u;
     return b;
 }
 
@@ -727,12 +738,13 @@ M2LLevels make_levels(const Context* c, int tile_targets, int& tiles) {
     lv.nlev = 0;
     tiles = 0;
     lv.tile_begin[0] = 0;
-    for (int l = 2; l <= L; ++l) {
+    for (int l = c->top(); l <= L; ++l) {
         const int i = lv.nlev++;
         lv.cell_off[i] = c->lvl_off[l];
         lv.pcount_off[i] = 8 * l;
-        // 2^l cells per axis: scale factor 2 / (width / 2^l) (FInterpMatrixKernel.hpp:124-133)
-        lv.scale[i] = 2. / (c->P.box_width / static_cast<double>(int64_t(1) << l));
+        // 2^l cells per axis of a set: scale factor 2 / (width / 2^l) (FInterpMatrixKernel.hpp:124-133)
+        const int lloc = l - c->set_levels;
+        lv.scale[i] = 2. / (c->P.box_width / static_cast<double>(int64_t(1) << lloc));
         // segment of 8 x (tiles of the largest parity); the parity counts are
         // on the device, ceil(ncell/tile) bounds every parity's tile count
         tiles += 8 * static_cast<int>((c->ncell[l] + tile_targets - 1) / tile_targets);
@@ -801,9 +813,8 @@ int m2l_impl(Context* c) {
     const unsigned grid = static_cast<unsigned>(tiles);
     const size_t shmem =
         sizeof(double) * kM2LWaves * static_cast<size_t>(c->kpad_max) * kM2LTile + sizeof(uint16_t) * 2 * c->LD;
-    if (shmem > 64 * 1024)
-        HFMM_HIP(c, hipFuncSetAttribute(reinterpret_cast<const void*>(&k_m2l_f64<P>),
-                                        hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(shmem)));
+    static int lds_limit[64] = {0};
+    HFMM_TRY(raise_lds_limit(c, reinterpret_cast<const void*>(&k_m2l_f64<P>), shmem, lds_limit));
     prof_begin(c, "m2l");
     hipLaunchKernelGGL(k_m2l_f64<P>, dim3(grid), dim3(64 * kM2LWaves), shmem, c->stream, lv,
                        c->d_m2l.as<int32_t>(), c->d_order.as<int32_t>(), c->d_pcount.as<int32_t>(),
@@ -868,16 +879,24 @@ int run_p2p(Context* c) {
     return check_launch(c, "p2p_big");
 }
 
-int zero_results(Context* c) {
-    HFMM_HIP(c, hipMemsetAsync(c->d_res_sorted.ptr, 0, sizeof(double) * 4 * c->n, c->stream));
-    return HFMM_OK;
+int zero_device(Context* c, void* ptr, size_t bytes) {
+    if (bytes % 4) return fail(c, HFMM_ERR_ARG, "zero_device: size not a multiple of 4");
+    const int64_t words = static_cast<int64_t>(bytes / 4);
+    if (words == 0) return HFMM_OK;
+    const int64_t blocks = std::min<int64_t>((words + 1023) / 1024, 65536);
+    hipLaunchKernelGGL(k_zero_words, dim3(static_cast<unsigned>(blocks)), dim3(256), 0, c->stream,
+                       static_cast<uint32_t*>(ptr), words);
+    return check_launch(c, "zero");
 }
 
-int scatter_results(Context* c) {
+int zero_results(Context* c) { return zero_device(c, c->d_res_sorted.ptr, sizeof(double) * 4 * c->n); }
+
+int scatter_results(Context* c, int rhs) {
     prof_begin(c, "scatter");
     hipLaunchKernelGGL(k_scatter, dim3(blocks256(c->n)), dim3(256), 0, c->stream, c->n, c->d_perm.as<int32_t>(),
-                       c->d_sorted.as<double>(), c->d_res_sorted.as<double>(), c->d_res.as<double>(),
-                       c->d_field.as<double>());
+                       c->d_sorted.as<double>(), c->d_res_sorted.as<double>(),
+                       c->d_res.as<double>() + static_cast<size_t>(rhs) * 4 * c->n,
+                       c->d_field.as<double>() + static_cast<size_t>(rhs) * 3 * c->n);
     prof_end(c);
     return check_launch(c, "scatter");
 }

@@ -714,8 +714,8 @@ int m2l_mixed_impl(Context* c) {
     const M2LLevels lv = make_levels(c, 16 * COLS, tiles);
     const int CR = ((c->kpad_max + 31) & ~31) + 4;
     const size_t shmem = sizeof(float) * kM2LWaves * static_cast<size_t>(CR) * 16 * COLS + 16 * COLS * 2 * 3 * 64;
-    HFMM_HIP(c, hipFuncSetAttribute(reinterpret_cast<const void*>(&k_m2l_mixed<P, COLS>),
-                                    hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(shmem)));
+    static int lds_limit[64] = {0};
+    HFMM_TRY(raise_lds_limit(c, reinterpret_cast<const void*>(&k_m2l_mixed<P, COLS>), shmem, lds_limit));
     const int64_t plane = (c->total_cells + 1) * static_cast<int64_t>(c->LD);
     prof_begin(c, "m2l_mixed");
     hipLaunchKernelGGL((k_m2l_mixed<P, COLS>), dim3(static_cast<unsigned>(tiles)), dim3(64 * kM2LWaves), shmem,
@@ -725,8 +725,8 @@ int m2l_mixed_impl(Context* c) {
                        c->LD, CR);
     prof_end(c);
     HFMM_TRY(check_launch(c, "m2l_mixed"));
-    // the M2L targets: every cell of levels 2..h-1, contiguous in the cell arrays
-    const int64_t first = c->lvl_off[2], ncells = c->lvl_off[c->height()] - first;
+    // the M2L targets: every cell of levels top..h-1, contiguous in the cell arrays
+    const int64_t first = c->lvl_off[c->top()], ncells = c->lvl_off[c->height()] - first;
     prof_begin(c, "modal_to_nodal");
     hipLaunchKernelGGL(k_modal_to_nodal<P>, dim3(static_cast<unsigned>((ncells + 3) / 4)), dim3(256), 0, c->stream,
                        ncells, c->d_Troots.as<double>(), c->d_L.as<double>() + first * c->LD, c->LD);
@@ -810,7 +810,7 @@ int run_p2p_mixed(Context* c) {
     HFMM_TRY(ensure(c, c->d_loc, sizeof(int4) * static_cast<size_t>(c->n + kKahanBlock)));
     const Box bx = make_box(c);
     int4* loc = c->d_loc.as<int4>();
-    HFMM_HIP(c, hipMemsetAsync(loc + c->n, 0, sizeof(int4) * kKahanBlock, c->stream));
+    HFMM_TRY(zero_device(c, loc + c->n, sizeof(int4) * kKahanBlock));
     prof_begin(c, "local_coords");
     hipLaunchKernelGGL(k_local_coords, dim3(blocks_of(c->n)), dim3(256), 0, c->stream, c->n, c->d_pleaf.as<int32_t>(),
                        c->d_cellkey.as<uint64_t>() + c->lvl_off[L], bx, c->d_sorted.as<double>(), loc);

@@ -198,7 +198,7 @@ static void free_all(Context* c) {
                       &c->d_scan_part, &c->d_qin, &c->d_flags, &c->d_scan, &c->d_sorted, &c->d_perm, &c->d_pleaf,
                       &c->d_count, &c->d_cellkey, &c->d_cellkey_tmp, &c->d_childstart, &c->d_leafstart,
                       &c->d_p2p, &c->d_p2pitems, &c->d_m2l, &c->d_order, &c->d_pcount, &c->d_dense, &c->d_M, &c->d_L, &c->d_Mb, &c->d_loc, &c->d_res_sorted,
-                      &c->d_res, &c->d_field};
+                      &c->d_res, &c->d_field, &c->d_setoff};
     for (DevBuf* b : bufs) release(*b);
 }
 
@@ -219,7 +219,13 @@ extern "C" {
 
 const char* hfmm_version(void) { return "scalfmm_amd 0.1 (gfx950, FP64 Chebyshev 1/r)"; }
 
-int hfmm_create(const hfmm_params* params, hfmm_ctx** out) {
+int hfmm_create(const hfmm_params* params, hfmm_ctx** out) { return hfmm::create_context(params, nullptr, out); }
+
+}  // extern "C"
+
+namespace hfmm {
+
+int create_context(const hfmm_params* params, hipStream_t shared_stream, hfmm_ctx** out) {
     if (!params || !out) return HFMM_ERR_ARG;
     *out = nullptr;
     if (params->order < 5 || params->order > 8) return HFMM_ERR_ARG;
@@ -231,7 +237,13 @@ int hfmm_create(const hfmm_params* params, hfmm_ctx** out) {
     Context* c = &h->c;
     c->P = *params;
     c->device = params->device;
-    if (hipSetDevice(c->device) != hipSuccess || hipStreamCreate(&c->stream) != hipSuccess) {
+    c->own_stream = shared_stream == nullptr;
+    if (hipSetDevice(c->device) != hipSuccess) {
+        delete h;
+        return HFMM_ERR_DEVICE;
+    }
+    if (shared_stream) c->stream = shared_stream;
+    else if (hipStreamCreate(&c->stream) != hipSuccess) {
         delete h;
         return HFMM_ERR_DEVICE;
     }
@@ -241,13 +253,47 @@ int hfmm_create(const hfmm_params* params, hfmm_ctx** out) {
     if (st == HFMM_OK && c->P.precision == HFMM_PRECISION_MIXED) st = hfmm::prepare_mixed_operators(c);
     if (st != HFMM_OK) {
         hfmm::free_all(c);
-        (void)hipStreamDestroy(c->stream);
+        if (c->own_stream) (void)hipStreamDestroy(c->stream);
         delete h;
         return st;
     }
     *out = h;
     return HFMM_OK;
 }
+
+// The passes of one charge vector: results into d_res_sorted.
+static int evaluate_passes(Context* c, uint32_t ops, size_t exp_bytes) {
+    const int L = c->height() - 1;
+    const int top = c->top();
+    // upward pass
+    if (ops & HFMM_P2M) HFMM_TRY(hfmm::run_p2m(c));
+    else HFMM_TRY(hfmm::zero_device(c, c->d_M.ptr, exp_bytes));
+    if (ops & HFMM_M2M)
+        for (int l = L - 1; l >= top; --l) HFMM_TRY(hfmm::run_m2m(c, l));
+    // transfer pass: M2L writes every local of levels top..h-1
+    const bool mixed = c->P.precision == HFMM_PRECISION_MIXED;
+    if (ops & HFMM_M2L) {
+        if (mixed) {
+            HFMM_TRY(hfmm::run_split_multipoles(c));
+            HFMM_TRY(hfmm::run_m2l_mixed(c));
+        } else {
+            HFMM_TRY(hfmm::run_m2l(c));
+        }
+    } else
+        HFMM_TRY(hfmm::zero_device(c, c->d_L.ptr, exp_bytes));
+    // downward pass
+    if (ops & HFMM_L2L)
+        for (int l = top; l <= L - 1; ++l) HFMM_TRY(hfmm::run_l2l(c, l));
+    // near field initialises the results, the merge adds the far field
+    if (ops & HFMM_P2P) HFMM_TRY(mixed ? hfmm::run_p2p_mixed(c) : hfmm::run_p2p(c));
+    else HFMM_TRY(hfmm::zero_results(c));
+    if (ops & HFMM_L2P) HFMM_TRY(hfmm::run_l2p(c));
+    return HFMM_OK;
+}
+
+}  // namespace hfmm
+
+extern "C" {
 
 int hfmm_destroy(hfmm_ctx* h) {
     CTX_CHECK(h);
@@ -260,7 +306,7 @@ int hfmm_destroy(hfmm_ctx* h) {
     }
     for (auto e : c->event_pool) (void)hipEventDestroy(e);
     hfmm::free_all(c);
-    if (c->stream) (void)hipStreamDestroy(c->stream);
+    if (c->stream && c->own_stream) (void)hipStreamDestroy(c->stream);
     delete h;
     return HFMM_OK;
 }
@@ -268,22 +314,50 @@ int hfmm_destroy(hfmm_ctx* h) {
 const char* hfmm_last_error(const hfmm_ctx* h) { return h ? h->c.err.c_str() : "null context"; }
 
 int hfmm_set_particles(hfmm_ctx* h, int64_t n, const double* x, const double* y, const double* z, const double* q) {
+    return hfmm_set_particle_sets(h, 1, nullptr, 1, n, x, y, z, q);
+}
+
+int hfmm_set_particle_sets(hfmm_ctx* h, int32_t nsets, const int64_t* set_offsets, int32_t nrhs, int64_t n,
+                           const double* x, const double* y, const double* z, const double* q) {
     CTX_CHECK(h);
     Context* c = &h->c;
     if (n <= 0 || !x || !y || !z || !q) return hfmm::fail(c, HFMM_ERR_ARG, "set_particles: bad arguments");
+    if (nsets < 1 || (nsets > 1 && !set_offsets)) return hfmm::fail(c, HFMM_ERR_ARG, "set_particles: bad sets");
+    if (nrhs < 1 || nrhs > HFMM_MAX_RHS) return hfmm::fail(c, HFMM_ERR_ARG, "set_particles: bad nrhs");
+    if (set_offsets) {
+        if (set_offsets[0] != 0 || set_offsets[nsets] != n)
+            return hfmm::fail(c, HFMM_ERR_ARG, "set_particles: set offsets must run from 0 to n");
+        for (int32_t s = 0; s < nsets; ++s)
+            if (set_offsets[s + 1] < set_offsets[s])
+                return hfmm::fail(c, HFMM_ERR_ARG, "set_particles: set offsets must not decrease");
+    }
+    // set levels: the smallest S with 8^S >= nsets; the virtual tree must fit the keys
+    int S = 0;
+    while ((int64_t(1) << (3 * S)) < nsets) ++S;
+    if (c->P.tree_height + S > hfmm::kMaxLevels)
+        return hfmm::fail(c, HFMM_ERR_ARG, "set_particles: too many sets for this tree height");
     HFMM_HIP(c, hipSetDevice(c->device));
-    HFMM_TRY(hfmm::ensure(c, c->d_in, sizeof(double) * 4 * n));
+    const size_t nv = static_cast<size_t>(3 + nrhs) * n;
+    HFMM_TRY(hfmm::ensure(c, c->d_in, sizeof(double) * nv));
     double* d = c->d_in.as<double>();
     HFMM_HIP(c, hipMemcpyAsync(d, x, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
     HFMM_HIP(c, hipMemcpyAsync(d + n, y, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
     HFMM_HIP(c, hipMemcpyAsync(d + 2 * n, z, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
-    HFMM_HIP(c, hipMemcpyAsync(d + 3 * n, q, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
+    HFMM_HIP(c, hipMemcpyAsync(d + 3 * n, q, sizeof(double) * n * nrhs, hipMemcpyHostToDevice, c->stream));
+    if (nsets > 1) {
+        HFMM_TRY(hfmm::ensure(c, c->d_setoff, sizeof(int64_t) * (nsets + 1)));
+        HFMM_HIP(c, hipMemcpyAsync(c->d_setoff.ptr, set_offsets, sizeof(int64_t) * (nsets + 1), hipMemcpyHostToDevice,
+                                   c->stream));
+    }
     HFMM_HIP(c, hipStreamSynchronize(c->stream));
     c->n = n;
     c->x = d;
     c->y = d + n;
     c->z = d + 2 * n;
     c->q = d + 3 * n;
+    c->nsets = nsets;
+    c->set_levels = S;
+    c->nrhs = nrhs;
     c->tree_built = false;
     c->evaluated = false;
     return HFMM_OK;
@@ -295,9 +369,10 @@ int hfmm_set_charges(hfmm_ctx* h, const double* q) {
     if (!q) return hfmm::fail(c, HFMM_ERR_ARG, "set_charges: bad arguments");
     if (!c->tree_built) return hfmm::fail(c, HFMM_ERR_STATE, "set_charges before build_tree");
     HFMM_HIP(c, hipSetDevice(c->device));
-    // the charges live in the owned input copy (device-set particles get one now)
-    HFMM_TRY(hfmm::ensure(c, c->d_qin, sizeof(double) * c->n));
-    HFMM_HIP(c, hipMemcpyAsync(c->d_qin.ptr, q, sizeof(double) * c->n, hipMemcpyHostToDevice, c->stream));
+    // nrhs charge vectors (RHS-major) into a buffer of the context
+    const size_t nq = static_cast<size_t>(c->n) * c->nrhs;
+    HFMM_TRY(hfmm::ensure(c, c->d_qin, sizeof(double) * nq));
+    HFMM_HIP(c, hipMemcpyAsync(c->d_qin.ptr, q, sizeof(double) * nq, hipMemcpyHostToDevice, c->stream));
     c->q = c->d_qin.as<double>();
     c->evaluated = false;
     return hfmm::update_charges_device(c);
@@ -319,6 +394,9 @@ int hfmm_set_particles_device(hfmm_ctx* h, int64_t n, const double* x, const dou
     CTX_CHECK(h);
     Context* c = &h->c;
     if (n <= 0 || !x || !y || !z || !q) return hfmm::fail(c, HFMM_ERR_ARG, "set_particles_device: bad arguments");
+    c->nsets = 1;
+    c->set_levels = 0;
+    c->nrhs = 1;
     c->n = n;
     c->x = x;
     c->y = y;
@@ -344,40 +422,22 @@ int hfmm_evaluate(hfmm_ctx* h, uint32_t ops) {
     if (!c->tree_built) return hfmm::fail(c, HFMM_ERR_STATE, "evaluate before build_tree");
     if (ops & ~HFMM_NEAR_AND_FAR) return hfmm::fail(c, HFMM_ERR_ARG, "unknown operation bits");
     HFMM_HIP(c, hipSetDevice(c->device));
-    const int L = c->height() - 1;
     const size_t exp_bytes = sizeof(double) * static_cast<size_t>(c->total_cells) * c->LD;
     // one extra zero multipole after the last cell: the M2L kernel reads it for missing sources
     HFMM_TRY(hfmm::ensure(c, c->d_M, exp_bytes + sizeof(double) * c->LD));
-    HFMM_HIP(c, hipMemsetAsync(c->d_M.as<double>() + static_cast<size_t>(c->total_cells) * c->LD, 0,
-                               sizeof(double) * c->LD, c->stream));
     HFMM_TRY(hfmm::ensure(c, c->d_L, exp_bytes));
     HFMM_TRY(hfmm::ensure(c, c->d_res_sorted, sizeof(double) * 4 * c->n));
-    HFMM_TRY(hfmm::ensure(c, c->d_res, sizeof(double) * 4 * c->n));
-    HFMM_TRY(hfmm::ensure(c, c->d_field, sizeof(double) * 3 * c->n));
-    // upward pass
-    if (ops & HFMM_P2M) HFMM_TRY(hfmm::run_p2m(c));
-    else HFMM_HIP(c, hipMemsetAsync(c->d_M.ptr, 0, exp_bytes, c->stream));
-    if (ops & HFMM_M2M)
-        for (int l = L - 1; l >= 2; --l) HFMM_TRY(hfmm::run_m2m(c, l));
-    // transfer pass: M2L writes every local of levels 2..h-1
-    const bool mixed = c->P.precision == HFMM_PRECISION_MIXED;
-    if (ops & HFMM_M2L) {
-        if (mixed) {
-            HFMM_TRY(hfmm::run_split_multipoles(c));
-            HFMM_TRY(hfmm::run_m2l_mixed(c));
-        } else {
-            HFMM_TRY(hfmm::run_m2l(c));
-        }
-    } else
-        HFMM_HIP(c, hipMemsetAsync(c->d_L.ptr, 0, exp_bytes, c->stream));
-    // downward pass
-    if (ops & HFMM_L2L)
-        for (int l = 2; l <= L - 1; ++l) HFMM_TRY(hfmm::run_l2l(c, l));
-    // near field initialises the results, the merge adds the far field
-    if (ops & HFMM_P2P) HFMM_TRY(mixed ? hfmm::run_p2p_mixed(c) : hfmm::run_p2p(c));
-    else HFMM_TRY(hfmm::zero_results(c));
-    if (ops & HFMM_L2P) HFMM_TRY(hfmm::run_l2p(c));
-    HFMM_TRY(hfmm::scatter_results(c));
+    HFMM_TRY(hfmm::ensure(c, c->d_res, sizeof(double) * 4 * c->n * c->nrhs));
+    HFMM_TRY(hfmm::ensure(c, c->d_field, sizeof(double) * 3 * c->n * c->nrhs));
+    HFMM_TRY(hfmm::zero_device(c, c->d_M.as<double>() + static_cast<size_t>(c->total_cells) * c->LD,
+                               sizeof(double) * c->LD));
+    // one pass sequence per charge vector (the tree and the lists are shared);
+    // with one vector the sorted charges are already in place
+    for (int r = 0; r < c->nrhs; ++r) {
+        if (c->nrhs > 1) HFMM_TRY(hfmm::update_charges_device(c, r));
+        HFMM_TRY(hfmm::evaluate_passes(c, ops, exp_bytes));
+        HFMM_TRY(hfmm::scatter_results(c, r));
+    }
     c->evaluated = true;
     return HFMM_OK;
 }
@@ -389,32 +449,33 @@ int hfmm_synchronize(hfmm_ctx* h) {
     return HFMM_OK;
 }
 
+// Copies nrhs blocks of n values per output (RHS-major) from a device array
+// laid out as [rhs][k][n].
+static int copy_outputs(Context* c, const double* src, int nk, double* const* outs) {
+    const size_t n = static_cast<size_t>(c->n);
+    for (int r = 0; r < c->nrhs; ++r)
+        for (int k = 0; k < nk; ++k)
+            if (outs[k])
+                HFMM_HIP(c, hipMemcpyAsync(outs[k] + r * n, src + (static_cast<size_t>(r) * nk + k) * n,
+                                           sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
+    HFMM_HIP(c, hipStreamSynchronize(c->stream));
+    return HFMM_OK;
+}
+
 int hfmm_get_results(hfmm_ctx* h, double* pot, double* fx, double* fy, double* fz) {
     CTX_CHECK(h);
     Context* c = &h->c;
     if (!c->evaluated) return hfmm::fail(c, HFMM_ERR_STATE, "get_results before evaluate");
-    const double* r = c->d_res.as<double>();
     double* outs[4] = {pot, fx, fy, fz};
-    for (int k = 0; k < 4; ++k)
-        if (outs[k])
-            HFMM_HIP(c, hipMemcpyAsync(outs[k], r + static_cast<size_t>(k) * c->n, sizeof(double) * c->n,
-                                       hipMemcpyDeviceToHost, c->stream));
-    HFMM_HIP(c, hipStreamSynchronize(c->stream));
-    return HFMM_OK;
+    return copy_outputs(c, c->d_res.as<double>(), 4, outs);
 }
 
 int hfmm_get_field(hfmm_ctx* h, double* ex, double* ey, double* ez) {
     CTX_CHECK(h);
     Context* c = &h->c;
     if (!c->evaluated) return hfmm::fail(c, HFMM_ERR_STATE, "get_field before evaluate");
-    const double* f = c->d_field.as<double>();
     double* outs[3] = {ex, ey, ez};
-    for (int k = 0; k < 3; ++k)
-        if (outs[k])
-            HFMM_HIP(c, hipMemcpyAsync(outs[k], f + static_cast<size_t>(k) * c->n, sizeof(double) * c->n,
-                                       hipMemcpyDeviceToHost, c->stream));
-    HFMM_HIP(c, hipStreamSynchronize(c->stream));
-    return HFMM_OK;
+    return copy_outputs(c, c->d_field.as<double>(), 3, outs);
 }
 
 int hfmm_get_field_device(hfmm_ctx* h, const double** ex, const double** ey, const double** ez) {
@@ -626,6 +687,7 @@ int hfmm_direct(hfmm_ctx* h, int64_t ntargets, const int64_t* targets, double* p
     CTX_CHECK(h);
     Context* c = &h->c;
     if (c->n <= 0) return hfmm::fail(c, HFMM_ERR_STATE, "direct before set_particles");
+    if (c->nsets > 1) return hfmm::fail(c, HFMM_ERR_ARG, "direct: the context holds several particle sets");
     if (ntargets <= 0 || !targets) return hfmm::fail(c, HFMM_ERR_ARG, "direct: bad targets");
     for (int64_t i = 0; i < ntargets; ++i)
         if (targets[i] < 0 || targets[i] >= c->n) return hfmm::fail(c, HFMM_ERR_ARG, "direct: target out of range");

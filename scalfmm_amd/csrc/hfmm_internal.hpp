@@ -38,6 +38,7 @@ struct Context {
     hfmm_params P{};
     int device = 0;
     hipStream_t stream = nullptr;
+    bool own_stream = true;  // false when the stream belongs to a batch
     std::string err;
 
     // ---- operators (host + device copies)
@@ -63,8 +64,17 @@ struct Context {
     const double* y = nullptr;
     const double* z = nullptr;
     const double* q = nullptr;
-    DevBuf d_in;  // owned copy when set from host: 4*n doubles
+    DevBuf d_in;  // owned copy when set from host: x, y, z, then nrhs charge vectors
     DevBuf d_qin; // charges set by hfmm_set_charges (host variant)
+    // Particle sets (a forest): set s owns input particles setoff[s]..setoff[s+1]-1
+    // and lives in its own copy of the box.  Its index sits in the key bits
+    // above the set-local Morton key, i.e. set s is the cell s of level
+    // set_levels of a virtual tree whose levels above set_levels+2 carry no
+    // work.  One set: set_levels = 0 and the virtual tree is the tree.
+    int32_t nsets = 1;
+    int set_levels = 0;
+    DevBuf d_setoff;  // int64 [nsets + 1] (unused when nsets == 1)
+    int32_t nrhs = 1;  // charge vectors per particle (RHS-major, n each)
 
     // ---- tree
     bool tree_built = false;
@@ -108,7 +118,10 @@ struct Context {
     std::vector<hipEvent_t> event_pool;
     std::vector<hfmm_kernel_time> totals;
 
-    int height() const { return P.tree_height; }
+    // height of the virtual tree: the per-set height plus the set levels
+    int height() const { return P.tree_height + set_levels; }
+    // first level with M2L / M2M / L2L work (2 below each set root)
+    int top() const { return 2 + set_levels; }
     int order() const { return P.order; }
 };
 
@@ -135,10 +148,13 @@ void prof_begin(Context* c, const char* name);
 void prof_end(Context* c);
 int check_launch(Context* c, const char* name);
 
+// A context on the given stream (nullptr: a stream of its own).
+int create_context(const hfmm_params* params, hipStream_t shared_stream, hfmm_ctx** out);
+
 // Tree construction (tree_device.hip)
 int build_tree_device(Context* c);
 // New charges (c->q, input order) on the built tree: refreshes the sorted charges.
-int update_charges_device(Context* c);
+int update_charges_device(Context* c, int rhs = 0);
 // Device counter holding the number of P2P work items of the big leaves.
 inline int32_t* p2p_item_count(Context* c) { return reinterpret_cast<int32_t*>(c->d_count.as<int64_t>() + 12); }
 
@@ -150,7 +166,10 @@ int run_l2l(Context* c, int level);  // parents at `level`
 int run_l2p(Context* c);
 int run_p2p(Context* c);
 int zero_results(Context* c);
-int scatter_results(Context* c);
+// results of charge vector `rhs` into its slot of the input-order outputs
+int scatter_results(Context* c, int rhs = 0);
+// clears device memory with a fill kernel (capturable, unlike hipMemsetAsync)
+int zero_device(Context* c, void* ptr, size_t bytes);
 int selftest_mfma_f64(Context* c, double* out);
 
 // Mixed-precision passes (fmm_mixed.hip)

@@ -268,13 +268,17 @@ int radix_sort(Context* c, uint64_t* keys, int32_t* vals, uint64_t* keys2, int32
 // ---------------------------------------------------------------------------
 // Leaf coordinates and keys (FCoordinateComputer::GetTreeCoordinate):
 // c = rel == width ? 2^(h-1)-1 : int(rel / leafwidth), rel in [0, width].
+// With particle sets the key is (set << 3(h-1)) | set-local key; the set of a
+// particle is found by bisection of the set offsets.
 __global__ __launch_bounds__(kBlock) void k_morton(int64_t n, const double* __restrict__ x,
                                                   const double* __restrict__ y, const double* __restrict__ z,
                                                   double cx, double cy, double cz, double width, double leafw,
-                                                  int lim, uint64_t* __restrict__ keys, int32_t* __restrict__ vals,
+                                                  int lbits, const int64_t* __restrict__ setoff, int nsets,
+                                                  uint64_t* __restrict__ keys, int32_t* __restrict__ vals,
                                                   int32_t* __restrict__ err) {
     const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= n) return;
+    const int lim = 1 << lbits;
     const double rel[3] = {x[i] - cx, y[i] - cy, z[i] - cz};
     int co[3];
     bool bad = false;
@@ -284,7 +288,16 @@ __global__ __launch_bounds__(kBlock) void k_morton(int64_t n, const double* __re
         if (co[d] >= lim) { bad = true; co[d] = lim - 1; }
     }
     if (bad) atomicOr(err, 1);
-    keys[i] = morton3(co[0], co[1], co[2]);
+    uint64_t set = 0;
+    if (setoff) {  // last set whose first particle is <= i
+        int lo = 0, hi = nsets - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (setoff[mid] <= i) lo = mid; else hi = mid - 1;
+        }
+        set = static_cast<uint64_t>(lo);
+    }
+    keys[i] = (set << (3 * lbits)) | morton3(co[0], co[1], co[2]);
     vals[i] = static_cast<int32_t>(i);
 }
 
@@ -336,22 +349,25 @@ __global__ __launch_bounds__(kBlock) void k_write_groups(int64_t n, const uint64
 __global__ void k_set_int(int32_t* p, int32_t v) { *p = v; }
 
 // P2P neighbour slots of every leaf: 26 slots in ascending position order,
-// slot -> leaf index or -1 (outside the box or empty).
+// slot -> leaf index or -1 (outside the box or empty).  lim = leaves per axis
+// of one set: the range test is on the set-local coordinates (the low bits),
+// so no list leaves the set of its cell.
 __global__ __launch_bounds__(kBlock) void k_p2p_slots(KeyIndex ix, int lim, int32_t* __restrict__ slots) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int nleaf = ix.n;
     if (i >= nleaf) return;
     int x, y, z;
     demorton3(ix.keys[i], x, y, z);
+    const int lx = x & (lim - 1), ly = y & (lim - 1), lz = z & (lim - 1);
     int s = 0;
     for (int dx = -1; dx <= 1; ++dx)
         for (int dy = -1; dy <= 1; ++dy)
             for (int dz = -1; dz <= 1; ++dz) {
                 if (!dx && !dy && !dz) continue;
-                const int nx = x + dx, ny = y + dy, nz = z + dz;
+                const int nx = lx + dx, ny = ly + dy, nz = lz + dz;
                 int f = -1;
                 if (nx >= 0 && nx < lim && ny >= 0 && ny < lim && nz >= 0 && nz < lim)
-                    f = ix.find(morton3(nx, ny, nz));
+                    f = ix.find(morton3(x + dx, y + dy, z + dz));
                 slots[static_cast<int64_t>(i) * kNbrSlots + s] = f;
                 ++s;
             }
@@ -359,22 +375,24 @@ __global__ __launch_bounds__(kBlock) void k_p2p_slots(KeyIndex ix, int lim, int3
 
 // M2L slots of every cell of a level: 189 slots in the reference enumeration
 // order (parent neighbour x,y,z from -1 to 1, then cousin 0..7, keeping the
-// non-adjacent ones); slot -> cell index in the level or -1.
+// non-adjacent ones); slot -> cell index in the level or -1.  plim = parents
+// per axis of one set (the range test is set-local, as for the P2P slots).
 __global__ __launch_bounds__(kBlock) void k_m2l_slots(KeyIndex ix, int plim, int32_t* __restrict__ slots) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= ix.n) return;
     int x, y, z;
     demorton3(ix.keys[i], x, y, z);
     const int px = x >> 1, py = y >> 1, pz = z >> 1;
+    const int lx = px & (plim - 1), ly = py & (plim - 1), lz = pz & (plim - 1);
     const int bx = x & 1, by = y & 1, bz = z & 1;
     int s = 0;
     for (int dx = -1; dx <= 1; ++dx)
         for (int dy = -1; dy <= 1; ++dy)
             for (int dz = -1; dz <= 1; ++dz) {
                 if (!dx && !dy && !dz) continue;
-                const int ox = px + dx, oy = py + dy, oz = pz + dz;
+                const int ox = lx + dx, oy = ly + dy, oz = lz + dz;
                 const bool inside = ox >= 0 && ox < plim && oy >= 0 && oy < plim && oz >= 0 && oz < plim;
-                const uint64_t pk = inside ? morton3(ox, oy, oz) : 0ull;
+                const uint64_t pk = inside ? morton3(px + dx, py + dy, pz + dz) : 0ull;
                 for (int cz = 0; cz < 8; ++cz) {
                     const int xd = 2 * dx + ((cz >> 2) & 1) - bx;
                     const int yd = 2 * dy + ((cz >> 1) & 1) - by;
@@ -454,18 +472,21 @@ int make_groups(Context* c, const uint64_t* keys, int64_t n, int shift, uint64_t
 
 }  // namespace
 
-int update_charges_device(Context* c) {
+int update_charges_device(Context* c, int rhs) {
     prof_begin(c, "gather_charges");
     hipLaunchKernelGGL(k_gather_charges, dim3(blocks_for(c->n)), dim3(kBlock), 0, c->stream, c->n,
-                       c->d_perm.as<int32_t>(), c->q, c->d_sorted.as<double>() + 3 * c->n);
+                       c->d_perm.as<int32_t>(), c->q + static_cast<size_t>(rhs) * c->n,
+                       c->d_sorted.as<double>() + 3 * c->n);
     prof_end(c);
     return check_launch(c, "gather_charges");
 }
 
 int build_tree_device(Context* c) {
     const int64_t n = c->n;
-    const int h = c->height();
+    const int h = c->height();  // of the virtual tree
     const int L = h - 1;
+    const int Lloc = c->P.tree_height - 1;  // leaf level inside one set
+    const int top = c->top();
     if (n <= 0) return fail(c, HFMM_ERR_ARG, "no particles");
     if (n > (int64_t(1) << 30)) return fail(c, HFMM_ERR_ARG, "more than 2^30 particles");
     HFMM_TRY(ensure(c, c->d_keys, sizeof(uint64_t) * n));
@@ -480,11 +501,12 @@ int build_tree_device(Context* c) {
 
     // 1. keys (boxCorner = center + (-(width/2)), FGroupTree.hpp:183)
     const double half = c->P.box_width / 2;
-    const double leafw = c->P.box_width / static_cast<double>(1 << L);
+    const double leafw = c->P.box_width / static_cast<double>(1 << Lloc);
     prof_begin(c, "morton");
     hipLaunchKernelGGL(k_morton, dim3(blocks_for(n)), dim3(kBlock), 0, c->stream, n, c->x, c->y, c->z,
                        c->P.box_center[0] + (-half), c->P.box_center[1] + (-half), c->P.box_center[2] + (-half),
-                       c->P.box_width, leafw, 1 << L, keys, vals, err);
+                       c->P.box_width, leafw, Lloc, c->nsets > 1 ? c->d_setoff.as<int64_t>() : nullptr, c->nsets,
+                       keys, vals, err);
     prof_end(c);
     HFMM_TRY(check_launch(c, "morton"));
     int32_t h_err = 0;
@@ -524,7 +546,7 @@ int build_tree_device(Context* c) {
     c->ncell[L] = nl;
     hipLaunchKernelGGL(k_set_int, dim3(1), dim3(1), 0, c->stream, c->d_leafstart.as<int32_t>() + nl,
                        static_cast<int32_t>(n));
-    for (int l = L - 1; l >= 2; --l) {
+    for (int l = L - 1; l >= top; --l) {
         int64_t np = 0;
         int32_t* cs = cstart + static_cast<size_t>(l) * (cap + 1);
         HFMM_TRY(make_groups(c, tkeys + static_cast<size_t>(l + 1) * cap, c->ncell[l + 1], 3,
@@ -533,16 +555,16 @@ int build_tree_device(Context* c) {
         hipLaunchKernelGGL(k_set_int, dim3(1), dim3(1), 0, c->stream, cs + np, static_cast<int32_t>(c->ncell[l + 1]));
         c->cs_off[l] = static_cast<int64_t>(l) * (cap + 1);
     }
-    // pack the cell keys of levels 2..h-1 contiguously
+    // pack the cell keys of levels top..h-1 contiguously
     int64_t off = 0;
     for (int l = 0; l <= L; ++l) {
         c->lvl_off[l] = off;
-        if (l >= 2) off += c->ncell[l];
+        if (l >= top) off += c->ncell[l];
     }
     c->lvl_off[L + 1] = off;
     c->total_cells = off;
     HFMM_TRY(ensure(c, c->d_cellkey, sizeof(uint64_t) * static_cast<size_t>(off)));
-    for (int l = 2; l <= L; ++l)
+    for (int l = top; l <= L; ++l)
         HFMM_HIP(c, hipMemcpyAsync(c->d_cellkey.as<uint64_t>() + c->lvl_off[l], tkeys + static_cast<size_t>(l) * cap,
                                    sizeof(uint64_t) * c->ncell[l], hipMemcpyDeviceToDevice, c->stream));
 
@@ -551,7 +573,7 @@ int build_tree_device(Context* c) {
     int64_t dense_off[kMaxLevels];
     for (int l = 0; l <= L; ++l) {
         dense_off[l] = -1;
-        if (l >= 2 && 3 * l <= 24) {
+        if (l >= top && 3 * l <= 24) {
             dense_off[l] = dense_total;
             dense_total += int64_t(1) << (3 * l);
         }
@@ -559,7 +581,7 @@ int build_tree_device(Context* c) {
     HFMM_TRY(ensure(c, c->d_dense, sizeof(int32_t) * static_cast<size_t>(dense_total + 1)));
     HFMM_HIP(c, hipMemsetAsync(c->d_dense.ptr, 0xFF, sizeof(int32_t) * static_cast<size_t>(dense_total), c->stream));
     KeyIndex ix[kMaxLevels];
-    for (int l = 2; l <= L; ++l) {
+    for (int l = top; l <= L; ++l) {
         ix[l].keys = c->d_cellkey.as<uint64_t>() + c->lvl_off[l];
         ix[l].n = static_cast<int>(c->ncell[l]);
         ix[l].dense = dense_off[l] >= 0 ? c->d_dense.as<int32_t>() + dense_off[l] : nullptr;
@@ -575,7 +597,7 @@ int build_tree_device(Context* c) {
     // 5. P2P slots of the leaves
     HFMM_TRY(ensure(c, c->d_p2p, sizeof(int32_t) * static_cast<size_t>(nl) * kNbrSlots));
     prof_begin(c, "p2p_lists");
-    hipLaunchKernelGGL(k_p2p_slots, dim3(blocks_for(nl)), dim3(kBlock), 0, c->stream, ix[L], 1 << L,
+    hipLaunchKernelGGL(k_p2p_slots, dim3(blocks_for(nl)), dim3(kBlock), 0, c->stream, ix[L], 1 << Lloc,
                        c->d_p2p.as<int32_t>());
     prof_end(c);
     HFMM_TRY(check_launch(c, "p2p_lists"));
@@ -599,17 +621,18 @@ int build_tree_device(Context* c) {
         HFMM_TRY(check_launch(c, "p2p_items"));
     }
 
-    // 6. M2L slots and parity-sorted target order of levels 2..h-1
+    // 6. M2L slots and parity-sorted target order of levels top..h-1
     HFMM_TRY(ensure(c, c->d_m2l, sizeof(int32_t) * static_cast<size_t>(off) * kFarSlots));
     HFMM_TRY(ensure(c, c->d_order, sizeof(int32_t) * static_cast<size_t>(off)));
     HFMM_TRY(ensure(c, c->d_pcount, sizeof(int32_t) * 8 * kMaxLevels));
     HFMM_HIP(c, hipMemsetAsync(c->d_pcount.ptr, 0, sizeof(int32_t) * 8 * kMaxLevels, c->stream));
-    for (int l = 2; l <= L; ++l) {
+    for (int l = top; l <= L; ++l) {
         const int64_t nc = c->ncell[l];
         c->m2l_off[l] = c->lvl_off[l];
         const uint64_t* ck = c->d_cellkey.as<uint64_t>() + c->lvl_off[l];
         prof_begin(c, "m2l_lists");
-        hipLaunchKernelGGL(k_m2l_slots, dim3(blocks_for(nc)), dim3(kBlock), 0, c->stream, ix[l], 1 << (l - 1),
+        hipLaunchKernelGGL(k_m2l_slots, dim3(blocks_for(nc)), dim3(kBlock), 0, c->stream, ix[l],
+                           1 << (l - 1 - c->set_levels),
                            c->d_m2l.as<int32_t>() + static_cast<size_t>(c->m2l_off[l]) * kFarSlots);
         prof_end(c);
         HFMM_TRY(check_launch(c, "m2l_lists"));

@@ -68,6 +68,8 @@ class Engine:
         self.h = h
         self.params = p
         self.n = 0
+        self.nrhs = 1
+        self.nsets = 1
         self.tree_stamp = 0  # counts tree builds (the autograd wrapper checks it)
         self.height = int(tree_height)
         self.order = int(order)
@@ -96,16 +98,32 @@ class Engine:
             raise ValueError("x, y, z, q must have the same length")
         self._check(self._L.hfmm_set_particles(self.h, n, *(_dp(a) for a in arrs)))
         self.n = n
+        self.nrhs = self.nsets = 1
+
+    def set_particle_sets(self, offsets, x, y, z, q):
+        """Independent particle sets (set s = particles offsets[s]..offsets[s+1]-1,
+        each in its own copy of the box) with q of shape (nrhs, n) or (n,)."""
+        arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (x, y, z)]
+        n = len(arrs[0])
+        qq = np.ascontiguousarray(np.atleast_2d(np.asarray(q, dtype=np.float64)))
+        off = np.ascontiguousarray(offsets, dtype=np.int64)
+        if any(len(a) != n for a in arrs) or qq.shape[1] != n:
+            raise ValueError("x, y, z and every charge vector must have the same length")
+        self._check(self._L.hfmm_set_particle_sets(self.h, len(off) - 1, _ip64(off), qq.shape[0], n,
+                                                   *(_dp(a) for a in arrs), _dp(qq)))
+        self.n = n
+        self.nrhs = qq.shape[0]
+        self.nsets = len(off) - 1
 
     def set_particles_device(self, n, x_ptr, y_ptr, z_ptr, q_ptr):
         self._check(self._L.hfmm_set_particles_device(self.h, int(n), x_ptr, y_ptr, z_ptr, q_ptr))
         self.n = int(n)
 
     def set_charges(self, q):
-        """New charges on the built tree (input order); the tree is kept."""
+        """New charges on the built tree (input order, (nrhs, n) or (n,)); the tree is kept."""
         q = np.ascontiguousarray(q, dtype=np.float64)
-        if len(q) != self.n:
-            raise ValueError("one charge per particle")
+        if q.size != self.n * self.nrhs:
+            raise ValueError("one charge per particle and charge vector")
         self._check(self._L.hfmm_set_charges(self.h, _dp(q)))
 
     def build_tree(self):
@@ -118,15 +136,19 @@ class Engine:
     def synchronize(self):
         self._check(self._L.hfmm_synchronize(self.h))
 
+    def _shape(self):
+        return (self.n,) if self.nrhs == 1 else (self.nrhs, self.n)
+
     def results(self):
-        out = [np.empty(self.n) for _ in range(4)]
+        """pot, fx, fy, fz in input order; (nrhs, n) arrays with several charge vectors."""
+        out = [np.empty(self._shape()) for _ in range(4)]
         self._check(self._L.hfmm_get_results(self.h, *(_dp(a) for a in out)))
         return tuple(out)
 
     # ---- inspection -----------------------------------------------------------
     def field(self):
         """grad(phi) at every particle (input order): the force without the target charge."""
-        out = [np.empty(self.n) for _ in range(3)]
+        out = [np.empty(self._shape()) for _ in range(3)]
         self._check(self._L.hfmm_get_field(self.h, *(_dp(a) for a in out)))
         return tuple(out)
 

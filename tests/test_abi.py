@@ -14,7 +14,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 def _declared():
     src = open(os.path.join(ROOT, "include", "scalfmm_hip.h")).read()
-    return sorted(set(re.findall(r"^\s*(?:int|const char\*)\s+(hfmm_\w+)\(", src, flags=re.M)))
+    return sorted(set(re.findall(r"^\s*(?:int32_t|int|const char\*)\s+(hfmm_\w+)\(", src, flags=re.M)))
 
 
 def test_library_exports_header():
@@ -47,6 +47,34 @@ def test_create_rejects_bad_params():
         p = _abi.Params(tree_height=height, order=order, box_width=width, epsilon=0.0, precision=0, device=0)
         h = ctypes.c_void_p()
         assert L.hfmm_create(ctypes.byref(p), ctypes.byref(h)) == _abi.ERR_ARG
+
+
+def test_batch_height_rule_and_argument_checks():
+    """Default set height: the smallest h in [3, 20] with n <= 32 * 8^(h-1);
+    hfmm_batch_create rejects bad set tables before touching a device."""
+    from scalfmm_amd import _abi
+    L = _abi.lib()
+    for n, h in ((1, 3), (1000, 3), (2048, 3), (2049, 4), (16384, 4), (16385, 5), (64000, 5), (131072, 5),
+                 (131073, 6), (10 ** 7, 8)):
+        assert L.hfmm_batch_default_height(n) == h, n
+    p = _abi.Params(tree_height=0, order=5, box_width=1.0, epsilon=0.0, precision=0, device=0)
+    h = ctypes.c_void_p()
+    i64 = ctypes.c_int64
+
+    def create(off, heights=None, nrhs=1):
+        o = (i64 * len(off))(*off)
+        hh = (ctypes.c_int32 * len(heights))(*heights) if heights else None
+        return L.hfmm_batch_create(ctypes.byref(p), len(off) - 1, o, hh, nrhs, ctypes.byref(h))
+
+    assert create([1, 5]) == _abi.ERR_ARG           # must start at 0
+    assert create([0, 5, 3]) == _abi.ERR_ARG        # decreasing
+    assert create([0, 0, 0]) == _abi.ERR_ARG        # no particles
+    assert create([0, 5], heights=[2]) == _abi.ERR_ARG
+    assert create([0, 5], heights=[21]) == _abi.ERR_ARG
+    assert create([0, 5], nrhs=0) == _abi.ERR_ARG
+    assert create([0, 5], nrhs=_abi.MAX_RHS + 1) == _abi.ERR_ARG
+    assert L.hfmm_batch_build(None) == _abi.ERR_ARG
+    assert L.hfmm_batch_last_error(None) == b"null batch"
 
 
 def test_drand48_matches_libc():
