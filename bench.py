@@ -46,8 +46,12 @@ WORKLOADS = {
     "plummer10M-h8-p8": ("plummer", 10_000_000, 8, 8, (156_250, 6)),
     "plummer10M-h8-p5": ("plummer", 10_000_000, 8, 5, (156_250, 6)),
     "uniform1M-h7-p7": ("uniform", 1_000_000, 7, 7, (125_000, 6)),
+    # 4,096 sets of log-uniform sizes in [1e3, 6.4e4], heights 3..5 by size,
+    # 8 charge vectors; the CPU sample is the first 512 sets with one vector
+    "batch4096-p5-r8": ("batch", 4096, None, 5, (512, None)),
 }
-SEED = {"plummer": 3, "uniform": 2}  # the configurations' documented seeds
+SEED = {"plummer": 3, "uniform": 2, "batch": 4}  # the configurations' documented seeds
+BATCH_RHS = 8
 
 
 def _canonical(i, j, k):
@@ -58,7 +62,8 @@ def _canonical(i, j, k):
 
 
 def algorithmic_m2l_flops(engine, height, order):
-    """The reference's M2L flop count summed over this tree's interaction lists."""
+    """The reference's M2L flop count summed over this tree's interaction
+    lists (interactions per relative position counted on the device)."""
     p3 = order ** 3
     rank_of = {}
     for i in range(-3, 4):
@@ -78,9 +83,9 @@ def algorithmic_m2l_flops(engine, height, order):
     total = 0.0
     count = 0
     for level in range(2, height):
-        _, _, pos = engine.m2l_list(level)
-        total += float(np.bincount(pos, minlength=343) @ flops_per_pos)
-        count += len(pos)
+        counts = engine.m2l_counts(level)
+        total += float(counts @ flops_per_pos)
+        count += int(counts.sum())
     return total, count
 
 
@@ -165,21 +170,29 @@ def run_form(parts, h, p, precision, steps, warmup):
     return e, ms, {k: v[0] / steps for k, v in sorted(times.items())}, {k: v[1] / steps for k, v in times.items()}
 
 
-def roofline(engine, kernel_ms, launches, h, p, mixed, workload):
-    """Roofline of the kernel with the largest time share."""
+def roofline(engines, kernel_ms, launches, p, mixed, workload, nrhs=1):
+    """Roofline of the kernel with the largest time share; `engines` are the
+    trees the kernel times cover (one, or the per-height trees of a batch),
+    each evaluated for `nrhs` charge vectors."""
     m2l_name = "m2l_mixed" if mixed else "m2l"
     p2p_small, p2p_big = ("p2p_mixed", "p2p_mixed_big") if mixed else ("p2p", "p2p_big")
     top = max(kernel_ms, key=kernel_ms.get)
     if top == m2l_name:
-        flops, n_int = algorithmic_m2l_flops(engine, h, p)
+        flops = n_int = 0
+        for e in engines:
+            f, c = algorithmic_m2l_flops(e, e.height, p)
+            flops += f * nrhs
+            n_int += c * nrhs
         work = 6 * flops if mixed else flops
         peak = PEAK["bf16_mfma"] if mixed else PEAK["fp64_mfma"]
         kname, bound = ("k_m2l_mixed", "mfma") if mixed else ("k_m2l_f64", "mfma")
         extra = {"m2l_interactions": n_int, "fp64_equivalent_flops_per_step": flops,
                  "work": "6 bf16 partial products per product" if mixed else "FP64 products"}
     elif top in (p2p_small, p2p_big):
-        small, big = p2p_pairs(engine)
-        pairs = big if top == p2p_big else small
+        pairs = 0.0
+        for e in engines:
+            small, big = p2p_pairs(e)
+            pairs += (big if top == p2p_big else small) * nrhs
         work = 34.0 * pairs
         peak = PEAK["fp32_vector"] if mixed else PEAK["fp64_vector"]
         kname = {"p2p_mixed": "k_p2p_mixed", "p2p_mixed_big": "k_p2p_mixed_big", "p2p": "k_p2p_f64",
@@ -195,6 +208,124 @@ def roofline(engine, kernel_ms, launches, h, p, mixed, workload):
            "launches_per_step": launches.get(top), "algorithmic_flops_per_launch": work / max(launches.get(top, 1), 1)}
     out.update(extra)
     return out
+
+
+def run_batch_form(sets, p, precision, steps, warmup):
+    """Build the per-height trees, run and capture the graph, then time
+    `steps` graph launches; one more eager run with HIP events gives the
+    kernel breakdown."""
+    from scalfmm_amd import BatchEngine
+    sizes, x, y, z, q = sets
+    b = BatchEngine(p, sizes, nrhs=q.shape[0], precision=precision)
+    b.set_particles(x, y, z, q)  # host -> HBM once
+    t0 = time.perf_counter()
+    b.build()
+    b.synchronize()
+    build_ms = (time.perf_counter() - t0) * 1e3
+    b.evaluate()  # eager run + capture
+    for _ in range(warmup):
+        b.evaluate()
+    b.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        b.evaluate()
+    b.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / steps
+    trees = b.trees()
+    for t in trees:
+        t.set_profiling(True)
+        t.reset_kernel_times()
+    b.evaluate()
+    b.synchronize()
+    kms, launches = {}, {}
+    for t in trees:
+        for k, (tot, cnt) in t.kernel_times().items():
+            kms[k] = kms.get(k, 0.0) + tot
+            launches[k] = launches.get(k, 0) + cnt
+        t.set_profiling(False)
+    return b, trees, ms, build_ms, dict(sorted(kms.items())), launches
+
+
+def run_batch(args):
+    """The batched configuration: every set and charge vector per graph launch."""
+    from scalfmm_amd import Engine, PRECISION_FP64, PRECISION_MIXED
+    from scalfmm_amd.distributions import batch_sets
+    dist, nsets, _, p, (csets, _) = WORKLOADS[args.workload]
+    print(f"[bench] generating {nsets} sets x {BATCH_RHS} charge vectors", file=sys.stderr, flush=True)
+    sets = batch_sets(nsets, BATCH_RHS, SEED[dist])
+    sizes, x, y, z, q = sets
+    n = int(sizes.sum())
+    mixed = args.precision == "mixed"
+    forms = [args.precision] + (["fp64"] if mixed and not args.no_fp64_compare else [])
+    res = {}
+    for form in forms:
+        print(f"[bench] {form} batch: build, capture, {args.warmup} warmup + {args.steps} timed graph launches",
+              file=sys.stderr, flush=True)
+        b, trees, ms, build_ms, kms, launches = run_batch_form(
+            sets, p, PRECISION_MIXED if form == "mixed" else PRECISION_FP64, args.steps, args.warmup)
+        rl = roofline(trees, kms, launches, p, form == "mixed", args.workload, nrhs=BATCH_RHS)
+        acc = None
+        if not args.no_accuracy:
+            # FP64 direct sums on sampled sets, every charge vector
+            print("[bench] direct sums on 16 sampled sets", file=sys.stderr, flush=True)
+            pot = b.results(fields=("pot",))[0]
+            rng = np.random.default_rng(5)
+            errs = []
+            for s in np.sort(rng.choice(nsets, 16, replace=False)):
+                lo, hi = int(b.offsets[s]), int(b.offsets[s + 1])
+                t = np.sort(rng.choice(hi - lo, min(200, hi - lo), replace=False))
+                ref = Engine(3, 1.0, (0.5, 0.5, 0.5), p)
+                for r in range(BATCH_RHS):
+                    ref.set_particles(x[lo:hi], y[lo:hi], z[lo:hi], q[r, lo:hi])
+                    d = ref.direct(t)[0]
+                    errs.append(float(np.linalg.norm(pot[r, lo + t] - d) / np.linalg.norm(d)))
+                ref.close()
+            acc = {"sets": 16, "targets_per_set": 200, "pot_rel_l2_max": max(errs),
+                   "pot_rel_l2_median": float(np.median(errs))}
+        res[form] = {"value": n * BATCH_RHS / (ms / 1e3) / 1e6, "ms_per_step": ms, "build_ms": build_ms,
+                     "kernel_ms_per_step": kms, "roofline": rl, "accuracy_vs_direct": acc,
+                     "trees": [{"height": t.set_height, "sets": t.nsets, "particles": t.n,
+                                "m2l_variant": t.m2l_variant()[0], "m2l_tune_ms": t.m2l_variant()[1][:3]}
+                               for t in trees]}
+        for t in trees:
+            t.close()
+        b.close()
+    main_form = res[args.precision]
+    out = {
+        "metric": "Mparticle-RHS/s, batched 1/r Cheb FMM (all sets and charge vectors per graph launch)",
+        "value": main_form["value"], "unit": "Mparticle-RHS/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": main_form["ms_per_step"], "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None,
+        "dtype": "bf16x3-mfma/f32 (M2L), f32 (P2P), f64 (rest)" if mixed else "f64",
+        "data": "synthetic",
+        "config": {"workload": args.workload, "sets": nsets, "particles": n, "rhs": BATCH_RHS, "order": p,
+                   "seed": SEED[dist], "heights": "smallest h in [3, 20] with n <= 32 * 8^(h-1)",
+                   "precision": args.precision},
+        "roofline": main_form["roofline"],
+        "build_ms": main_form["build_ms"],
+        "kernel_ms_per_step": main_form["kernel_ms_per_step"],
+        "trees": main_form["trees"],
+        "accuracy_vs_direct": main_form["accuracy_vs_direct"],
+        "fp64_gpu": res.get("fp64") if mixed else None,
+        "mixed_vs_fp64": (main_form["value"] / res["fp64"]["value"]) if "fp64" in res and mixed else None,
+    }
+    if not args.no_cpu_baseline:
+        from oracle import pyoracle
+        from scalfmm_amd import default_height
+        print(f"[bench] cpu baseline (first {csets} sets, one charge vector)", file=sys.stderr, flush=True)
+        cheb = pyoracle.Cheb(p)
+        cores = os.cpu_count() or 1
+        tot, m = 0.0, 0
+        for s in range(csets):
+            lo, hi = int(sets[0][:s].sum()), int(sets[0][:s + 1].sum())
+            st = pyoracle.fmm(x[lo:hi], y[lo:hi], z[lo:hi], q[0, lo:hi], (0.5, 0.5, 0.5), 1.0,
+                              default_height(hi - lo), p, cheb=cheb, nthreads=cores)[4]
+            tot += st.t_total
+            m += hi - lo
+        out["cpu_baseline"] = {"value": m / tot / 1e6, "unit": "Mparticle-RHS/s", "cores": cores, "kind": "port",
+                               "sample": f"oracle FP64 OpenMP FMM, the first {csets} sets ({m} particles), one "
+                                         f"charge vector, one tree per set; {tot:.2f} s"}
+    print(json.dumps(out))
 
 
 def main():
@@ -221,6 +352,9 @@ def main():
 
     from scalfmm_amd import PRECISION_FP64, PRECISION_MIXED
 
+    if WORKLOADS[args.workload][0] == "batch":
+        run_batch(args)
+        return
     dist, n, h, p, (cn, ch) = WORKLOADS[args.workload]
     print(f"[bench] generating {n} {dist} particles", file=sys.stderr, flush=True)
     parts = make_particles(dist, n)
@@ -229,8 +363,9 @@ def main():
           flush=True)
     e, ms, kms, launches = run_form(parts, h, p, PRECISION_MIXED if mixed else PRECISION_FP64, args.steps,
                                     args.warmup)
+    m2l_variant = e.m2l_variant()
     value = n / (ms / 1e3) / 1e6
-    rl = roofline(e, kms, launches, h, p, mixed, args.workload)
+    rl = roofline([e], kms, launches, p, mixed, args.workload)
     got = e.results()
     direct = None
     if not args.no_accuracy:
@@ -245,7 +380,7 @@ def main():
         print("[bench] fp64 form on the same particles", file=sys.stderr, flush=True)
         e64, ms64, kms64, launches64 = run_form(parts, h, p, PRECISION_FP64, args.steps, args.warmup)
         fp64 = {"value": n / (ms64 / 1e3) / 1e6, "ms_per_step": ms64, "kernel_ms_per_step": kms64,
-                "roofline": roofline(e64, kms64, launches64, h, p, False, args.workload)}
+                "roofline": roofline([e64], kms64, launches64, p, False, args.workload)}
         got64 = e64.results()
         e64.close()
 
@@ -271,6 +406,7 @@ def main():
                    "seed": SEED[dist], "height": h, "order": p, "epsilon": 10.0 ** -p, "precision": args.precision},
         "roofline": rl,
         "kernel_ms_per_step": kms,
+        "m2l_variant": {"chosen": m2l_variant[0], "tune_ms": m2l_variant[1][:3]} if mixed else None,
         "fp64_gpu": fp64,
         "mixed_vs_fp64": (value / fp64["value"]) if fp64 else None,
         "accuracy_vs_direct": acc,

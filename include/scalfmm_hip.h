@@ -183,6 +183,11 @@ int hfmm_get_p2p_list(hfmm_ctx* ctx, int64_t* offsets, int64_t* nbr, int32_t* po
 /* M2L list of a level (2..height-1) in CSR form and reference order:
  * offsets[n_cells+1], src[] = cell index in the level, pos[] = 0..342. */
 int hfmm_get_m2l_list(hfmm_ctx* ctx, int32_t level, int64_t* offsets, int64_t* src, int32_t* pos);
+/* Interactions of one level per relative position: counts[343] (position
+ * ((dx+3)*7 + dy+3)*7 + dz+3), counted on the device; the reference's flop
+ * count of the level is sum_pos counts[pos] (4 p^3 k_pos - k_pos - p^3)
+ * (FChebFlopsSymKernel.hpp:62-63). */
+int hfmm_get_m2l_counts(hfmm_ctx* ctx, int32_t level, int64_t* counts);
 /* Compressed operator of canonical position idx (FChebSymM2LHandler.hpp:165-189):
  * *rank, and K = [U | V] column-major nnodes x rank each (2*rank*nnodes doubles,
  * K may be NULL to query the rank). */
@@ -215,6 +220,18 @@ int hfmm_selftest_mfma_f64(hfmm_ctx* ctx, double* out);
  * D = A B (16 x 16, row-major) through the operand layout the mixed-precision
  * M2L kernel assumes; no counterpart in the reference. */
 int hfmm_selftest_mfma_bf16(hfmm_ctx* ctx, float* out);
+
+/* ---- M2L kernel variants (autotuned) ---------------------------------------- */
+/* The mixed-precision M2L has several tile shapes: 0 = four waves share 64
+ * targets (every order), 1 = one 16-target tile per wave (orders 5 and 6),
+ * 2 = two 16-target tiles per wave (order 5).  Variant -1 (the default) times
+ * every available variant once at the next M2L on the actual tree and keeps
+ * the fastest for the context's later evaluations.  The FP64 form has one
+ * kernel (its variant stays -1).  hfmm_get_m2l_variant also returns the
+ * times (ms) the last tuning measured per variant (4 floats, 0 = not run).
+ * No counterpart in the reference (its M2L has one CPU implementation). */
+int hfmm_set_m2l_variant(hfmm_ctx* ctx, int32_t variant);
+int hfmm_get_m2l_variant(hfmm_ctx* ctx, int32_t* variant, float* tune_ms);
 
 /* ---- batches of particle sets ----------------------------------------------- */
 /* Many independent particle sets of uneven sizes, each with nrhs charge
@@ -249,6 +266,11 @@ int hfmm_batch_synchronize(hfmm_batch* b);
 int hfmm_batch_get_results(hfmm_batch* b, double* pot, double* fx, double* fy, double* fz);
 /* Number of per-height trees and whether the next evaluate launches a graph. */
 int hfmm_batch_info(const hfmm_batch* b, int32_t* ntrees, int32_t* graph_ready);
+/* The context of tree i (0 .. ntrees-1, increasing height), owned by the batch,
+ * for inspection (tree info, lists, counts) and profiling: while a tree has
+ * profiling enabled, hfmm_batch_evaluate runs the passes eagerly so that the
+ * kernel events are recorded. */
+int hfmm_batch_tree(hfmm_batch* b, int32_t i, hfmm_ctx** tree, int32_t* height, int32_t* nsets);
 
 /* Library version string. */
 const char* hfmm_version(void);

@@ -36,6 +36,7 @@ class BatchEngine:
         self.nsets = len(sizes)
         self.n = int(self.offsets[-1])
         self.nrhs = int(nrhs)
+        self.order = int(order)
         p = _abi.Params()
         p.tree_height = 0
         p.order = int(order)
@@ -91,10 +92,26 @@ class BatchEngine:
     def synchronize(self):
         self._check(self._L.hfmm_batch_synchronize(self.h))
 
-    def results(self):
-        out = [np.empty((self.nrhs, self.n)) for _ in range(4)]
-        self._check(self._L.hfmm_batch_get_results(self.h, *(_dp(a) for a in out)))
+    def results(self, fields=("pot", "fx", "fy", "fz")):
+        """(nrhs, n) arrays in input order for the requested fields (None for the others)."""
+        names = ("pot", "fx", "fy", "fz")
+        out = [np.empty((self.nrhs, self.n)) if k in fields else None for k in names]
+        self._check(self._L.hfmm_batch_get_results(self.h, *(_dp(a) if a is not None else None for a in out)))
         return tuple(out)
+
+    def trees(self):
+        """Views (Engine, not owning) of the per-height trees, increasing
+        height: tree inspection, M2L counts and kernel profiling."""
+        from .fmm import Engine
+        out = []
+        for i in range(self.info()["trees"]):
+            h, hh, ns = C.c_void_p(), C.c_int32(), C.c_int32()
+            self._check(self._L.hfmm_batch_tree(self.h, i, C.byref(h), C.byref(hh), C.byref(ns)))
+            e = Engine._borrowed(h, hh.value, self.order, self)
+            e.set_height = hh.value
+            e.nsets = ns.value
+            out.append(e)
+        return out
 
     def info(self):
         nt, gr = C.c_int32(), C.c_int32()

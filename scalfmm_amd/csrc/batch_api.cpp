@@ -192,6 +192,14 @@ int hfmm_batch_evaluate(hfmm_batch* b, uint32_t ops) {
     if (!b->built) return bfail(b, HFMM_ERR_STATE, "evaluate before build");
     if (ops & ~HFMM_NEAR_AND_FAR) return bfail(b, HFMM_ERR_ARG, "unknown operation bits");
     if (hipSetDevice(b->P.device) != hipSuccess) return bfail(b, HFMM_ERR_DEVICE, "hipSetDevice failed");
+    // profiling brackets launches with events: such runs stay eager
+    bool profiling = false;
+    for (const Group& g : b->groups) profiling = profiling || hfmm::context_of(g.ctx)->profiling;
+    if (profiling) {
+        const int st = enqueue_all(b, ops);
+        if (st == HFMM_OK) b->evaluated = true;
+        return st;
+    }
     if (b->exec && b->graph_ops == ops) {
         const hipError_t e = hipGraphLaunch(b->exec, b->stream);
         if (e != hipSuccess) return bfail(b, HFMM_ERR_DEVICE, std::string("hipGraphLaunch: ") + hipGetErrorString(e));
@@ -263,6 +271,15 @@ int hfmm_batch_get_results(hfmm_batch* b, double* pot, double* fx, double* fy, d
                 }
         }
     }
+    return HFMM_OK;
+}
+
+int hfmm_batch_tree(hfmm_batch* b, int32_t i, hfmm_ctx** tree, int32_t* height, int32_t* nsets) {
+    if (!b || !tree) return HFMM_ERR_ARG;
+    if (i < 0 || i >= static_cast<int32_t>(b->groups.size())) return bfail(b, HFMM_ERR_ARG, "no such tree");
+    *tree = b->groups[i].ctx;
+    if (height) *height = b->groups[i].height;
+    if (nsets) *nsets = static_cast<int32_t>(b->groups[i].sets.size());
     return HFMM_OK;
 }
 

@@ -76,12 +76,15 @@ __device__ __forceinline__ void split3d(double x, uint16_t& h, uint16_t& m, uint
 // with W3 = W (x) W (x) W the nodal -> modal transform of one dimension,
 // W[a][j] = (a ? 2 : 1) / p T_a(x_j) (discrete orthogonality of T at the
 // roots), node / mode index n = n1 + p n2 + p^2 n3.  The padded permutation
-// entries point at the zero row nn of U_c / V_c.
+// entries point at the zero row nn of U_c / V_c.  With `interleave` the rank
+// columns of U_pos are stored in the order the wave-level kernel consumes
+// them: in each block of 32, rank 16 h + 4 g + e (h < 2, g < 4, e < 4) at
+// column 8 g + 4 h + e.
 __global__ __launch_bounds__(256) void k_pack_positions(const int16_t* __restrict__ plist, const int32_t* __restrict__ pkt,
                                                        const int64_t* __restrict__ posoff, const uint32_t* __restrict__ pidx,
                                                        const uint16_t* __restrict__ pvec, const int64_t* __restrict__ opoff,
                                                        const double* __restrict__ ops, const double* __restrict__ Troots,
-                                                       int p, int LD, uint16_t* __restrict__ opm) {
+                                                       int p, int LD, int interleave, uint16_t* __restrict__ opm) {
     __shared__ double sW[16 * 16];
     const int nn = p * p * p;
     for (int i = threadIdx.x; i < p * p; i += blockDim.x) sW[i] = (i >= p ? 2. : 1.) / p * Troots[i];
@@ -114,7 +117,9 @@ __global__ __launch_bounds__(256) void k_pack_positions(const int16_t* __restric
                     v += w23 * t;
                 }
         }
-        split3d(v, u[e], u[nu + e], u[2 * nu + e]);
+        const int kk = interleave ? 32 * (k / 32) + 8 * ((k % 16) / 4) + 4 * ((k % 32) / 16) + (k % 4) : k;
+        const int64_t d = static_cast<int64_t>(m) * kp32 + kk;
+        split3d(v, u[d], u[nu + d], u[2 * nu + d]);
     }
 }
 
@@ -275,9 +280,11 @@ __device__ __forceinline__ void mix_segment(MixState<P, COLS>& st, int i0, int i
     for (int i = i0; i < i1; ++i) {
         const uint16_t* Mrow[COLS];
         bool any = false;
+        bool has[COLS];  // this lane's target has a source in this slot
 #pragma unroll
         for (int ct = 0; ct < COLS; ++ct) {
             const int s = nsrc[ct];
+            has[ct] = s >= 0;
             any |= s >= 0;
             // a missing source reads the zero multipole (no divergent branches)
             Mrow[ct] = (s >= 0 ? Mbl + static_cast<int64_t>(s) * LD : Mbz) + 8 * kq;
@@ -382,12 +389,17 @@ __device__ __forceinline__ void mix_segment(MixState<P, COLS>& st, int i0, int i
                 }
             }
         }
+        // (a target without a source here keeps its sum and compensation as
+        // they are, whatever the other targets of the tile do)
 #pragma unroll
         for (int ct = 0; ct < COLS; ++ct) {
             const f4 y = fresh[ct] - st.comp[ct];
             const f4 tt = st.acc[0][ct] + y;
-            st.comp[ct] = (tt - st.acc[0][ct]) - y;
-            st.acc[0][ct] = tt;
+            const f4 cn = (tt - st.acc[0][ct]) - y;
+            if (has[ct]) {
+                st.comp[ct] = cn;
+                st.acc[0][ct] = tt;
+            }
         }
     }
 }
@@ -488,6 +500,223 @@ __global__ __launch_bounds__(256) void k_m2l_mixed(M2LLevels lv, const int32_t* 
                 }
             }
         }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// M2L for small orders (p <= 6): every wave owns CT column tiles of 16
+// targets and runs both GEMMs of each slot by itself, with no LDS traffic and
+// no barrier.  GEMM1 (C = VT_pos M_src) covers the whole node range, and its
+// accumulator registers are GEMM2's B operand as they stand: a lane holds
+// rows 4 g .. 4 g + 3 of every rank tile of its target column, and U_pos is
+// stored with its rank columns interleaved (k_pack_positions, `interleave`)
+// so that K elements 8 g .. 8 g + 7 of a 32-wide block are rows 4 g .. +3 of
+// rank tiles 2 s and 2 s + 1.  The first four row tiles of the modal output
+// (modes 0..63) are accumulated per slot from zero and added with
+// compensation, as in the four-wave kernel.
+template <int P, int CT>
+struct WaveMixState {
+    static constexpr int NN = P * P * P;
+    static constexpr int LDC = (NN + 15) / 16 * 16;
+    static constexpr int NT = LDC / 16;          // row tiles of the output
+    static constexpr int NC = NT < 4 ? NT : 4;   // compensated row tiles
+    static constexpr int KB = LDC / 32;          // 32-node blocks of GEMM1
+    f4 acc[NT][CT];
+    f4 comp[NC][CT];
+};
+
+template <int P, int CT, int KT>
+__device__ __forceinline__ void wave_segment(WaveMixState<P, CT>& st, int i0, int i1, const int16_t* s_slot,
+                                             const int64_t* s_off, const int32_t* lslots, const int (&tgt)[CT],
+                                             int (&nsrc)[CT], const uint16_t* __restrict__ Mbl,
+                                             const uint16_t* __restrict__ Mbz, int64_t plane,
+                                             const uint16_t* __restrict__ opm, int col, int kq) {
+    constexpr int NT = WaveMixState<P, CT>::NT;
+    constexpr int NC = WaveMixState<P, CT>::NC;
+    constexpr int KB = WaveMixState<P, CT>::KB;
+    constexpr int LD = WaveMixState<P, CT>::LDC;
+    constexpr int KP16 = 16 * KT;       // GEMM1 rows: rank padded to 16
+    constexpr int KB32 = (KT + 1) / 2;  // GEMM2 K blocks: rank padded to 32
+    constexpr int KP32 = 32 * KB32;
+    for (int i = i0; i < i1; ++i) {
+        const uint16_t* Mrow[CT];
+        bool any = false;
+        bool has[CT];  // this lane's target has a source in this slot
+#pragma unroll
+        for (int ct = 0; ct < CT; ++ct) {
+            const int s = nsrc[ct];
+            has[ct] = s >= 0;
+            any |= s >= 0;
+            Mrow[ct] = (s >= 0 ? Mbl + static_cast<int64_t>(s) * LD : Mbz) + 8 * kq;
+            nsrc[ct] = (tgt[ct] >= 0 && i + 1 < kFarSlots)
+                           ? lslots[static_cast<int64_t>(tgt[ct]) * kFarSlots + s_slot[i + 1]]
+                           : -1;
+        }
+        if (__ballot(any) == 0) continue;  // uniform over the wave
+        const uint16_t* VT = opm + s_off[i];
+        const uint16_t* U = VT + 3 * static_cast<int64_t>(KP16) * LD;
+        f4 c1[KT][CT];
+#pragma unroll
+        for (int rt = 0; rt < KT; ++rt)
+#pragma unroll
+            for (int ct = 0; ct < CT; ++ct) c1[rt][ct] = f4{0.f, 0.f, 0.f, 0.f};
+        // GEMM1: fragments of block kb + 1 are requested before block kb's MFMAs
+        bf8 b3[2][3][CT];
+        load_mfrag<CT>(b3[0], Mrow, plane, 0);
+#pragma unroll
+        for (int kb = 0; kb < KB; ++kb) {
+            if (kb + 1 < KB) load_mfrag<CT>(b3[(kb + 1) & 1], Mrow, plane, kb + 1);
+#pragma unroll
+            for (int rt = 0; rt < KT; ++rt) {
+                bf8 a[3];
+                load_vfrag<KP16>(a, VT, LD, rt, col, kq, kb);
+#pragma unroll
+                for (int ct = 0; ct < CT; ++ct)
+                    c1[rt][ct] = mma6(a[0], a[1], a[2], b3[kb & 1][0][ct], b3[kb & 1][1][ct], b3[kb & 1][2][ct],
+                                      c1[rt][ct]);
+            }
+        }
+        // GEMM2's B fragments straight from the accumulators, split in three
+        bf8 bh[KB32][CT], bm[KB32][CT], bl[KB32][CT];
+#pragma unroll
+        for (int sb = 0; sb < KB32; ++sb)
+#pragma unroll
+            for (int ct = 0; ct < CT; ++ct)
+#pragma unroll
+                for (int e = 0; e < 8; ++e) {
+                    const int rt = 2 * sb + (e >> 2);
+                    const float v = rt < KT ? c1[rt < KT ? rt : 0][ct][e & 3] : 0.f;
+                    __bf16 h, m, l;
+                    split3(v, h, m, l);
+                    bh[sb][ct][e] = h;
+                    bm[sb][ct][e] = m;
+                    bl[sb][ct][e] = l;
+                }
+        f4 fresh[NC][CT];
+#pragma unroll
+        for (int t = 0; t < NC; ++t)
+#pragma unroll
+            for (int ct = 0; ct < CT; ++ct) fresh[t][ct] = f4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int t = 0; t < NT; ++t) {
+#pragma unroll
+            for (int sb = 0; sb < KB32; ++sb) {
+                const uint16_t* a = U + static_cast<int64_t>(16 * t + col) * KP32 + 32 * sb + 8 * kq;
+                const bf8 ah = ld8(a), am = ld8(a + static_cast<int64_t>(LD) * KP32),
+                          al = ld8(a + 2 * static_cast<int64_t>(LD) * KP32);
+#pragma unroll
+                for (int ct = 0; ct < CT; ++ct) {
+                    if (t < NC)
+                        fresh[t][ct] = mma6(ah, am, al, bh[sb][ct], bm[sb][ct], bl[sb][ct], fresh[t][ct]);
+                    else
+                        st.acc[t][ct] = mma6(ah, am, al, bh[sb][ct], bm[sb][ct], bl[sb][ct], st.acc[t][ct]);
+                }
+            }
+        }
+#pragma unroll
+        for (int t = 0; t < NC; ++t)
+#pragma unroll
+            for (int ct = 0; ct < CT; ++ct) {
+                const f4 y = fresh[t][ct] - st.comp[t][ct];
+                const f4 tt = st.acc[t][ct] + y;
+                const f4 cn = (tt - st.acc[t][ct]) - y;
+                if (has[ct]) {  // targets without a source here stay as they are
+                    st.comp[t][ct] = cn;
+                    st.acc[t][ct] = tt;
+                }
+            }
+    }
+}
+
+template <int P, int CT>
+__global__ __launch_bounds__(256) void k_m2l_mixed_wave(M2LLevels lv, const int32_t* __restrict__ slots,
+                                                       const int32_t* __restrict__ order,
+                                                       const int32_t* __restrict__ pcount,
+                                                       const int16_t* __restrict__ slotord,
+                                                       const int32_t* __restrict__ slotseg,
+                                                       const int64_t* __restrict__ posoff,
+                                                       const uint16_t* __restrict__ opm, const uint16_t* __restrict__ Mb,
+                                                       int64_t plane, int64_t zero_cell, double* __restrict__ L) {
+    constexpr int NN = P * P * P;
+    constexpr int LD = WaveMixState<P, CT>::LDC;
+    constexpr int NT = WaveMixState<P, CT>::NT;
+    constexpr int NC = WaveMixState<P, CT>::NC;
+    constexpr int W = kM2LWaves;
+    constexpr int TCW = 16 * CT;  // targets per wave
+    constexpr int TC = W * TCW;   // targets per workgroup
+    __shared__ int16_t s_slot[kFarSlots];
+    __shared__ int64_t s_off[kFarSlots];
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int col = lane & 15, kq = lane >> 4;
+
+    const int bid = blockIdx.x;
+    if (bid >= lv.tile_begin[lv.nlev]) return;
+    int li = 0;
+    while (bid >= lv.tile_begin[li + 1]) ++li;
+    const int lb = bid - lv.tile_begin[li];
+    const int e = lb % 8;  // parity: consecutive workgroups land on different XCDs
+    const int ptile = lb / 8;
+    const int32_t* pc = pcount + lv.pcount_off[li];
+    int first = 0;
+    for (int q = 0; q < e; ++q) first += pc[q];
+    const int cnt = min(TC, pc[e] - ptile * TC);
+    if (cnt <= 0) return;
+    first += ptile * TC;
+    for (int i = tid; i < kFarSlots; i += 64 * W) {
+        s_slot[i] = slotord[(e * kFarSlots + i) * 2];
+        s_off[i] = posoff[slotord[(e * kFarSlots + i) * 2 + 1]];
+    }
+    int seg[5];
+#pragma unroll
+    for (int q = 0; q < 5; ++q) seg[q] = slotseg[e * 5 + q];
+    const int64_t coff = lv.cell_off[li];
+    const int32_t* lslots = slots + coff * kFarSlots;
+    const uint16_t* Mbl = Mb + coff * LD;
+    const uint16_t* Mbz = Mb + zero_cell * LD;
+    double* Ll = L + coff * LD;
+    int tgt[CT];
+    bool mine = false;
+#pragma unroll
+    for (int ct = 0; ct < CT; ++ct) {
+        const int cc = TCW * wave + 16 * ct + col;
+        tgt[ct] = cc < cnt ? order[coff + first + cc] : -1;
+        mine |= tgt[ct] >= 0;
+    }
+    __syncthreads();  // the slot tables; no barrier follows
+    if (__ballot(mine) == 0) return;  // uniform over the wave
+    int nsrc[CT];
+#pragma unroll
+    for (int ct = 0; ct < CT; ++ct)
+        nsrc[ct] = tgt[ct] >= 0 ? lslots[static_cast<int64_t>(tgt[ct]) * kFarSlots + s_slot[0]] : -1;
+    WaveMixState<P, CT> st;
+#pragma unroll
+    for (int t = 0; t < NT; ++t)
+#pragma unroll
+        for (int ct = 0; ct < CT; ++ct) st.acc[t][ct] = f4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int t = 0; t < NC; ++t)
+#pragma unroll
+        for (int ct = 0; ct < CT; ++ct) st.comp[t][ct] = f4{0.f, 0.f, 0.f, 0.f};
+    wave_segment<P, CT, 1>(st, seg[0], seg[1], s_slot, s_off, lslots, tgt, nsrc, Mbl, Mbz, plane, opm, col, kq);
+    wave_segment<P, CT, 2>(st, seg[1], seg[2], s_slot, s_off, lslots, tgt, nsrc, Mbl, Mbz, plane, opm, col, kq);
+    wave_segment<P, CT, 3>(st, seg[2], seg[3], s_slot, s_off, lslots, tgt, nsrc, Mbl, Mbz, plane, opm, col, kq);
+    wave_segment<P, CT, 4>(st, seg[3], seg[4], s_slot, s_off, lslots, tgt, nsrc, Mbl, Mbz, plane, opm, col, kq);
+    const double scale = lv.scale[li];
+#pragma unroll
+    for (int ct = 0; ct < CT; ++ct) {
+        if (tgt[ct] < 0) continue;
+#pragma unroll
+        for (int t = 0; t < NT; ++t)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int row = 16 * t + 4 * kq + r;
+                if (row < NN)
+                    Ll[static_cast<int64_t>(tgt[ct]) * LD + row] =
+                        scale * (t < NC ? static_cast<double>(st.acc[t][ct][r]) - static_cast<double>(st.comp[t][ct][r])
+                                        : static_cast<double>(st.acc[t][ct][r]));
+            }
     }
 }
 
@@ -709,7 +938,7 @@ __global__ void k_mfma_probe_bf16(float* out) {
 inline unsigned blocks_of(int64_t n) { return static_cast<unsigned>((n + 255) / 256); }
 
 template <int P, int COLS>
-int m2l_mixed_impl(Context* c) {
+int m2l_mixed_launch(Context* c) {
     int tiles = 0;
     const M2LLevels lv = make_levels(c, 16 * COLS, tiles);
     const int CR = ((c->kpad_max + 31) & ~31) + 4;
@@ -717,14 +946,95 @@ int m2l_mixed_impl(Context* c) {
     static int lds_limit[64] = {0};
     HFMM_TRY(raise_lds_limit(c, reinterpret_cast<const void*>(&k_m2l_mixed<P, COLS>), shmem, lds_limit));
     const int64_t plane = (c->total_cells + 1) * static_cast<int64_t>(c->LD);
-    prof_begin(c, "m2l_mixed");
     hipLaunchKernelGGL((k_m2l_mixed<P, COLS>), dim3(static_cast<unsigned>(tiles)), dim3(64 * kM2LWaves), shmem,
                        c->stream, lv, c->d_m2l.as<int32_t>(), c->d_order.as<int32_t>(), c->d_pcount.as<int32_t>(),
                        c->d_slotord.as<int16_t>(), c->d_slotseg.as<int32_t>(), c->d_posoff.as<int64_t>(),
                        c->d_opm.as<uint16_t>(), c->d_Mb.as<uint16_t>(), plane, c->total_cells, c->d_L.as<double>(),
                        c->LD, CR);
+    return check_launch(c, "m2l_mixed");
+}
+
+template <int P, int CT>
+int m2l_mixed_wave_launch(Context* c) {
+    if (c->LD != WaveMixState<P, CT>::LDC) return fail(c, HFMM_ERR_ARG, "wave-level M2L: padded node count");
+    int tiles = 0;
+    const M2LLevels lv = make_levels(c, 16 * CT * kM2LWaves, tiles);
+    const int64_t plane = (c->total_cells + 1) * static_cast<int64_t>(c->LD);
+    hipLaunchKernelGGL((k_m2l_mixed_wave<P, CT>), dim3(static_cast<unsigned>(tiles)), dim3(64 * kM2LWaves), 0,
+                       c->stream, lv, c->d_m2l.as<int32_t>(), c->d_order.as<int32_t>(), c->d_pcount.as<int32_t>(),
+                       c->d_slotord.as<int16_t>(), c->d_slotseg.as<int32_t>(), c->d_posoff.as<int64_t>(),
+                       c->d_opm2.as<uint16_t>(), c->d_Mb.as<uint16_t>(), plane, c->total_cells, c->d_L.as<double>());
+    return check_launch(c, "m2l_mixed");
+}
+
+// Mixed M2L variants (the tile shapes the autotuner picks from):
+//   0  four waves share 64 targets, GEMM1 split over the waves (every order)
+//   1  one 16-target tile per wave, waves independent (p <= 6)
+//   2  two 16-target tiles per wave (p = 5)
+template <int P>
+int m2l_mixed_variant(Context* c, int v) {
+    switch (v) {
+        case 0: return m2l_mixed_launch<P, 4>(c);
+        case 1:
+            if constexpr (P <= kWaveM2LMaxOrder) return m2l_mixed_wave_launch<P, 1>(c);
+            break;
+        case 2:
+            if constexpr (P <= 5) return m2l_mixed_wave_launch<P, 2>(c);
+            break;
+        default: break;
+    }
+    return fail(c, HFMM_ERR_ARG, "mixed M2L variant not available at this order");
+}
+
+constexpr int kMixedVariants = 3;
+inline bool mixed_variant_ok(int p, int v) { return v == 0 || (v == 1 && p <= kWaveM2LMaxOrder) || (v == 2 && p <= 5); }
+
+// Times every available variant once on the current tree (each writes every
+// local of the M2L levels, so the runs are interchangeable) and keeps the
+// fastest; the choice holds for the context's later evaluations.
+template <int P>
+int m2l_mixed_autotune(Context* c) {
+    hipEvent_t ev[2 * kMixedVariants];
+    for (auto& e : ev) HFMM_HIP(c, hipEventCreate(&e));
+    float best = 0.f;
+    int pick = 0;
+    int st = HFMM_OK;
+    for (int v = 0; v < kMixedVariants && st == HFMM_OK; ++v) {
+        if (!mixed_variant_ok(P, v)) continue;
+        (void)hipEventRecord(ev[2 * v], c->stream);
+        st = m2l_mixed_variant<P>(c, v);
+        (void)hipEventRecord(ev[2 * v + 1], c->stream);
+        if (st != HFMM_OK) break;
+        if (hipEventSynchronize(ev[2 * v + 1]) != hipSuccess) {
+            st = fail(c, HFMM_ERR_DEVICE, "autotune: event wait failed");
+            break;
+        }
+        float ms = 0.f;
+        (void)hipEventElapsedTime(&ms, ev[2 * v], ev[2 * v + 1]);
+        c->m2l_tune_ms[v] = ms;
+        if (v == 0 || ms < best) {
+            best = ms;
+            pick = v;
+        }
+    }
+    for (auto& e : ev) (void)hipEventDestroy(e);
+    if (st != HFMM_OK) return st;
+    c->m2l_variant = pick;
+    return HFMM_OK;
+}
+
+template <int P>
+int m2l_mixed_impl(Context* c) {
+    if (c->m2l_variant < 0) {
+        HFMM_TRY(m2l_mixed_autotune<P>(c));
+    } else if (!mixed_variant_ok(P, c->m2l_variant)) {
+        return fail(c, HFMM_ERR_ARG, "mixed M2L variant not available at this order");
+    }
+    // (the autotuned run already wrote the locals; the chosen variant runs again
+    // so that the profile and the results come from one launch of it)
+    prof_begin(c, "m2l_mixed");
+    HFMM_TRY(m2l_mixed_variant<P>(c, c->m2l_variant));
     prof_end(c);
-    HFMM_TRY(check_launch(c, "m2l_mixed"));
     // the M2L targets: every cell of levels top..h-1, contiguous in the cell arrays
     const int64_t first = c->lvl_off[c->top()], ncells = c->lvl_off[c->height()] - first;
     prof_begin(c, "modal_to_nodal");
@@ -766,14 +1076,23 @@ int prepare_mixed_operators(Context* c) {
         HFMM_TRY(ensure(c, d_pkt, sizeof(int32_t) * pkt.size()));
         HFMM_TRY(ensure(c, c->d_posoff, sizeof(int64_t) * posoff.size()));
         HFMM_TRY(ensure(c, c->d_opm, sizeof(uint16_t) * static_cast<size_t>(total)));
+        if (op.p <= kWaveM2LMaxOrder) HFMM_TRY(ensure(c, c->d_opm2, sizeof(uint16_t) * static_cast<size_t>(total)));
         HFMM_HIP(c, hipMemcpy(d_plist.ptr, plist.data(), sizeof(int16_t) * plist.size(), hipMemcpyHostToDevice));
         HFMM_HIP(c, hipMemcpy(d_pkt.ptr, pkt.data(), sizeof(int32_t) * pkt.size(), hipMemcpyHostToDevice));
         HFMM_HIP(c, hipMemcpy(c->d_posoff.ptr, posoff.data(), sizeof(int64_t) * posoff.size(), hipMemcpyHostToDevice));
         hipLaunchKernelGGL(k_pack_positions, dim3(static_cast<unsigned>(plist.size())), dim3(256), 0, c->stream,
                            d_plist.as<int16_t>(), d_pkt.as<int32_t>(), c->d_posoff.as<int64_t>(),
                            c->d_pidx.as<uint32_t>(), c->d_pvec.as<uint16_t>(), c->d_opoff.as<int64_t>(),
-                           c->d_ops.as<double>(), c->d_Troots.as<double>(), op.p, LD, c->d_opm.as<uint16_t>());
+                           c->d_ops.as<double>(), c->d_Troots.as<double>(), op.p, LD, 0, c->d_opm.as<uint16_t>());
         HFMM_TRY(check_launch(c, "pack_positions"));
+        if (op.p <= kWaveM2LMaxOrder) {  // the same operators for the wave-level kernel
+            hipLaunchKernelGGL(k_pack_positions, dim3(static_cast<unsigned>(plist.size())), dim3(256), 0, c->stream,
+                               d_plist.as<int16_t>(), d_pkt.as<int32_t>(), c->d_posoff.as<int64_t>(),
+                               c->d_pidx.as<uint32_t>(), c->d_pvec.as<uint16_t>(), c->d_opoff.as<int64_t>(),
+                               c->d_ops.as<double>(), c->d_Troots.as<double>(), op.p, LD, 1,
+                               c->d_opm2.as<uint16_t>());
+            HFMM_TRY(check_launch(c, "pack_positions"));
+        }
         HFMM_HIP(c, hipStreamSynchronize(c->stream));
         return HFMM_OK;
     };
@@ -796,12 +1115,18 @@ int run_split_multipoles(Context* c) {
 int run_m2l_mixed(Context* c) {
     if (c->kpad_max > 64) return fail(c, HFMM_ERR_ARG, "operator rank above 64 is not supported");
     switch (c->order()) {
-        case 5: return m2l_mixed_impl<5, 4>(c);
-        case 6: return m2l_mixed_impl<6, 4>(c);
-        case 7: return m2l_mixed_impl<7, 4>(c);
-        case 8: return m2l_mixed_impl<8, 4>(c);
+        case 5: return m2l_mixed_impl<5>(c);
+        case 6: return m2l_mixed_impl<6>(c);
+        case 7: return m2l_mixed_impl<7>(c);
+        case 8: return m2l_mixed_impl<8>(c);
         default: return fail(c, HFMM_ERR_ARG, "order not compiled (5..8)");
     }
+}
+
+int m2l_variant_count(Context* c) { return c->P.precision == HFMM_PRECISION_MIXED ? kMixedVariants : 1; }
+bool m2l_variant_available(Context* c, int v) {
+    if (c->P.precision != HFMM_PRECISION_MIXED) return v == 0;
+    return v >= 0 && v < kMixedVariants && mixed_variant_ok(c->order(), v);
 }
 
 int run_p2p_mixed(Context* c) {

@@ -194,7 +194,7 @@ static int upload_operators(Context* c) {
 
 static void free_all(Context* c) {
     DevBuf* bufs[] = {&c->d_Troots, &c->d_child, &c->d_pidx, &c->d_pvec, &c->d_rank, &c->d_opoff, &c->d_ops,
-                      &c->d_slotpos, &c->d_slotord, &c->d_slotseg, &c->d_posoff, &c->d_opm, &c->d_in, &c->d_keys, &c->d_keys2, &c->d_vals, &c->d_vals2, &c->d_hist,
+                      &c->d_slotpos, &c->d_slotord, &c->d_slotseg, &c->d_posoff, &c->d_opm, &c->d_opm2, &c->d_in, &c->d_keys, &c->d_keys2, &c->d_vals, &c->d_vals2, &c->d_hist,
                       &c->d_scan_part, &c->d_qin, &c->d_flags, &c->d_scan, &c->d_sorted, &c->d_perm, &c->d_pleaf,
                       &c->d_count, &c->d_cellkey, &c->d_cellkey_tmp, &c->d_childstart, &c->d_leafstart,
                       &c->d_p2p, &c->d_p2pitems, &c->d_m2l, &c->d_order, &c->d_pcount, &c->d_dense, &c->d_M, &c->d_L, &c->d_Mb, &c->d_loc, &c->d_res_sorted,
@@ -224,6 +224,8 @@ int hfmm_create(const hfmm_params* params, hfmm_ctx** out) { return hfmm::create
 }  // extern "C"
 
 namespace hfmm {
+
+Context* context_of(hfmm_ctx* h) { return &h->c; }
 
 int create_context(const hfmm_params* params, hipStream_t shared_stream, hfmm_ctx** out) {
     if (!params || !out) return HFMM_ERR_ARG;
@@ -516,11 +518,10 @@ int hfmm_get_tree_info(hfmm_ctx* h, hfmm_tree_info* info) {
     std::vector<int32_t> s(static_cast<size_t>(c->ncell[L]) * hfmm::kNbrSlots);
     HFMM_HIP(c, hipMemcpy(s.data(), c->d_p2p.ptr, sizeof(int32_t) * s.size(), hipMemcpyDeviceToHost));
     info->p2p_entries = std::count_if(s.begin(), s.end(), [](int32_t v) { return v >= 0; });
-    for (int l = 2; l <= L; ++l) {
-        std::vector<int32_t> m(static_cast<size_t>(c->ncell[l]) * hfmm::kFarSlots);
-        HFMM_HIP(c, hipMemcpy(m.data(), c->d_m2l.as<int32_t>() + static_cast<size_t>(c->m2l_off[l]) * hfmm::kFarSlots,
-                              sizeof(int32_t) * m.size(), hipMemcpyDeviceToHost));
-        info->m2l_entries[l] = std::count_if(m.begin(), m.end(), [](int32_t v) { return v >= 0; });
+    for (int l = c->top(); l <= L; ++l) {
+        int64_t counts[343];
+        HFMM_TRY(hfmm_get_m2l_counts(h, l, counts));
+        for (int pos = 0; pos < 343; ++pos) info->m2l_entries[l] += counts[pos];
     }
     return HFMM_OK;
 }
@@ -619,6 +620,32 @@ int hfmm_get_m2l_list(hfmm_ctx* h, int32_t level, int64_t* offsets, int64_t* src
     return HFMM_OK;
 }
 
+int hfmm_get_m2l_counts(hfmm_ctx* h, int32_t level, int64_t* counts) {
+    CTX_CHECK(h);
+    Context* c = &h->c;
+    if (!c->tree_built) return hfmm::fail(c, HFMM_ERR_STATE, "tree not built");
+    if (level < 2 || level >= c->height() || !counts) return hfmm::fail(c, HFMM_ERR_ARG, "bad level");
+    std::fill(counts, counts + 343, int64_t(0));
+    if (level < c->top()) return HFMM_OK;
+    HFMM_HIP(c, hipSetDevice(c->device));
+    hfmm::DevBuf b;
+    HFMM_TRY(hfmm::ensure(c, b, sizeof(unsigned long long) * 8 * hfmm::kFarSlots));
+    std::vector<unsigned long long> hc(8 * hfmm::kFarSlots);
+    int st = hfmm::m2l_slot_counts(c, level, b.as<unsigned long long>());
+    if (st == HFMM_OK && (hipMemcpyAsync(hc.data(), b.ptr, sizeof(unsigned long long) * hc.size(),
+                                         hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+                          hipStreamSynchronize(c->stream) != hipSuccess))
+        st = hfmm::fail(c, HFMM_ERR_DEVICE, "m2l counts copy failed");
+    hfmm::release(b);
+    if (st != HFMM_OK) return st;
+    for (int e = 0; e < 8; ++e) {
+        hfmm::FarSlot table[hfmm::kFarSlots];
+        hfmm::far_slots(e, table);
+        for (int j = 0; j < hfmm::kFarSlots; ++j) counts[table[j].pos] += static_cast<int64_t>(hc[e * hfmm::kFarSlots + j]);
+    }
+    return HFMM_OK;
+}
+
 int hfmm_get_operator(hfmm_ctx* h, int32_t idx, int32_t* rank, double* K) {
     CTX_CHECK(h);
     Context* c = &h->c;
@@ -648,6 +675,24 @@ int hfmm_get_expansions(hfmm_ctx* h, int32_t level, double* M, double* L) {
             std::memcpy(out + static_cast<size_t>(i) * nn, tmp.data() + static_cast<size_t>(i) * c->LD,
                         sizeof(double) * nn);
     }
+    return HFMM_OK;
+}
+
+int hfmm_set_m2l_variant(hfmm_ctx* h, int32_t variant) {
+    CTX_CHECK(h);
+    Context* c = &h->c;
+    if (variant != -1 && !hfmm::m2l_variant_available(c, variant))
+        return hfmm::fail(c, HFMM_ERR_ARG, "M2L variant not available for this order and precision");
+    c->m2l_variant = variant;
+    return HFMM_OK;
+}
+
+int hfmm_get_m2l_variant(hfmm_ctx* h, int32_t* variant, float* tune_ms) {
+    CTX_CHECK(h);
+    Context* c = &h->c;
+    if (variant) *variant = c->m2l_variant;
+    if (tune_ms)
+        for (int v = 0; v < 4; ++v) tune_ms[v] = c->m2l_tune_ms[v];
     return HFMM_OK;
 }
 

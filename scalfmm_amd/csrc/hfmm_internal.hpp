@@ -20,6 +20,7 @@ constexpr int kTileTargets = 16;  // targets per M2L tile (the MFMA N dimension)
 constexpr int kP2PBig = 64;    // leaves with more particles run P2P through LDS-staged sources
 constexpr int kP2PItem = 128;  // targets per P2P work item of a big leaf (two per lane of a wave)
 constexpr int kSortedPad = 16; // padding doubles after the sorted particle SoA
+constexpr int kWaveM2LMaxOrder = 6;  // highest order of the wave-level mixed M2L kernel
 
 // Grow-only device allocation.
 struct DevBuf {
@@ -57,6 +58,9 @@ struct Context {
     DevBuf d_slotseg;        // [8][5] int32 start of the 1..4 rank-tile segments (+ end)
     DevBuf d_posoff;         // [343] int64 offset of a position's mixed operators in d_opm
     DevBuf d_opm;            // mixed precision: position-specific operators, bf16 hi/mid/lo planes
+    DevBuf d_opm2;           // the same with U's rank columns interleaved (wave-level kernel, p <= 6)
+    int m2l_variant = -1;    // M2L kernel variant; -1: pick by timing at the next M2L
+    float m2l_tune_ms[4] = {0.f, 0.f, 0.f, 0.f};  // times measured by the last autotune
 
     // ---- particles
     int64_t n = 0;
@@ -150,11 +154,15 @@ int check_launch(Context* c, const char* name);
 
 // A context on the given stream (nullptr: a stream of its own).
 int create_context(const hfmm_params* params, hipStream_t shared_stream, hfmm_ctx** out);
+// The context behind a handle.
+Context* context_of(hfmm_ctx* h);
 
 // Tree construction (tree_device.hip)
 int build_tree_device(Context* c);
 // New charges (c->q, input order) on the built tree: refreshes the sorted charges.
 int update_charges_device(Context* c, int rhs = 0);
+// Occupied M2L slots of a level per (parity, slot) into counts[8][189] (device).
+int m2l_slot_counts(Context* c, int level, unsigned long long* counts);
 // Device counter holding the number of P2P work items of the big leaves.
 inline int32_t* p2p_item_count(Context* c) { return reinterpret_cast<int32_t*>(c->d_count.as<int64_t>() + 12); }
 
@@ -177,6 +185,8 @@ int prepare_mixed_operators(Context* c);  // position-specific bf16 operators
 int run_split_multipoles(Context* c);     // d_M -> d_Mb
 int run_m2l_mixed(Context* c);
 int run_p2p_mixed(Context* c);
+int m2l_variant_count(Context* c);
+bool m2l_variant_available(Context* c, int v);
 int selftest_mfma_bf16(Context* c, float* out);
 
 // FP64 direct sum for sampled targets (direct.hip)

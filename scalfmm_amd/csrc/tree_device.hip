@@ -424,6 +424,24 @@ __global__ __launch_bounds__(kBlock) void k_parity_keys(int ncell, const uint64_
 
 inline unsigned blocks_for(int64_t n) { return static_cast<unsigned>((n + kBlock - 1) / kBlock); }
 
+// Occupied M2L slots of one level counted per (parity, slot): counts[8][189].
+__global__ __launch_bounds__(kBlock) void k_m2l_slot_counts(int ncell, const uint64_t* __restrict__ keys,
+                                                           const int32_t* __restrict__ slots,
+                                                           unsigned long long* __restrict__ counts) {
+    __shared__ unsigned int bins[8 * kFarSlots];
+    for (int j = threadIdx.x; j < 8 * kFarSlots; j += blockDim.x) bins[j] = 0u;
+    __syncthreads();
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < ncell) {
+        const int par = static_cast<int>(keys[i] & 7u);
+        for (int j = 0; j < kFarSlots; ++j)
+            if (slots[static_cast<int64_t>(i) * kFarSlots + j] >= 0) atomicAdd(&bins[par * kFarSlots + j], 1u);
+    }
+    __syncthreads();
+    for (int j = threadIdx.x; j < 8 * kFarSlots; j += blockDim.x)
+        if (bins[j]) atomicAdd(&counts[j], static_cast<unsigned long long>(bins[j]));
+}
+
 // Run-length groups of sorted keys >> shift: writes group keys and starts,
 // returns the count on the host.
 // P2P work items of the big leaves: ceil(count / kP2PItem) items of
@@ -471,6 +489,16 @@ int make_groups(Context* c, const uint64_t* keys, int64_t n, int shift, uint64_t
 }
 
 }  // namespace
+
+int m2l_slot_counts(Context* c, int level, unsigned long long* counts) {
+    const int64_t nc = c->ncell[level];
+    HFMM_TRY(zero_device(c, counts, sizeof(unsigned long long) * 8 * kFarSlots));
+    if (nc == 0) return HFMM_OK;
+    hipLaunchKernelGGL(k_m2l_slot_counts, dim3(blocks_for(nc)), dim3(kBlock), 0, c->stream, static_cast<int>(nc),
+                       c->d_cellkey.as<uint64_t>() + c->lvl_off[level],
+                       c->d_m2l.as<int32_t>() + static_cast<size_t>(c->m2l_off[level]) * kFarSlots, counts);
+    return check_launch(c, "m2l_slot_counts");
+}
 
 int update_charges_device(Context* c, int rhs) {
     prof_begin(c, "gather_charges");

@@ -9,6 +9,10 @@ blocks by jumping ahead.
   uniform_cube  positions x,y,z = drand48() per particle in that order
                 (FRandomLoader::fillParticle, FRandomLoader.hpp:85-90), then
                 charges 2*drand48() - 1 in a second pass
+  batch_sets    set sizes floor(nmin * ratio^u) from the first nsets draws,
+                then positions of every particle (x, y, z per particle, sets
+                concatenated, each in its own unit cube), then nrhs charge
+                vectors 2*drand48() - 1 over all particles
   plummer       unit-sphere directions for all particles first, then Plummer
                 radii with rejection above R (unifRandomPlummerLike,
                 FGenerateDistribution.hpp:292-331, 395-410), mapped into the
@@ -72,6 +76,17 @@ def uniform_cube(n: int, seed: int, width: float = 1.0, center=(0.5, 0.5, 0.5)):
     pos = [xyz[:, d] * width + center[d] - width / 2 for d in range(3)]
     q = 2.0 * g.draw(n) - 1.0
     return pos[0].copy(), pos[1].copy(), pos[2].copy(), q
+
+
+def batch_sets(nsets: int, nrhs: int, seed: int, nmin: int = 1000, ratio: float = 64.0):
+    """Sets of log-uniform sizes in [nmin, nmin * ratio], each uniform in the
+    unit cube; returns (sizes, x, y, z, q) with q of shape (nrhs, n)."""
+    g = Drand48(seed)
+    sizes = np.floor(nmin * ratio ** g.draw(nsets)).astype(np.int64)
+    n = int(sizes.sum())
+    xyz = g.draw(3 * n).reshape(n, 3)
+    q = 2.0 * g.draw(nrhs * n).reshape(nrhs, n) - 1.0
+    return sizes, xyz[:, 0].copy(), xyz[:, 1].copy(), xyz[:, 2].copy(), q
 
 
 def plummer(n: int, seed: int, R: float = 10.0):

@@ -75,7 +75,29 @@ class Engine:
         self.order = int(order)
         self.nnodes = self.order ** 3
 
+    @classmethod
+    def _borrowed(cls, handle, tree_height, order, owner):
+        """A view of a context owned by `owner` (a batch): inspection and
+        profiling only; closing it does not destroy the context."""
+        e = cls.__new__(cls)
+        e._L = _abi.lib()
+        e.h = handle
+        e._owner = owner
+        e.n = 0
+        e.nrhs = e.nsets = 1
+        e.tree_stamp = 0
+        e.height = int(tree_height)
+        e.order = int(order)
+        e.nnodes = e.order ** 3
+        ti = e.tree_info()
+        e.n = int(ti.n_particles)
+        e.height = int(ti.height)
+        return e
+
     def close(self):
+        if getattr(self, "_owner", None) is not None:
+            self.h = None
+            return
         if getattr(self, "h", None):
             self._L.hfmm_destroy(self.h)
             self.h = None
@@ -194,6 +216,23 @@ class Engine:
         ps = np.empty(max(m, 1), np.int32)
         self._check(self._L.hfmm_get_m2l_list(self.h, level, _ip64(off), _ip64(sr), _ip32(ps)))
         return off, sr[:m], ps[:m]
+
+    def set_m2l_variant(self, v):
+        """Mixed M2L tile shape (0, 1, 2) or -1 to autotune at the next M2L."""
+        self._check(self._L.hfmm_set_m2l_variant(self.h, int(v)))
+
+    def m2l_variant(self):
+        """(variant in use or -1, times per variant measured by the last autotune)."""
+        v = C.c_int32()
+        t = (C.c_float * 4)()
+        self._check(self._L.hfmm_get_m2l_variant(self.h, C.byref(v), t))
+        return v.value, [float(x) for x in t]
+
+    def m2l_counts(self, level):
+        """Interactions of a level per relative position (343 counts)."""
+        out = np.zeros(343, np.int64)
+        self._check(self._L.hfmm_get_m2l_counts(self.h, int(level), _ip64(out)))
+        return out
 
     def operator(self, idx):
         r = C.c_int32()

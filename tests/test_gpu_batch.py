@@ -39,9 +39,11 @@ def _sets(sizes, nrhs, seed):
     return off, pts[:, 0].copy(), pts[:, 1].copy(), pts[:, 2].copy(), q
 
 
-def _single(h, p, precision, x, y, z, q):
+def _single(h, p, precision, x, y, z, q, variant=None):
     from scalfmm_amd import Engine
     e = Engine(h, 1.0, (0.5, 0.5, 0.5), p, 0.0, 0, precision)
+    if variant is not None and precision:
+        e.set_m2l_variant(variant)
     e.set_particles(x, y, z, q)
     e.build_tree()
     e.evaluate()
@@ -79,14 +81,16 @@ def test_forest_keys_prefix_the_set():
     assert e.tree_info().height == h + 1  # 5 sets fit one set level (8 >= 5)
 
 
-@pytest.mark.parametrize("mode", ["fp64", "mixed"])
-def test_forest_matches_single_sets(mode):
+@pytest.mark.parametrize("mode,variant", [("fp64", None), ("mixed", 0), ("mixed", 1), ("mixed", 2)])
+def test_forest_matches_single_sets(mode, variant):
     from scalfmm_amd import Engine
     h, p, nrhs = 4, 5, 3
     sizes = [4000, 0, 2500, 9000, 700]
     off, x, y, z, q = _sets(sizes, nrhs, 32)
-    _say(f"forest {mode} sizes={sizes} nrhs={nrhs} h={h} p={p}")
+    _say(f"forest {mode} (M2L variant {variant}) sizes={sizes} nrhs={nrhs} h={h} p={p}")
     e = Engine(h, 1.0, (0.5, 0.5, 0.5), p, 0.0, 0, _precision(mode))
+    if variant is not None:
+        e.set_m2l_variant(variant)
     e.set_particle_sets(off, x, y, z, q)
     e.build_tree()
     e.evaluate()
@@ -97,7 +101,7 @@ def test_forest_matches_single_sets(mode):
         if a == b:
             continue
         for r in range(nrhs):
-            ref = _single(h, p, _precision(mode), x[a:b], y[a:b], z[a:b], q[r, a:b])
+            ref = _single(h, p, _precision(mode), x[a:b], y[a:b], z[a:b], q[r, a:b], variant)
             for k in range(4):
                 assert np.array_equal(got[k][r, a:b], ref[k]), (s, r, k)
 
@@ -152,12 +156,15 @@ def test_batch_graph_replays(mode):
         again = b.results()
         for k in range(4):
             assert np.array_equal(again[k], first[k]), (rep, k)
+    # the single-set engines use the M2L variant each tree's autotuner picked
+    variant = {t.set_height: t.m2l_variant()[0] for t in b.trees()}
     for s in range(len(sizes)):
         lo, hi = off[s], off[s + 1]
         if lo == hi:
             continue
         for r in range(nrhs):
-            ref = _single(b.heights[s], p, _precision(mode), x[lo:hi], y[lo:hi], z[lo:hi], q[r, lo:hi])
+            ref = _single(b.heights[s], p, _precision(mode), x[lo:hi], y[lo:hi], z[lo:hi], q[r, lo:hi],
+                          variant[b.heights[s]])
             for k in range(4):
                 assert np.array_equal(first[k][r, lo:hi], ref[k]), (s, r, k)
 
@@ -176,5 +183,5 @@ def test_batch_rebuild_recaptures():
         got = b.results()
         for s in range(2):
             lo, hi = off[s], off[s + 1]
-            ref = _single(b.heights[s], 5, 0, x[lo:hi], y[lo:hi], z[lo:hi], q[0, lo:hi])
+            ref = _single(b.heights[s], 5, 0, x[lo:hi], y[lo:hi], z[lo:hi], q[0, lo:hi])  # FP64: one M2L kernel
             assert np.array_equal(got[0][0, lo:hi], ref[0])

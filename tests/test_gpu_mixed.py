@@ -159,3 +159,49 @@ def test_mixed_matches_fp64_engine_lists():
     for level in range(2, 5):
         for u, v in zip(a.m2l_list(level), b.m2l_list(level)):
             assert np.array_equal(u, v)
+
+
+@pytest.mark.parametrize("order,h,n,variant", [(5, 5, 20000, 0), (5, 5, 20000, 1), (5, 5, 20000, 2),
+                                               (6, 4, 8000, 0), (6, 4, 8000, 1)])
+def test_mixed_m2l_variants(oracle, order, h, n, variant):
+    """Every M2L tile shape of the mixed form against the FP64 oracle, and the
+    wave-level shapes against the four-wave one (same products, other
+    accumulation order: within the mixed tolerance)."""
+    from scalfmm_amd import FFmmFarField
+    from scalfmm_amd.distributions import uniform_cube
+    x, y, z, q = uniform_cube(n, 300 + order)
+    _say(f"mixed M2L variant {variant}: n={n} h={h} p={order}")
+    got = {}
+    for v in sorted({0, variant}):
+        e = _mixed(h, order)
+        e.set_m2l_variant(v)
+        e.set_particles(x, y, z, q)
+        e.build_tree()
+        e.evaluate(FFmmFarField)
+        assert e.m2l_variant()[0] == v
+        got[v] = e.results()
+    ref = oracle.fmm(x, y, z, q, (0.5, 0.5, 0.5), 1.0, h, order, ops=FFmmFarField)
+    _check(got[variant], ref)
+    for a, b in zip(got[variant], got[0]):
+        assert _rel(a, b) <= 1e-6
+
+
+def test_mixed_m2l_autotune():
+    """Variant -1 times the available shapes once and keeps the fastest."""
+    from scalfmm_amd.distributions import uniform_cube
+    x, y, z, q = uniform_cube(30000, 12)
+    e = _mixed(5, 5)
+    assert e.m2l_variant()[0] == -1
+    e.set_particles(x, y, z, q)
+    e.build_tree()
+    e.evaluate()
+    v, times = e.m2l_variant()
+    _say(f"  autotune picked {v}; times {times}")
+    assert v in (0, 1, 2)
+    assert all(t > 0 for t in times[:3])
+    assert times[v] == min(times[:3])
+    a = e.results()
+    e.evaluate()
+    assert all(np.array_equal(u, w) for u, w in zip(a, e.results()))
+    with pytest.raises(Exception):
+        _mixed(7, 7).set_m2l_variant(1)  # the wave-level shapes stop at order 6
