@@ -127,3 +127,68 @@ def test_gradient_sampled(mode, n, h):
     _say(f"{mode} n={n}: phi {_rel(g[t], phi_t):.2e}, dL/dq {eq:.2e}, dL/dx {ex:.2e} on 64 particles vs direct; "
          f"forward {t1 - t0:.2f} s, backward {t2 - t1:.2f} s (host round trips included)")
     assert eq < 1e-5 and ex < 1e-4
+
+
+def test_gradient_fd_full_size():
+    """The differentiable configuration at full size (2M particles, charges
+    N(0,1), height 7, order 6, mixed form): dL/dx on 64 sampled coordinates
+    against FP64 central finite differences (h = 1e-4) of the direct-sum loss
+    L = sum_i phi_i^2 / 2, and dL/dq on 16 sampled charges against the same.
+
+    Moving x_j by +-h changes phi_i (i != j) by d_i = q_j (1/|x_i - x_j'| -
+    1/|x_i - x_j|) and recomputes phi_j, so L(+) - L(-) is summed from these
+    differences directly (no cancellation of two large losses); phi is the
+    FP64 direct sum at every particle (device kernel).  L is linear-quadratic
+    in q: its central difference in q_j is sum_{i != j} phi_i / r_ij."""
+    import time
+    from scalfmm_amd import PRECISION_MIXED
+    from scalfmm_amd.autograd import fmm_potential
+    n, h, p = 2_000_000, 7, 6
+    X, q = _gauss(n, 7)
+    e = _engine(h, p, PRECISION_MIXED)
+    xt = torch.from_numpy(X.copy()).requires_grad_(True)
+    qt = torch.from_numpy(q.copy()).requires_grad_(True)
+    phi = fmm_potential(xt, qt, e)
+    (0.5 * (phi * phi).sum()).backward()
+    dq, dx = qt.grad.numpy(), xt.grad.numpy()
+    e.close()
+    ref = _engine(3, 5, 0)
+    ref.set_particles(X[:, 0].copy(), X[:, 1].copy(), X[:, 2].copy(), q)
+    t0 = time.perf_counter()
+    phi_d = ref.direct(np.arange(n))[0]  # FP64 direct potential at every particle
+    _say(f"  FP64 direct sum at {n} targets: {time.perf_counter() - t0:.1f} s")
+    rng = np.random.default_rng(11)
+    hs = 1e-4
+    fd, got = [], []
+    for _ in range(64):
+        j, d = int(rng.integers(n)), int(rng.integers(3))
+        diff = X - X[j]
+        r0 = np.sqrt(np.einsum("ij,ij->i", diff, diff))
+        r0[j] = np.inf
+        parts = []
+        for sgn in (1.0, -1.0):
+            xj = X[j].copy()
+            xj[d] += sgn * hs
+            dd = X - xj
+            r = np.sqrt(np.einsum("ij,ij->i", dd, dd))
+            r[j] = np.inf
+            delta = q[j] * (1.0 / r - 1.0 / r0)
+            phij = np.sum(q / r)
+            parts.append((delta, phij))
+        (dp, pp), (dm, pm) = parts
+        dL = np.sum(phi_d * (dp - dm) + 0.5 * (dp * dp - dm * dm)) + 0.5 * (pp * pp - pm * pm)
+        fd.append(dL / (2 * hs))
+        got.append(dx[j, d])
+    fd, got = np.array(fd), np.array(got)
+    ex = float(np.max(np.abs(got - fd)) / np.max(np.abs(fd)))
+    fq, gq = [], []
+    for _ in range(16):
+        j = int(rng.integers(n))
+        diff = X - X[j]
+        r = np.sqrt(np.einsum("ij,ij->i", diff, diff))
+        r[j] = np.inf
+        fq.append(np.sum(phi_d / r))
+        gq.append(dq[j])
+    eq = float(np.max(np.abs(np.array(gq) - np.array(fq))) / np.max(np.abs(fq)))
+    _say(f"  finite differences at n={n}: dL/dx {ex:.2e}, dL/dq {eq:.2e} (max error / max entry)")
+    assert ex < 1e-4 and eq < 1e-5

@@ -196,6 +196,28 @@ def test_fmm_uniform_orders(oracle, order, h, n):
         assert _rel(a, b) <= FP64_TOL
 
 
+@pytest.mark.parametrize("mode", ["fp64", "mixed"])
+def test_config_uniform20k_h5_p5(oracle, mode):
+    """20,000 drand48 particles (seed 1), height 5, order 5: the FP64 form
+    within 1e-12 of the oracle, both forms against the O(N^2) direct sum
+    (order-5 truncation error, 8.1e-4 for the potential)."""
+    from scalfmm_amd import PRECISION_FP64, PRECISION_MIXED, Engine
+    from scalfmm_amd.distributions import uniform_cube
+    x, y, z, q = uniform_cube(20000, 1)
+    e = Engine(5, 1.0, (0.5, 0.5, 0.5), 5, 0.0, 0, PRECISION_FP64 if mode == "fp64" else PRECISION_MIXED)
+    e.set_particles(x, y, z, q)
+    e.build_tree()
+    e.evaluate()
+    got = e.results()
+    ref = oracle.fmm(x, y, z, q, (0.5, 0.5, 0.5), 1.0, 5, 5)
+    direct = oracle.direct(x, y, z, q)
+    vs_oracle = [_rel(a, b) for a, b in zip(got, ref[:4])]
+    vs_direct = [_rel(a, b) for a, b in zip(got, direct)]
+    _say(f"  {mode}: vs oracle {max(vs_oracle):.2e}, vs direct pot {vs_direct[0]:.2e} forces {max(vs_direct[1:]):.2e}")
+    assert max(vs_oracle) <= (FP64_TOL if mode == "fp64" else 3e-6)
+    assert vs_direct[0] <= 1e-3 and max(vs_direct[1:]) <= 1.2e-3
+
+
 def test_fmm_clustered_expansions(oracle):
     x, y, z, q = _clustered(20000, 5)
     h, order = 6, 6
