@@ -30,7 +30,8 @@ struct ChebOperators {
 // (Src/Kernels/Chebyshev/FChebRoots.hpp:90-297).
 bool cheb_roots(int p, double* out);
 
-// Builds all operators; returns an HFMM_* status and fills err on failure.
+// Builds the roots, interpolators and symmetry tables (not the compressed M2L
+// operators: compress_operators_device); returns an HFMM_* status.
 int build_cheb_operators(int order, double eps, ChebOperators& ops, std::string& err);
 
 // The 189 far-field slots of a cell of parity e (= Morton child index 0..7):

@@ -251,6 +251,7 @@ int create_context(const hfmm_params* params, hipStream_t shared_stream, hfmm_ct
     }
     std::string err;
     int st = hfmm::build_cheb_operators(params->order, params->epsilon, c->ops, err);
+    if (st == HFMM_OK) st = hfmm::compress_operators_device(c);
     if (st == HFMM_OK) st = hfmm::upload_operators(c);
     if (st == HFMM_OK && c->P.precision == HFMM_PRECISION_MIXED) st = hfmm::prepare_mixed_operators(c);
     if (st != HFMM_OK) {

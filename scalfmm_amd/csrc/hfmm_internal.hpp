@@ -152,6 +152,8 @@ void prof_begin(Context* c, const char* name);
 void prof_end(Context* c);
 int check_launch(Context* c, const char* name);
 
+// The 16 canonical compressed M2L operators (FP64, on the device) into c->ops.
+int compress_operators_device(Context* c);
 // A context on the given stream (nullptr: a stream of its own).
 int create_context(const hfmm_params* params, hipStream_t shared_stream, hfmm_ctx** out);
 // The context behind a handle.
