@@ -130,6 +130,15 @@ int hfmm_build_tree(hfmm_ctx* ctx);
  * with FChebSymKernel's P2M/M2M/M2L/L2L/L2P/P2P (FChebSymKernel.hpp:181-486).
  * Potentials and forces are reset, then accumulated by the selected passes. */
 int hfmm_evaluate(hfmm_ctx* ctx, uint32_t ops);
+/* Same over the working levels [upper, lower), 2 <= upper <= lower <= height:
+ * FAbstractAlgorithm::execute(operationsToProceed, upperWorkingLevel,
+ * lowerWorkingLevel) (Src/Core/FCoreCommon.hpp:81-86) with the loop bounds of
+ * FGroupSeqAlgorithm (FGroupSeqAlgorithm.hpp:88, 137, 238): M2M parents from
+ * min(height-2, lower-1) down to upper, M2L at levels lower-1 .. upper, L2L
+ * parents upper .. lower-2; P2M, L2P and P2P on the leaves.  hfmm_evaluate is
+ * hfmm_evaluate_levels(ctx, ops, 2, height).  With particle sets the levels
+ * are those of each set's tree. */
+int hfmm_evaluate_levels(hfmm_ctx* ctx, uint32_t ops, int32_t upper, int32_t lower);
 
 /* Waits for the context's stream. */
 int hfmm_synchronize(hfmm_ctx* ctx);

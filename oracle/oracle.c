@@ -1068,7 +1068,13 @@ static void p2p_oneway(const struct orc_run* r, int64_t t0, int64_t tn, int64_t 
     }
 }
 
-static void run_passes(struct orc_run* r, unsigned ops, const double center[3], double width) {
+/* Working levels [upper, lower) as FAbstractAlgorithm::execute(ops, upper, lower)
+ * sets them (Core/FCoreCommon.hpp:81-86), with the loop bounds of
+ * FGroupSeqAlgorithm (GroupTree/Core/FGroupSeqAlgorithm.hpp:88, 137, 238): M2M
+ * parents from min(h-2, lower-1) down to upper, M2L levels lower-1 down to
+ * upper, L2L parents upper .. lower-2; P2M, L2P and P2P work on the leaves. */
+static void run_passes(struct orc_run* r, unsigned ops, const double center[3], double width, int upper,
+                       int lower) {
     const orc_cheb* c = r->c;
     const orc_tree* t = r->t;
     const int h = t->height, L = h - 1, nn = c->nn;
@@ -1092,7 +1098,7 @@ static void run_passes(struct orc_run* r, unsigned ops, const double center[3], 
     /* upward pass: M2M levels h-2..2 */
     t0 = wall_seconds();
     if (ops & ORC_M2M) {
-        for (int l = L - 1; l >= 2; --l) {
+        for (int l = (L - 1 < lower - 1 ? L - 1 : lower - 1); l >= upper; --l) {
             const int64_t nc = t->ncell[l], nch = t->ncell[l + 1];
             const int64_t* ck = t->keys[l + 1];
 #pragma omp parallel for schedule(dynamic, 16)
@@ -1109,7 +1115,7 @@ static void run_passes(struct orc_run* r, unsigned ops, const double center[3], 
     /* transfer pass: M2L levels 2..h-1 */
     t0 = wall_seconds();
     if (ops & ORC_M2L) {
-        for (int l = 2; l <= L; ++l) {
+        for (int l = lower - 1; l >= upper; --l) {
             /* getScaleFactor(RootCellWidth, level) = 2/(W/2^l) (FInterpMatrixKernel.hpp:124-133) */
             const double scale = 2. / (width / pow(2., (double)l));
             const int64_t nc = t->ncell[l];
@@ -1125,7 +1131,7 @@ static void run_passes(struct orc_run* r, unsigned ops, const double center[3], 
     /* downward pass: L2L levels 2..h-2 */
     t0 = wall_seconds();
     if (ops & ORC_L2L) {
-        for (int l = 2; l <= L - 1; ++l) {
+        for (int l = upper; l < lower - 1; ++l) {
             const int64_t nc = t->ncell[l], nch = t->ncell[l + 1];
             const int64_t* ck = t->keys[l + 1];
 #pragma omp parallel for schedule(dynamic, 16)
@@ -1213,7 +1219,7 @@ int orc_run_create(const orc_cheb* c, int64_t n, const double* x, const double* 
     if (!c || !out) return ORC_ERR_ARG;
     int st = run_setup(c, n, x, y, z, q, center, width, height, out);
     if (st) return st;
-    run_passes(*out, ops, center, width);
+    run_passes(*out, ops, center, width, 2, height);
     return ORC_OK;
 }
 
@@ -1230,7 +1236,15 @@ int orc_fmm_with(const orc_cheb* c, int64_t n, const double* x, const double* y,
                  const double* q, const double center[3], double width, int height,
                  unsigned ops, int nthreads,
                  double* pot, double* fx, double* fy, double* fz, orc_stats* stats) {
+    return orc_fmm_levels(c, n, x, y, z, q, center, width, height, ops, 2, height, nthreads, pot, fx, fy, fz,
+                          stats);
+}
+
+int orc_fmm_levels(const orc_cheb* c, int64_t n, const double* x, const double* y, const double* z,
+                   const double* q, const double center[3], double width, int height, unsigned ops, int upper,
+                   int lower, int nthreads, double* pot, double* fx, double* fy, double* fz, orc_stats* stats) {
     if (!c || n < 0) return ORC_ERR_ARG;
+    if (upper < 2 || upper > lower || lower > height) return ORC_ERR_ARG;  /* validateLevels, FCoreCommon.hpp:49-52 */
 #ifdef _OPENMP
     const int saved = omp_get_max_threads();
     if (nthreads > 0) omp_set_num_threads(nthreads);
@@ -1241,7 +1255,7 @@ int orc_fmm_with(const orc_cheb* c, int64_t n, const double* x, const double* y,
     struct orc_run* r = NULL;
     int st = run_setup(c, n, x, y, z, q, center, width, height, &r);
     if (st == ORC_OK) {
-        run_passes(r, ops, center, width);
+        run_passes(r, ops, center, width, upper, lower);
         for (int64_t i = 0; i < n; ++i) {
             const int64_t o = r->t->perm[i];
             pot[o] = r->pot[i]; fx[o] = r->fx[i]; fy[o] = r->fy[i]; fz[o] = r->fz[i];

@@ -137,6 +137,12 @@ int orc_fmm_with(const orc_cheb* c, int64_t n, const double* x, const double* y,
                  unsigned ops, int nthreads,
                  double* pot, double* fx, double* fy, double* fz, orc_stats* stats);
 
+/* Same, with the working levels [upper, lower) of FAbstractAlgorithm::execute
+ * (ops, upper, lower) (Core/FCoreCommon.hpp:81-86): 2 <= upper <= lower <= height. */
+int orc_fmm_levels(const orc_cheb* c, int64_t n, const double* x, const double* y, const double* z,
+                   const double* q, const double center[3], double width, int height, unsigned ops, int upper,
+                   int lower, int nthreads, double* pot, double* fx, double* fy, double* fz, orc_stats* stats);
+
 /* Expansions of the last level-by-level run for inspection: the tree and
  * multipole/local arrays ([ncell][nnodes] per level) of an FMM run. */
 typedef struct orc_run orc_run;

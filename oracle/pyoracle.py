@@ -82,6 +82,8 @@ def lib():
                           C.c_double, C.c_uint, C.c_int, _f64p, _f64p, _f64p, _f64p, C.POINTER(Stats)]
     L.orc_fmm_with.argtypes = [vp, C.c_int64, _f64p, _f64p, _f64p, _f64p, _f64p, C.c_double, C.c_int,
                                C.c_uint, C.c_int, _f64p, _f64p, _f64p, _f64p, C.POINTER(Stats)]
+    L.orc_fmm_levels.argtypes = [vp, C.c_int64, _f64p, _f64p, _f64p, _f64p, _f64p, C.c_double, C.c_int,
+                                 C.c_uint, C.c_int, C.c_int, C.c_int, _f64p, _f64p, _f64p, _f64p, C.POINTER(Stats)]
     L.orc_run_create.argtypes = [vp, C.c_int64, _f64p, _f64p, _f64p, _f64p, _f64p, C.c_double, C.c_int,
                                  C.c_uint, C.POINTER(vp)]
     L.orc_run_destroy.argtypes = [vp]
@@ -280,11 +282,20 @@ class Tree:
 
 
 # ---- evaluation ------------------------------------------------------------------
-def fmm(x, y, z, q, center, width, height, order, epsilon=0.0, ops=ALL, nthreads=0, cheb=None):
+def fmm(x, y, z, q, center, width, height, order, epsilon=0.0, ops=ALL, nthreads=0, cheb=None, levels=None):
+    """FMM over the tree of `height`; `levels` = (upper, lower) restricts the
+    working levels as FAbstractAlgorithm::execute(ops, upper, lower) does."""
     n = len(x)
     pot, fx, fy, fz = (np.zeros(max(n, 1)) for _ in range(4))
     st = Stats()
-    if cheb is not None:
+    if levels is not None:
+        own = cheb is None
+        cheb = cheb or Cheb(order, epsilon)
+        rc = lib().orc_fmm_levels(cheb.h, n, _f(x), _f(y), _f(z), _f(q), _center(center), float(width), height,
+                                  ops, int(levels[0]), int(levels[1]), nthreads, pot, fx, fy, fz, C.byref(st))
+        if own:
+            del cheb
+    elif cheb is not None:
         rc = lib().orc_fmm_with(cheb.h, n, _f(x), _f(y), _f(z), _f(q), _center(center), float(width), height,
                                 ops, nthreads, pot, fx, fy, fz, C.byref(st))
     else:

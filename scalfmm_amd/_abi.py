@@ -35,7 +35,7 @@ EXPORTS = (
     "hfmm_batch_default_height", "hfmm_batch_create", "hfmm_batch_destroy", "hfmm_batch_last_error",
     "hfmm_batch_set_particles", "hfmm_batch_build", "hfmm_batch_evaluate", "hfmm_batch_synchronize",
     "hfmm_batch_get_results", "hfmm_batch_info", "hfmm_batch_tree", "hfmm_get_m2l_counts",
-    "hfmm_set_m2l_variant", "hfmm_get_m2l_variant",
+    "hfmm_set_m2l_variant", "hfmm_get_m2l_variant", "hfmm_evaluate_levels",
 )
 MAX_RHS = 64
 _NON_STATUS = ("hfmm_last_error", "hfmm_version", "hfmm_batch_last_error", "hfmm_batch_default_height")
@@ -121,6 +121,7 @@ def lib(path: str | None = None):
     L.hfmm_batch_tree.argtypes = [vp, i32, C.POINTER(vp), C.POINTER(i32), C.POINTER(i32)]
     L.hfmm_get_m2l_counts.argtypes = [vp, i32, C.POINTER(i64)]
     L.hfmm_set_m2l_variant.argtypes = [vp, i32]
+    L.hfmm_evaluate_levels.argtypes = [vp, C.c_uint32, i32, i32]
     L.hfmm_get_m2l_variant.argtypes = [vp, C.POINTER(i32), C.POINTER(C.c_float)]
     for name in EXPORTS:
         if name not in _NON_STATUS:

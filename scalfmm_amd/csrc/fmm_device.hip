@@ -738,7 +738,10 @@ M2LLevels make_levels(const Context* c, int tile_targets, int& tiles) {
     lv.nlev = 0;
     tiles = 0;
     lv.tile_begin[0] = 0;
-    for (int l = c->top(); l <= L; ++l) {
+    // the M2L levels of the working range (every level top..h-1 by default)
+    const int lo = c->work_upper > c->top() ? c->work_upper : c->top();
+    const int hi = c->work_lower > 0 && c->work_lower - 1 < L ? c->work_lower - 1 : L;
+    for (int l = lo; l <= hi; ++l) {
         const int i = lv.nlev++;
         lv.cell_off[i] = c->lvl_off[l];
         lv.pcount_off[i] = 8 * l;
@@ -810,6 +813,7 @@ template <int P>
 int m2l_impl(Context* c) {
     int tiles = 0;
     const M2LLevels lv = make_levels(c, kM2LTile, tiles);
+    if (tiles == 0) return HFMM_OK;  // no M2L level in the working range
     const unsigned grid = static_cast<unsigned>(tiles);
     const size_t shmem =
         sizeof(double) * kM2LWaves * static_cast<size_t>(c->kpad_max) * kM2LTile + sizeof(uint16_t) * 2 * c->LD;

@@ -941,6 +941,7 @@ template <int P, int COLS>
 int m2l_mixed_launch(Context* c) {
     int tiles = 0;
     const M2LLevels lv = make_levels(c, 16 * COLS, tiles);
+    if (tiles == 0) return HFMM_OK;  // no M2L level in the working range
     const int CR = ((c->kpad_max + 31) & ~31) + 4;
     const size_t shmem = sizeof(float) * kM2LWaves * static_cast<size_t>(CR) * 16 * COLS + 16 * COLS * 2 * 3 * 64;
     static int lds_limit[64] = {0};
@@ -959,6 +960,7 @@ int m2l_mixed_wave_launch(Context* c) {
     if (c->LD != WaveMixState<P, CT>::LDC) return fail(c, HFMM_ERR_ARG, "wave-level M2L: padded node count");
     int tiles = 0;
     const M2LLevels lv = make_levels(c, 16 * CT * kM2LWaves, tiles);
+    if (tiles == 0) return HFMM_OK;  // no M2L level in the working range
     const int64_t plane = (c->total_cells + 1) * static_cast<int64_t>(c->LD);
     hipLaunchKernelGGL((k_m2l_mixed_wave<P, CT>), dim3(static_cast<unsigned>(tiles)), dim3(64 * kM2LWaves), 0,
                        c->stream, lv, c->d_m2l.as<int32_t>(), c->d_order.as<int32_t>(), c->d_pcount.as<int32_t>(),

@@ -264,17 +264,23 @@ int create_context(const hfmm_params* params, hipStream_t shared_stream, hfmm_ct
     return HFMM_OK;
 }
 
-// The passes of one charge vector: results into d_res_sorted.
+// The passes of one charge vector over the working levels [work_upper,
+// work_lower) with the loop bounds of FGroupSeqAlgorithm
+// (GroupTree/Core/FGroupSeqAlgorithm.hpp:88, 137, 238): M2M parents from
+// min(h-2, lower-1) down to upper, M2L levels lower-1 .. upper, L2L parents
+// upper .. lower-2.  Results into d_res_sorted.
 static int evaluate_passes(Context* c, uint32_t ops, size_t exp_bytes) {
     const int L = c->height() - 1;
-    const int top = c->top();
+    const int up = c->work_upper, lo = c->work_lower;
     // upward pass
     if (ops & HFMM_P2M) HFMM_TRY(hfmm::run_p2m(c));
     else HFMM_TRY(hfmm::zero_device(c, c->d_M.ptr, exp_bytes));
     if (ops & HFMM_M2M)
-        for (int l = L - 1; l >= top; --l) HFMM_TRY(hfmm::run_m2m(c, l));
-    // transfer pass: M2L writes every local of levels top..h-1
+        for (int l = std::min(L - 1, lo - 1); l >= up; --l) HFMM_TRY(hfmm::run_m2m(c, l));
+    // transfer pass: M2L writes every local of its levels; the others start at zero
     const bool mixed = c->P.precision == HFMM_PRECISION_MIXED;
+    const bool all_levels = up == c->top() && lo == c->height();
+    if ((ops & HFMM_M2L) && !all_levels) HFMM_TRY(hfmm::zero_device(c, c->d_L.ptr, exp_bytes));
     if (ops & HFMM_M2L) {
         if (mixed) {
             HFMM_TRY(hfmm::run_split_multipoles(c));
@@ -286,7 +292,7 @@ static int evaluate_passes(Context* c, uint32_t ops, size_t exp_bytes) {
         HFMM_TRY(hfmm::zero_device(c, c->d_L.ptr, exp_bytes));
     // downward pass
     if (ops & HFMM_L2L)
-        for (int l = top; l <= L - 1; ++l) HFMM_TRY(hfmm::run_l2l(c, l));
+        for (int l = up; l < lo - 1; ++l) HFMM_TRY(hfmm::run_l2l(c, l));
     // near field initialises the results, the merge adds the far field
     if (ops & HFMM_P2P) HFMM_TRY(mixed ? hfmm::run_p2p_mixed(c) : hfmm::run_p2p(c));
     else HFMM_TRY(hfmm::zero_results(c));
@@ -421,9 +427,19 @@ int hfmm_build_tree(hfmm_ctx* h) {
 
 int hfmm_evaluate(hfmm_ctx* h, uint32_t ops) {
     CTX_CHECK(h);
+    return hfmm_evaluate_levels(h, ops, 2, h->c.P.tree_height);
+}
+
+int hfmm_evaluate_levels(hfmm_ctx* h, uint32_t ops, int32_t upper, int32_t lower) {
+    CTX_CHECK(h);
     Context* c = &h->c;
     if (!c->tree_built) return hfmm::fail(c, HFMM_ERR_STATE, "evaluate before build_tree");
     if (ops & ~HFMM_NEAR_AND_FAR) return hfmm::fail(c, HFMM_ERR_ARG, "unknown operation bits");
+    // validateLevels (FCoreCommon.hpp:49-52), levels of one set's tree
+    if (upper < 2 || upper > lower || lower > c->P.tree_height)
+        return hfmm::fail(c, HFMM_ERR_ARG, "working levels must satisfy 2 <= upper <= lower <= height");
+    c->work_upper = upper + c->set_levels;
+    c->work_lower = lower + c->set_levels;
     HFMM_HIP(c, hipSetDevice(c->device));
     const size_t exp_bytes = sizeof(double) * static_cast<size_t>(c->total_cells) * c->LD;
     // one extra zero multipole after the last cell: the M2L kernel reads it for missing sources

@@ -79,6 +79,9 @@ struct Context {
     int set_levels = 0;
     DevBuf d_setoff;  // int64 [nsets + 1] (unused when nsets == 1)
     int32_t nrhs = 1;  // charge vectors per particle (RHS-major, n each)
+    // working levels [work_upper, work_lower) of the current evaluation, in
+    // levels of the virtual tree (FAbstractAlgorithm upper/lowerWorkingLevel)
+    int work_upper = 0, work_lower = 0;
 
     // ---- tree
     bool tree_built = false;

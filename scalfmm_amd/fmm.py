@@ -152,8 +152,13 @@ class Engine:
         self.tree_stamp += 1
         self._check(self._L.hfmm_build_tree(self.h))
 
-    def evaluate(self, ops=NEAR_AND_FAR):
-        self._check(self._L.hfmm_evaluate(self.h, int(ops)))
+    def evaluate(self, ops=NEAR_AND_FAR, levels=None):
+        """Runs the selected passes; `levels` = (upper, lower) restricts the
+        working levels as FAbstractAlgorithm::execute(ops, upper, lower)."""
+        if levels is None:
+            self._check(self._L.hfmm_evaluate(self.h, int(ops)))
+        else:
+            self._check(self._L.hfmm_evaluate_levels(self.h, int(ops), int(levels[0]), int(levels[1])))
 
     def synchronize(self):
         self._check(self._L.hfmm_synchronize(self.h))
@@ -331,11 +336,17 @@ class FGroupTaskAlgorithm:
         self.tree = tree
         self.kernel = kernel
 
-    def execute(self, operations=FFmmNearAndFarFields):
+    def execute(self, operations=FFmmNearAndFarFields, upper_level=None, lower_level=None):
+        """execute(ops) or execute(ops, upperWorkingLevel, lowerWorkingLevel)
+        (Src/Core/FCoreCommon.hpp:65-94)."""
         e = self.kernel.engine
         t = self.tree
         e.set_particles(t.x, t.y, t.z, t.q)
         e.build_tree()
-        e.evaluate(operations)
+        levels = None
+        if upper_level is not None or lower_level is not None:
+            levels = (2 if upper_level is None else upper_level,
+                      t.tree_height if lower_level is None else lower_level)
+        e.evaluate(operations, levels)
         t.potentials, t.forces_x, t.forces_y, t.forces_z = e.results()
         return t

@@ -185,3 +185,30 @@ def test_batch_rebuild_recaptures():
             lo, hi = off[s], off[s + 1]
             ref = _single(b.heights[s], 5, 0, x[lo:hi], y[lo:hi], z[lo:hi], q[0, lo:hi])  # FP64: one M2L kernel
             assert np.array_equal(got[0][0, lo:hi], ref[0])
+
+
+@pytest.mark.parametrize("mode", ["fp64", "mixed"])
+def test_forest_working_levels(mode):
+    """Working levels apply to every set's own tree: a forest evaluated over
+    levels (3, 5) equals the single-set runs over the same levels, bit for bit."""
+    from scalfmm_amd import Engine
+    h, p = 5, 5
+    sizes = [9000, 3000]
+    off, x, y, z, q = _sets(sizes, 1, 35)
+    e = Engine(h, 1.0, (0.5, 0.5, 0.5), p, 0.0, 0, _precision(mode))
+    if mode == "mixed":
+        e.set_m2l_variant(0)
+    e.set_particle_sets(off, x, y, z, q)
+    e.build_tree()
+    e.evaluate(levels=(3, 5))
+    got = e.results()
+    for s in range(2):
+        a, b = off[s], off[s + 1]
+        f = Engine(h, 1.0, (0.5, 0.5, 0.5), p, 0.0, 0, _precision(mode))
+        if mode == "mixed":
+            f.set_m2l_variant(0)
+        f.set_particles(x[a:b], y[a:b], z[a:b], q[0, a:b])
+        f.build_tree()
+        f.evaluate(levels=(3, 5))
+        for k, ref in enumerate(f.results()):
+            assert np.array_equal(got[k][a:b], ref), (s, k)

@@ -218,6 +218,28 @@ def test_config_uniform20k_h5_p5(oracle, mode):
     assert vs_direct[0] <= 1e-3 and max(vs_direct[1:]) <= 1.2e-3
 
 
+@pytest.mark.parametrize("levels", [(2, 5), (3, 5), (2, 4), (3, 4), (4, 5), (5, 5)])
+def test_fmm_working_levels(oracle, levels):
+    """hfmm_evaluate_levels against the oracle's FAbstractAlgorithm::execute(ops,
+    upper, lower) semantics (FP64, 1e-12)."""
+    from scalfmm_amd.distributions import uniform_cube
+    x, y, z, q = uniform_cube(20000, 77)
+    h, p = 5, 6
+    _say(f"working levels {levels}: n=20000 h={h} p={p}")
+    e = _engine(h, p)
+    e.set_particles(x, y, z, q)
+    e.build_tree()
+    e.evaluate(levels=levels)
+    got = e.results()
+    ref = oracle.fmm(x, y, z, q, (0.5, 0.5, 0.5), 1.0, h, p, levels=levels)
+    for a, b in zip(got, ref[:4]):
+        assert _rel(a, b) <= FP64_TOL
+    with pytest.raises(Exception):
+        e.evaluate(levels=(1, h))
+    with pytest.raises(Exception):
+        e.evaluate(levels=(4, 3))
+
+
 def test_fmm_clustered_expansions(oracle):
     x, y, z, q = _clustered(20000, 5)
     h, order = 6, 6

@@ -115,3 +115,35 @@ def test_config_uniform20k_h5_p5_vs_direct(oracle):
     errs = [oracle.accuracy(r, g)["rel_l2"] for r, g in zip(ref, got[:4])]
     assert 6e-4 < errs[0] < 1.0e-3, errs
     assert all(4e-4 < e < 1.2e-3 for e in errs[1:]), errs
+
+
+def test_fmm_working_levels(oracle):
+    """Working levels [upper, lower) (FAbstractAlgorithm::execute(ops, upper,
+    lower)): (2, h) is the full FMM bit for bit; (h, h) leaves only the near
+    field; (3, h) drops exactly the level-2 interaction lists, so the
+    difference to the full FMM is the direct sum over the pairs whose level-2
+    cells are well separated with adjacent parents (order-6 accuracy)."""
+    from scalfmm_amd.distributions import uniform_cube
+    x, y, z, q = uniform_cube(3000, 21)
+    h, p, c = 4, 6, (0.5, 0.5, 0.5)
+    full = oracle.fmm(x, y, z, q, c, 1.0, h, p)
+    same = oracle.fmm(x, y, z, q, c, 1.0, h, p, levels=(2, h))
+    assert all(np.array_equal(a, b) for a, b in zip(full[:4], same[:4]))
+    near = oracle.fmm(x, y, z, q, c, 1.0, h, p, ops=oracle.P2P)
+    top = oracle.fmm(x, y, z, q, c, 1.0, h, p, levels=(h, h))
+    assert all(np.array_equal(a, b) for a, b in zip(near[:4], top[:4]))
+    no2 = oracle.fmm(x, y, z, q, c, 1.0, h, p, levels=(3, h))
+    X = np.stack([x, y, z], 1)
+    c2 = np.minimum((X * 4).astype(int), 3)
+    dp = np.abs((c2 // 2)[:, None, :] - (c2 // 2)[None, :, :]).max(-1)
+    dc = np.abs(c2[:, None, :] - c2[None, :, :]).max(-1)
+    D = X[:, None, :] - X[None, :, :]
+    r = np.sqrt((D ** 2).sum(-1))
+    np.fill_diagonal(r, np.inf)
+    dropped = ((q[None, :] / r) * ((dp <= 1) & (dc > 1))).sum(1)
+    err = np.linalg.norm(full[0] - no2[0] - dropped) / np.linalg.norm(dropped)
+    assert err < 5e-5, err  # measured 1.3e-5
+    with pytest.raises(RuntimeError):
+        oracle.fmm(x, y, z, q, c, 1.0, h, p, levels=(1, h))
+    with pytest.raises(RuntimeError):
+        oracle.fmm(x, y, z, q, c, 1.0, h, p, levels=(3, 2))
