@@ -38,6 +38,8 @@ extern "C" {
 #define HFMM_ERR_NOMEM 4       /* host or device allocation failed */
 #define HFMM_ERR_DEVICE 5      /* a HIP runtime call failed */
 #define HFMM_ERR_STATE 6       /* call out of order (e.g. evaluate before build) */
+#define HFMM_ERR_IO 7          /* a file could not be opened, read or written, or has a bad format
+                                  (the reference's loaders exit the process) */
 
 /* ---- operation bits: the values of FFmmOperations (Src/Core/FCoreCommon.hpp:17-29) */
 #define HFMM_P2P (1u << 0)
@@ -229,6 +231,28 @@ int hfmm_selftest_mfma_f64(hfmm_ctx* ctx, double* out);
  * D = A B (16 x 16, row-major) through the operand layout the mixed-precision
  * M2L kernel assumes; no counterpart in the reference. */
 int hfmm_selftest_mfma_bf16(hfmm_ctx* ctx, float* out);
+
+/* ---- files -------------------------------------------------------------------- */
+/* Particle files of FFmaGenericLoader / FFmaGenericWriter
+ * (Src/Files/FFmaGenericLoader.hpp:34-850): ".bfma" in the name selects the
+ * binary format, else ".fma" the text format.  The header holds the element
+ * size (8), the values per particle (>= 4: x, y, z, q, then e.g. potential and
+ * forces), the particle count, HALF the box width and the box centre.
+ * hfmm_fma_read with data = NULL reads the header only; data then takes
+ * n * nvals values, particle-major.  box_width is the full width. */
+int hfmm_fma_read(const char* path, int64_t* n, int32_t* nvals, double* box_width, double* center, double* data);
+int hfmm_fma_write(const char* path, int64_t n, int32_t nvals, double box_width, const double* center,
+                   const double* data);
+/* The compressed M2L operators in the reference's cache file format
+ * (ComputeAndCompressAndStoreInBinaryFile / ReadFromBinaryFile,
+ * FChebSymM2LHandler.hpp:471-567): per stored operator, int index, int rank,
+ * U then V (rank * p^3 doubles each, column-major).  Loading replaces the
+ * context's operators (the 16 canonical ones must be present; ranks <= 64) and
+ * re-packs the device tables.  hfmm_operator_file_name writes the reference's
+ * file name, sym2l_d_o<ORDER>_e<epsilon>.bin (epsilon <= 0: 10^-order). */
+int hfmm_save_operators(hfmm_ctx* ctx, const char* path);
+int hfmm_load_operators(hfmm_ctx* ctx, const char* path);
+int hfmm_operator_file_name(int32_t order, double epsilon, char* buf, int32_t size);
 
 /* ---- M2L kernel variants (autotuned) ---------------------------------------- */
 /* The mixed-precision M2L has several tile shapes: 0 = four waves share 64

@@ -11,7 +11,7 @@ import os
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "_lib", "libscalfmm_hip.so")
 
-OK, ERR_ARG, ERR_OUTSIDE_BOX, ERR_NUMERIC, ERR_NOMEM, ERR_DEVICE, ERR_STATE = range(7)
+OK, ERR_ARG, ERR_OUTSIDE_BOX, ERR_NUMERIC, ERR_NOMEM, ERR_DEVICE, ERR_STATE, ERR_IO = range(8)
 
 # FFmmOperations (Src/Core/FCoreCommon.hpp:17-29)
 P2P, P2M, M2M, M2L, L2L, L2P = 1, 2, 4, 8, 16, 32
@@ -36,6 +36,7 @@ EXPORTS = (
     "hfmm_batch_set_particles", "hfmm_batch_build", "hfmm_batch_evaluate", "hfmm_batch_synchronize",
     "hfmm_batch_get_results", "hfmm_batch_info", "hfmm_batch_tree", "hfmm_get_m2l_counts",
     "hfmm_set_m2l_variant", "hfmm_get_m2l_variant", "hfmm_evaluate_levels",
+    "hfmm_fma_read", "hfmm_fma_write", "hfmm_save_operators", "hfmm_load_operators", "hfmm_operator_file_name",
 )
 MAX_RHS = 64
 _NON_STATUS = ("hfmm_last_error", "hfmm_version", "hfmm_batch_last_error", "hfmm_batch_default_height")
@@ -122,6 +123,11 @@ def lib(path: str | None = None):
     L.hfmm_get_m2l_counts.argtypes = [vp, i32, C.POINTER(i64)]
     L.hfmm_set_m2l_variant.argtypes = [vp, i32]
     L.hfmm_evaluate_levels.argtypes = [vp, C.c_uint32, i32, i32]
+    L.hfmm_fma_read.argtypes = [C.c_char_p, C.POINTER(i64), C.POINTER(i32), dp, dp, dp]
+    L.hfmm_fma_write.argtypes = [C.c_char_p, i64, i32, d, dp, dp]
+    L.hfmm_save_operators.argtypes = [vp, C.c_char_p]
+    L.hfmm_load_operators.argtypes = [vp, C.c_char_p]
+    L.hfmm_operator_file_name.argtypes = [i32, d, C.c_char_p, i32]
     L.hfmm_get_m2l_variant.argtypes = [vp, C.POINTER(i32), C.POINTER(C.c_float)]
     for name in EXPORTS:
         if name not in _NON_STATUS:

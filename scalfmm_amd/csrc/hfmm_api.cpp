@@ -663,6 +663,32 @@ int hfmm_get_m2l_counts(hfmm_ctx* h, int32_t level, int64_t* counts) {
     return HFMM_OK;
 }
 
+int hfmm_save_operators(hfmm_ctx* h, const char* path) {
+    CTX_CHECK(h);
+    if (!path) return hfmm::fail(&h->c, HFMM_ERR_ARG, "save_operators: no path");
+    return hfmm::save_operators(&h->c, path);
+}
+
+int hfmm_load_operators(hfmm_ctx* h, const char* path) {
+    CTX_CHECK(h);
+    Context* c = &h->c;
+    if (!path) return hfmm::fail(c, HFMM_ERR_ARG, "load_operators: no path");
+    std::array<int, 343> rank;
+    std::array<std::vector<double>, 343> K;
+    HFMM_TRY(hfmm::read_operators(c, path, rank, K));
+    for (int idx = 0; idx < 343; ++idx)
+        if (rank[idx] > 64) return hfmm::fail(c, HFMM_ERR_ARG, "operator rank above 64 is not supported");
+    HFMM_HIP(c, hipSetDevice(c->device));
+    HFMM_HIP(c, hipStreamSynchronize(c->stream));
+    c->ops.rank = rank;
+    c->ops.K = std::move(K);
+    HFMM_TRY(hfmm::upload_operators(c));
+    if (c->P.precision == HFMM_PRECISION_MIXED) HFMM_TRY(hfmm::prepare_mixed_operators(c));
+    c->m2l_variant = -1;  // other operators: tune again
+    c->evaluated = false;
+    return HFMM_OK;
+}
+
 int hfmm_get_operator(hfmm_ctx* h, int32_t idx, int32_t* rank, double* K) {
     CTX_CHECK(h);
     Context* c = &h->c;

@@ -4,6 +4,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <array>
 #include <cstdint>
 #include <string>
 #include <vector>
@@ -159,6 +160,9 @@ int check_launch(Context* c, const char* name);
 int compress_operators_device(Context* c);
 // A context on the given stream (nullptr: a stream of its own).
 int create_context(const hfmm_params* params, hipStream_t shared_stream, hfmm_ctx** out);
+// Operator cache file of the reference (fma_io.cpp).
+int save_operators(Context* c, const char* path);
+int read_operators(Context* c, const char* path, std::array<int, 343>& rank, std::array<std::vector<double>, 343>& K);
 // The context behind a handle.
 Context* context_of(hfmm_ctx* h);
 

@@ -239,6 +239,14 @@ class Engine:
         self._check(self._L.hfmm_get_m2l_counts(self.h, int(level), _ip64(out)))
         return out
 
+    def save_operators(self, path):
+        """The compressed M2L operators in the reference's cache file format."""
+        self._check(self._L.hfmm_save_operators(self.h, str(path).encode()))
+
+    def load_operators(self, path):
+        """Replaces the compressed M2L operators by those of a cache file."""
+        self._check(self._L.hfmm_load_operators(self.h, str(path).encode()))
+
     def operator(self, idx):
         r = C.c_int32()
         self._check(self._L.hfmm_get_operator(self.h, idx, C.byref(r), None))

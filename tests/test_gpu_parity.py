@@ -240,6 +240,74 @@ def test_fmm_working_levels(oracle, levels):
         e.evaluate(levels=(4, 3))
 
 
+def _canonical():
+    return [(i + 3) * 49 + (j + 3) * 7 + (k + 3) for i in (2, 3) for j in range(i + 1) for k in range(j + 1)]
+
+
+def _write_operator_file(path, ops):
+    """ops: {index: (U, V)} written as the reference's cache file
+    (FChebSymM2LHandler.hpp:471-507): int index, int rank, U, V column-major."""
+    with open(path, "wb") as f:
+        for idx in sorted(ops):
+            U, V = ops[idx]
+            f.write(np.array([idx, U.shape[1]], np.int32).tobytes())
+            f.write(np.ascontiguousarray(U.T).tobytes())
+            f.write(np.ascontiguousarray(V.T).tobytes())
+
+
+def test_operator_file_round_trip(oracle, tmp_path):
+    """hfmm_save_operators writes the reference's layout; operators written by
+    the oracle in that layout and loaded into a context reproduce the oracle's
+    FMM; a saved file loaded into a fresh context gives the same bits."""
+    from scalfmm_amd.distributions import uniform_cube
+    p, h = 6, 4
+    e = _engine(h, p)
+    path = tmp_path / "sym2l.bin"
+    e.save_operators(path)
+    raw = path.read_bytes()
+    off, seen = 0, []
+    while off < len(raw):
+        idx, r = np.frombuffer(raw[off:off + 8], np.int32)
+        off += 8
+        K = np.frombuffer(raw[off:off + 16 * r * p ** 3], np.float64)
+        off += 16 * r * p ** 3
+        U, V = e.operator(int(idx))
+        assert np.array_equal(K[: r * p ** 3].reshape(r, -1).T, U)
+        assert np.array_equal(K[r * p ** 3:].reshape(r, -1).T, V)
+        seen.append(int(idx))
+    assert seen == sorted(_canonical())
+    x, y, z, q = uniform_cube(8000, 91)
+    e.set_particles(x, y, z, q)
+    e.build_tree()
+    e.evaluate()
+    a = e.results()
+    f = _engine(h, p)
+    f.load_operators(path)
+    f.set_particles(x, y, z, q)
+    f.build_tree()
+    f.evaluate()
+    assert all(np.array_equal(u, v) for u, v in zip(a, f.results()))
+    # the oracle's own operators through the file
+    cheb = oracle.Cheb(p)
+    _write_operator_file(tmp_path / "orc.bin", {i: cheb.K(i) for i in _canonical()})
+    g = _engine(h, p)
+    g.load_operators(tmp_path / "orc.bin")
+    g.set_particles(x, y, z, q)
+    g.build_tree()
+    g.evaluate()
+    ref = oracle.fmm(x, y, z, q, (0.5, 0.5, 0.5), 1.0, h, p)
+    for u, v in zip(g.results(), ref[:4]):
+        assert _rel(u, v) <= FP64_TOL
+    # a file without one canonical operator is refused, and the context keeps working
+    ops = {i: cheb.K(i) for i in _canonical()[1:]}
+    _write_operator_file(tmp_path / "bad.bin", ops)
+    with pytest.raises(Exception):
+        g.load_operators(tmp_path / "bad.bin")
+    (tmp_path / "cut.bin").write_bytes(raw[: len(raw) // 2])
+    with pytest.raises(Exception):
+        g.load_operators(tmp_path / "cut.bin")
+
+
 def test_fmm_clustered_expansions(oracle):
     x, y, z, q = _clustered(20000, 5)
     h, order = 6, 6
