@@ -6,3 +6,4 @@ from .fmm import (  # noqa: F401
     FFmmNearField, FFmmP2M, FFmmP2P, FGroupTaskAlgorithm, FGroupTree, FmmError,
 )
 from .batch import BatchEngine, default_height  # noqa: F401
+from .octree import FFmmAlgorithm, FFmmAlgorithmThread, FOctree  # noqa: F401
