@@ -287,7 +287,7 @@ def run_batch(args):
         res[form] = {"value": n * BATCH_RHS / (ms / 1e3) / 1e6, "ms_per_step": ms, "build_ms": build_ms,
                      "kernel_ms_per_step": kms, "roofline": rl, "accuracy_vs_direct": acc,
                      "trees": [{"height": t.set_height, "sets": t.nsets, "particles": t.n,
-                                "m2l_variant": t.m2l_variant()[0], "m2l_tune_ms": t.m2l_variant()[1][:3]}
+                                "m2l_variant": t.m2l_variant()[0], "m2l_tune_ms": t.m2l_variant()[1]}
                                for t in trees]}
         for t in trees:
             t.close()
@@ -409,7 +409,7 @@ def main():
                    "seed": SEED[dist], "height": h, "order": p, "epsilon": 10.0 ** -p, "precision": args.precision},
         "roofline": rl,
         "kernel_ms_per_step": kms,
-        "m2l_variant": {"chosen": m2l_variant[0], "tune_ms": m2l_variant[1][:3]} if mixed else None,
+        "m2l_variant": {"chosen": m2l_variant[0], "tune_ms": m2l_variant[1]} if mixed else None,
         "operator_setup_ms": setup_ms,
         "fp64_gpu": fp64,
         "mixed_vs_fp64": (value / fp64["value"]) if fp64 else None,

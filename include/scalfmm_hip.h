@@ -257,7 +257,8 @@ int hfmm_operator_file_name(int32_t order, double epsilon, char* buf, int32_t si
 /* ---- M2L kernel variants (autotuned) ---------------------------------------- */
 /* The mixed-precision M2L has several tile shapes: 0 = four waves share 64
  * targets (every order), 1 = one 16-target tile per wave (orders 5 and 6),
- * 2 = two 16-target tiles per wave (order 5).  Variant -1 (the default) times
+ * 2 = two 16-target tiles per wave (order 5), 3 = four waves share 32
+ * targets (every order).  Variant -1 (the default) times
  * every available variant once at the next M2L on the actual tree and keeps
  * the fastest for the context's later evaluations.  The FP64 form has one
  * kernel (its variant stays -1).  hfmm_get_m2l_variant also returns the

@@ -254,8 +254,9 @@ __device__ __forceinline__ void g1_block(f4 (&c1)[KT][COLS], const bf8 (&b)[3][C
 //   GEMM1   each wave multiplies its quarter of the node dimension into a
 //           partial C tile, stored as [wave][target][row] floats;
 //   barrier
-//   reduce  wave w sums the four partials of column tile w once, splits the
-//           sums into three bf16 parts and stores them as B fragments;
+//   reduce  the (column tile, K block) pairs are spread over the waves: each
+//           sums the four partials of its pair once, splits the sums into
+//           three bf16 parts and stores them as B fragments;
 //   barrier
 //   GEMM2   every wave reads the B fragments (16-byte LDS loads) for its row
 //           tiles of U.
@@ -271,7 +272,7 @@ __device__ __forceinline__ void mix_segment(MixState<P, COLS>& st, int i0, int i
                                             const uint16_t* __restrict__ opm, float* Cp, bf8* Bs, int LD, int CR,
                                             int wave, int lane, int col, int kq, int kb0, int kb1) {
     constexpr int W = kM2LWaves;
-    static_assert(COLS == W, "one column tile of C per wave in the reduction");
+    static_assert(COLS == 2 || COLS == 4, "the reduction spreads (column tile, K block) jobs over 4 waves");
     constexpr int NTW = MixState<P, COLS>::NTW;
     constexpr int TC = 16 * COLS;
     constexpr int KP16 = 16 * KT;       // GEMM1 rows: rank padded to 16
@@ -332,9 +333,11 @@ __device__ __forceinline__ void mix_segment(MixState<P, COLS>& st, int i0, int i
         __syncthreads();
         // Rows 16 KT .. KP32 of C are zero: the segments run with growing KT,
         // so no earlier slot wrote them, and the LDS was cleared at the start.
+        // reduction jobs (column tile ct, K block kb), job = ct + COLS kb, over the waves
 #pragma unroll
-        for (int kb = 0; kb < KB32; ++kb) {
-            const float* q = Cp + (16 * wave + col) * CR + 32 * kb + 8 * kq;
+        for (int job = wave; job < COLS * KB32; job += W) {
+            const int ct = job % COLS, kb = job / COLS;
+            const float* q = Cp + (16 * ct + col) * CR + 32 * kb + 8 * kq;
             f4 v0 = *reinterpret_cast<const f4*>(q), v1 = *reinterpret_cast<const f4*>(q + 4);
 #pragma unroll
             for (int w = 1; w < W; ++w) {
@@ -350,7 +353,7 @@ __device__ __forceinline__ void mix_segment(MixState<P, COLS>& st, int i0, int i
                 m8[e] = m;
                 l8[e] = l;
             }
-            bf8* b = Bs + ((wave * 2 + kb) * 3) * 64 + lane;
+            bf8* b = Bs + ((ct * 2 + kb) * 3) * 64 + lane;
             b[0] = h8;
             b[64] = m8;
             b[128] = l8;
@@ -973,10 +976,13 @@ int m2l_mixed_wave_launch(Context* c) {
 //   0  four waves share 64 targets, GEMM1 split over the waves (every order)
 //   1  one 16-target tile per wave, waves independent (p <= 6)
 //   2  two 16-target tiles per wave (p = 5)
+//   3  four waves share 32 targets (every order): half the registers and LDS
+//      of shape 0 for twice its operator reads per target
 template <int P>
 int m2l_mixed_variant(Context* c, int v) {
     switch (v) {
         case 0: return m2l_mixed_launch<P, 4>(c);
+        case 3: return m2l_mixed_launch<P, 2>(c);
         case 1:
             if constexpr (P <= kWaveM2LMaxOrder) return m2l_mixed_wave_launch<P, 1>(c);
             break;
@@ -988,8 +994,10 @@ int m2l_mixed_variant(Context* c, int v) {
     return fail(c, HFMM_ERR_ARG, "mixed M2L variant not available at this order");
 }
 
-constexpr int kMixedVariants = 3;
-inline bool mixed_variant_ok(int p, int v) { return v == 0 || (v == 1 && p <= kWaveM2LMaxOrder) || (v == 2 && p <= 5); }
+constexpr int kMixedVariants = 4;
+inline bool mixed_variant_ok(int p, int v) {
+    return v == 0 || v == 3 || (v == 1 && p <= kWaveM2LMaxOrder) || (v == 2 && p <= 5);
+}
 
 // Times every available variant once on the current tree (each writes every
 // local of the M2L levels, so the runs are interchangeable) and keeps the

@@ -223,7 +223,7 @@ class Engine:
         return off, sr[:m], ps[:m]
 
     def set_m2l_variant(self, v):
-        """Mixed M2L tile shape (0, 1, 2) or -1 to autotune at the next M2L."""
+        """Mixed M2L tile shape (0, 1, 2, 3) or -1 to autotune at the next M2L."""
         self._check(self._L.hfmm_set_m2l_variant(self.h, int(v)))
 
     def m2l_variant(self):
