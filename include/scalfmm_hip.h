@@ -142,6 +142,46 @@ int hfmm_evaluate(hfmm_ctx* ctx, uint32_t ops);
  * are those of each set's tree. */
 int hfmm_evaluate_levels(hfmm_ctx* ctx, uint32_t ops, int32_t upper, int32_t lower);
 
+/* The group tree's particle blocks as the boundary: FGroupSeqAlgorithm /
+ * FGroupTaskAlgorithm::execute(ops, upper, lower) (FGroupSeqAlgorithm.hpp:61-
+ * 83, FGroupTaskAlgorithm.hpp:82-111) over an FGroupTree whose leaves are
+ * FP2PGroupParticleContainer<double, 1, 1, nvals>.  blocks[b] is particle
+ * group b's getRawBuffer() (GroupTree/Core/FGroupOfParticles.hpp:211-214: a
+ * 64-byte header, the leaf headers, then x, y, z and the nvals physical values
+ * as arrays of nbParticlesAllocatedInGroup doubles), attributes[b] its
+ * getRawAttributesBuffer() (:222-231: potential[v], forceX[v], forceY[v],
+ * forceZ[v] at arrays v, nvals+v, 2 nvals+v, 3 nvals+v).  The particles are
+ * evaluated as one set (the blocks of one tree, in any block order) and the
+ * results are ADDED to the attribute buffers, as the kernel's P2P and L2P add
+ * into the containers (FChebSymKernel.hpp:432-486).  Fails with HFMM_ERR_ARG
+ * on inconsistent headers (the reference asserts on the same conditions) or on
+ * a particle whose position does not lie in the leaf it is stored under; the
+ * attribute buffers are then untouched.  The context's tree height, box and
+ * order must be the group tree's.  The cells of the group tree are not read or
+ * written: multipoles and locals stay inside the engine. */
+int hfmm_evaluate_particle_blocks(hfmm_ctx* ctx, int32_t nblocks, const void* const* blocks, double* const* attributes,
+                                  int32_t nvals, uint32_t ops, int32_t upper, int32_t lower);
+
+/* The state of the cells (multipoles and locals) between evaluations.
+ * HFMM_CELLS_FRESH (the default): every evaluation starts from cells in their
+ * initial state, as after tree.forEachCell(resetToInitialState)
+ * (FChebCell.hpp:151-152), so evaluating twice gives the same results.
+ * HFMM_CELLS_KEPT: the cells keep their expansions from one evaluation to the
+ * next and every pass adds into them, as the reference's P2M, M2M, M2L and
+ * L2L add into the tree's cells (FChebInterpolator.hpp:627-700, 560-582;
+ * FChebSymKernel.hpp:193-206, 210-328, 331-350): execute(P2M|M2M) followed by
+ * execute(M2L|L2L|L2P) computes the far field, and executing twice without a
+ * reset accumulates as the reference does.  The particle results of an
+ * evaluation are its own contributions in both modes (the containers of the
+ * reference's trees add them up).  A new tree (hfmm_build_tree), a change of
+ * mode and hfmm_reset_cells put the cells back in their initial state.  With
+ * several charge vectors every vector keeps its own cells; hfmm_get_expansions
+ * shows the last vector's. */
+#define HFMM_CELLS_FRESH 0
+#define HFMM_CELLS_KEPT 1
+int hfmm_set_cell_mode(hfmm_ctx* ctx, int32_t mode);
+int hfmm_reset_cells(hfmm_ctx* ctx);
+
 /* Waits for the context's stream. */
 int hfmm_synchronize(hfmm_ctx* ctx);
 

@@ -1043,6 +1043,7 @@ struct orc_run {
     /* sorted particle SoA */
     double *x, *y, *z, *q, *pot, *fx, *fy, *fz;
     orc_stats st;
+    double center[3], width;
 };
 
 /* P2P for one target leaf against the particles of another leaf, one-way.
@@ -1185,6 +1186,8 @@ static int run_setup(const orc_cheb* c, int64_t n, const double* x, const double
     double t0 = wall_seconds();
     struct orc_run* r = (struct orc_run*)calloc(1, sizeof(struct orc_run));
     r->c = c;
+    for (int d = 0; d < 3; ++d) r->center[d] = center[d];
+    r->width = width;
     int st = orc_tree_build(n, x, y, z, center, width, height, &r->t);
     if (st) { free(r); return st; }
     const size_t nb = sizeof(double) * (size_t)(n > 0 ? n : 1);
@@ -1224,6 +1227,39 @@ int orc_run_create(const orc_cheb* c, int64_t n, const double* x, const double* 
 }
 
 const orc_tree* orc_run_tree(const orc_run* r) { return r->t; }
+
+/* Another execute(ops, upper, lower) on the run's tree: the passes add into
+ * the cells and the particle results left by earlier executions, as the
+ * reference's tree keeps them (every kernel operator accumulates:
+ * FChebInterpolator.hpp:627-700, 560-582; FChebSymKernel.hpp:193-206,
+ * 210-328, 331-350, 432-486).  pot..fz receive the accumulated particle
+ * results in input order (NULL: not copied). */
+int orc_run_execute(orc_run* r, unsigned ops, int upper, int lower, double* pot, double* fx, double* fy,
+                    double* fz) {
+    if (!r) return ORC_ERR_ARG;
+    const int h = r->t->height;
+    if (upper < 2 || upper > lower || lower > h) return ORC_ERR_ARG;
+    run_passes(r, ops, r->center, r->width, upper, lower);
+    for (int64_t i = 0; i < r->t->n; ++i) {
+        const int64_t o = r->t->perm[i];
+        if (pot) pot[o] = r->pot[i];
+        if (fx) fx[o] = r->fx[i];
+        if (fy) fy[o] = r->fy[i];
+        if (fz) fz[o] = r->fz[i];
+    }
+    return ORC_OK;
+}
+
+/* tree.forEachCell(resetToInitialState) (FChebCell.hpp:151-152): zero
+ * multipoles and locals; the particles keep their results. */
+void orc_run_reset_cells(orc_run* r) {
+    if (!r) return;
+    for (int l = 2; l < r->t->height; ++l) {
+        const size_t sz = sizeof(double) * (size_t)(r->t->ncell[l] > 0 ? r->t->ncell[l] : 1) * r->c->nn;
+        memset(r->M[l], 0, sz);
+        memset(r->L[l], 0, sz);
+    }
+}
 
 void orc_run_expansions(const orc_run* r, int level, double* M, double* L) {
     if (level < 2 || level >= r->t->height) return;

@@ -152,6 +152,13 @@ int orc_run_create(const orc_cheb* c, int64_t n, const double* x, const double* 
 void orc_run_destroy(orc_run* r);
 const orc_tree* orc_run_tree(const orc_run* r);
 void orc_run_expansions(const orc_run* r, int level, double* M, double* L);
+/* Another execute on the run's tree, adding into its cells and particle results
+ * (the reference's trees keep both between executions); outputs the
+ * accumulated particle results in input order. */
+int orc_run_execute(orc_run* r, unsigned ops, int upper, int lower, double* pot, double* fx, double* fy,
+                    double* fz);
+/* forEachCell(resetToInitialState): zero multipoles and locals. */
+void orc_run_reset_cells(orc_run* r);
 
 /* O(N^2) direct sum (Kernels/P2P/FP2PR.hpp:14-42 conventions):
  * pot_i = sum_{j!=i} q_j/r_ij ; f_i = q_i grad_i(pot_i) = -q_i sum q_j (x_i-x_j)/r^3.

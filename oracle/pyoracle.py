@@ -90,6 +90,8 @@ def lib():
     L.orc_run_tree.restype = vp
     L.orc_run_tree.argtypes = [vp]
     L.orc_run_expansions.argtypes = [vp, C.c_int, _f64p, _f64p]
+    L.orc_run_execute.argtypes = [vp, C.c_uint, C.c_int, C.c_int, _f64p, _f64p, _f64p, _f64p]
+    L.orc_run_reset_cells.argtypes = [vp]
     L.orc_direct.argtypes = [C.c_int64, _f64p, _f64p, _f64p, _f64p, C.c_int64, vp, C.c_int,
                              _f64p, _f64p, _f64p, _f64p]
     L.orc_qr.argtypes = [C.c_int, C.c_int, _f64p, _f64p, _f64p]
@@ -317,12 +319,28 @@ class Run:
             raise RuntimeError(f"orc_run_create failed: {rc}")
         self.h = h
         self.cheb = cheb
+        self.n = len(x)
+        self.height = int(height)
         self.tree = Tree(x, y, z, center, width, height, handle=lib().orc_run_tree(h), owner=False)
 
     def __del__(self):
         if getattr(self, "h", None) and _lib is not None:
             _lib.orc_run_destroy(self.h)
             self.h = None
+
+    def execute(self, ops=ALL, upper=2, lower=None):
+        """Another execute on this run's tree: passes add into the cells and
+        particle results of earlier ones; returns the accumulated results."""
+        n = self.n
+        out = [np.zeros(max(n, 1)) for _ in range(4)]
+        lower = self.height if lower is None else lower
+        rc = lib().orc_run_execute(self.h, ops, int(upper), int(lower), *out)
+        if rc:
+            raise RuntimeError(f"orc_run_execute failed: {rc}")
+        return tuple(a[:n] for a in out)
+
+    def reset_cells(self):
+        lib().orc_run_reset_cells(self.h)
 
     def expansions(self, level):
         nc = self.tree.num_cells(level)

@@ -19,6 +19,9 @@ NEAR_FIELD = P2P
 FAR_FIELD = P2M | M2M | M2L | L2L | L2P
 NEAR_AND_FAR = NEAR_FIELD | FAR_FIELD
 
+CELLS_FRESH = 0
+CELLS_KEPT = 1
+
 PRECISION_FP64 = 0
 PRECISION_MIXED = 1
 
@@ -37,6 +40,7 @@ EXPORTS = (
     "hfmm_batch_get_results", "hfmm_batch_info", "hfmm_batch_tree", "hfmm_get_m2l_counts",
     "hfmm_set_m2l_variant", "hfmm_get_m2l_variant", "hfmm_evaluate_levels",
     "hfmm_fma_read", "hfmm_fma_write", "hfmm_save_operators", "hfmm_load_operators", "hfmm_operator_file_name",
+    "hfmm_evaluate_particle_blocks", "hfmm_set_cell_mode", "hfmm_reset_cells",
 )
 MAX_RHS = 64
 _NON_STATUS = ("hfmm_last_error", "hfmm_version", "hfmm_batch_last_error", "hfmm_batch_default_height")
@@ -129,6 +133,9 @@ def lib(path: str | None = None):
     L.hfmm_load_operators.argtypes = [vp, C.c_char_p]
     L.hfmm_operator_file_name.argtypes = [i32, d, C.c_char_p, i32]
     L.hfmm_get_m2l_variant.argtypes = [vp, C.POINTER(i32), C.POINTER(C.c_float)]
+    L.hfmm_set_cell_mode.argtypes = [vp, i32]
+    L.hfmm_reset_cells.argtypes = [vp]
+    L.hfmm_evaluate_particle_blocks.argtypes = [vp, i32, C.POINTER(vp), C.POINTER(dp), i32, C.c_uint32, i32, i32]
     for name in EXPORTS:
         if name not in _NON_STATUS:
             getattr(L, name).restype = C.c_int

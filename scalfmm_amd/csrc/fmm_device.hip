@@ -720,6 +720,15 @@ __global__ __launch_bounds__(256) void k_zero_words(uint32_t* __restrict__ p, in
     for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < words; i += stride) p[i] = 0u;
 }
 
+// Grid-stride copy (ADD = false) or sum (ADD = true) of count doubles.
+template <bool ADD>
+__global__ __launch_bounds__(256) void k_combine(double* __restrict__ dst, const double* __restrict__ src,
+                                                 int64_t count) {
+    const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
+    for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < count; i += stride)
+        dst[i] = ADD ? dst[i] + src[i] : src[i];
+}
+
 }  // namespace
 
 Box make_box(const Context* c) {
@@ -891,6 +900,18 @@ int zero_device(Context* c, void* ptr, size_t bytes) {
     hipLaunchKernelGGL(k_zero_words, dim3(static_cast<unsigned>(blocks)), dim3(256), 0, c->stream,
                        static_cast<uint32_t*>(ptr), words);
     return check_launch(c, "zero");
+}
+
+int combine_device(Context* c, double* dst, const double* src, int64_t count, bool add) {
+    if (count <= 0) return HFMM_OK;
+    const int64_t blocks = std::min<int64_t>((count + 1023) / 1024, 65536);
+    if (add)
+        hipLaunchKernelGGL(k_combine<true>, dim3(static_cast<unsigned>(blocks)), dim3(256), 0, c->stream, dst, src,
+                           count);
+    else
+        hipLaunchKernelGGL(k_combine<false>, dim3(static_cast<unsigned>(blocks)), dim3(256), 0, c->stream, dst, src,
+                           count);
+    return check_launch(c, "combine");
 }
 
 int zero_results(Context* c) { return zero_device(c, c->d_res_sorted.ptr, sizeof(double) * 4 * c->n); }

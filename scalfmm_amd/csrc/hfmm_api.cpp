@@ -198,7 +198,7 @@ static void free_all(Context* c) {
                       &c->d_scan_part, &c->d_qin, &c->d_flags, &c->d_scan, &c->d_sorted, &c->d_perm, &c->d_pleaf,
                       &c->d_count, &c->d_cellkey, &c->d_cellkey_tmp, &c->d_childstart, &c->d_leafstart,
                       &c->d_p2p, &c->d_p2pitems, &c->d_m2l, &c->d_order, &c->d_pcount, &c->d_dense, &c->d_M, &c->d_L, &c->d_Mb, &c->d_loc, &c->d_res_sorted,
-                      &c->d_res, &c->d_field, &c->d_setoff};
+                      &c->d_res, &c->d_field, &c->d_setoff, &c->d_keep, &c->d_kept};
     for (DevBuf* b : bufs) release(*b);
 }
 
@@ -264,33 +264,55 @@ int create_context(const hfmm_params* params, hipStream_t shared_stream, hfmm_ct
     return HFMM_OK;
 }
 
+// Kept cells: runs `pass`, which overwrites `count` doubles at `dst`, so that
+// it adds to them instead (snapshot, pass, add the snapshot back).  Fresh
+// cells: runs the pass.
+template <class Pass>
+static int accumulating(Context* c, double* dst, int64_t count, Pass pass) {
+    if (!c->keep_cells) return pass();
+    double* snap = c->d_keep.as<double>();
+    HFMM_TRY(combine_device(c, snap, dst, count, false));
+    HFMM_TRY(pass());
+    return combine_device(c, dst, snap, count, true);
+}
+
 // The passes of one charge vector over the working levels [work_upper,
 // work_lower) with the loop bounds of FGroupSeqAlgorithm
 // (GroupTree/Core/FGroupSeqAlgorithm.hpp:88, 137, 238): M2M parents from
 // min(h-2, lower-1) down to upper, M2L levels lower-1 .. upper, L2L parents
-// upper .. lower-2.  Results into d_res_sorted.
+// upper .. lower-2.  Results into d_res_sorted.  With fresh cells every
+// evaluation starts from zero expansions; with kept cells every pass adds
+// into the expansions left by earlier evaluations, as the reference's P2M,
+// M2M, M2L and L2L add into the tree's cells (FChebInterpolator.hpp:627-700,
+// 560-582; FChebSymKernel.hpp:193-206, 210-328, 331-350).
 static int evaluate_passes(Context* c, uint32_t ops, size_t exp_bytes) {
     const int L = c->height() - 1;
     const int up = c->work_upper, lo = c->work_lower;
+    const int64_t LD = c->LD;
+    double* M = c->d_M.as<double>();
+    double* Lx = c->d_L.as<double>();
+    const bool keep = c->keep_cells;
     // upward pass
-    if (ops & HFMM_P2M) HFMM_TRY(hfmm::run_p2m(c));
-    else HFMM_TRY(hfmm::zero_device(c, c->d_M.ptr, exp_bytes));
+    if (ops & HFMM_P2M)
+        HFMM_TRY(accumulating(c, M + c->lvl_off[L] * LD, c->ncell[L] * LD, [&] { return hfmm::run_p2m(c); }));
+    else if (!keep)
+        HFMM_TRY(hfmm::zero_device(c, c->d_M.ptr, exp_bytes));
     if (ops & HFMM_M2M)
-        for (int l = std::min(L - 1, lo - 1); l >= up; --l) HFMM_TRY(hfmm::run_m2m(c, l));
+        for (int l = std::min(L - 1, lo - 1); l >= up; --l)
+            HFMM_TRY(accumulating(c, M + c->lvl_off[l] * LD, c->ncell[l] * LD, [&] { return hfmm::run_m2m(c, l); }));
     // transfer pass: M2L writes every local of its levels; the others start at zero
     const bool mixed = c->P.precision == HFMM_PRECISION_MIXED;
     const bool all_levels = up == c->top() && lo == c->height();
-    if ((ops & HFMM_M2L) && !all_levels) HFMM_TRY(hfmm::zero_device(c, c->d_L.ptr, exp_bytes));
     if (ops & HFMM_M2L) {
-        if (mixed) {
+        HFMM_TRY(accumulating(c, Lx, static_cast<int64_t>(exp_bytes / sizeof(double)), [&] {
+            if (keep || !all_levels) HFMM_TRY(hfmm::zero_device(c, c->d_L.ptr, exp_bytes));
+            if (!mixed) return hfmm::run_m2l(c);
             HFMM_TRY(hfmm::run_split_multipoles(c));
-            HFMM_TRY(hfmm::run_m2l_mixed(c));
-        } else {
-            HFMM_TRY(hfmm::run_m2l(c));
-        }
-    } else
+            return hfmm::run_m2l_mixed(c);
+        }));
+    } else if (!keep)
         HFMM_TRY(hfmm::zero_device(c, c->d_L.ptr, exp_bytes));
-    // downward pass
+    // downward pass (L2L adds into the children)
     if (ops & HFMM_L2L)
         for (int l = up; l < lo - 1; ++l) HFMM_TRY(hfmm::run_l2l(c, l));
     // near field initialises the results, the merge adds the far field
@@ -422,6 +444,7 @@ int hfmm_build_tree(hfmm_ctx* h) {
     if (c->n <= 0) return hfmm::fail(c, HFMM_ERR_STATE, "build_tree before set_particles");
     HFMM_HIP(c, hipSetDevice(c->device));
     c->tree_built = false;
+    c->cells_dirty = true;  // a new tree: cells in their initial state
     return hfmm::build_tree_device(c);
 }
 
@@ -450,11 +473,34 @@ int hfmm_evaluate_levels(hfmm_ctx* h, uint32_t ops, int32_t upper, int32_t lower
     HFMM_TRY(hfmm::ensure(c, c->d_field, sizeof(double) * 3 * c->n * c->nrhs));
     HFMM_TRY(hfmm::zero_device(c, c->d_M.as<double>() + static_cast<size_t>(c->total_cells) * c->LD,
                                sizeof(double) * c->LD));
+    // kept cells of several charge vectors: the cells of vector r are stored at
+    // d_kept + 2 r exp (M, then L) between evaluations
+    const int64_t exp_words = static_cast<int64_t>(exp_bytes / sizeof(double));
+    const bool kept_many = c->keep_cells && c->nrhs > 1;
+    if (c->keep_cells) {
+        HFMM_TRY(hfmm::ensure(c, c->d_keep, exp_bytes));
+        if (kept_many) HFMM_TRY(hfmm::ensure(c, c->d_kept, 2 * exp_bytes * c->nrhs));
+        if (c->cells_dirty) {  // the cells of a new tree, or after hfmm_reset_cells
+            HFMM_TRY(hfmm::zero_device(c, c->d_M.ptr, exp_bytes));
+            HFMM_TRY(hfmm::zero_device(c, c->d_L.ptr, exp_bytes));
+            if (kept_many) HFMM_TRY(hfmm::zero_device(c, c->d_kept.ptr, 2 * exp_bytes * c->nrhs));
+            c->cells_dirty = false;
+        }
+    }
     // one pass sequence per charge vector (the tree and the lists are shared);
     // with one vector the sorted charges are already in place
     for (int r = 0; r < c->nrhs; ++r) {
+        double* kept = kept_many ? c->d_kept.as<double>() + 2 * exp_words * r : nullptr;
         if (c->nrhs > 1) HFMM_TRY(hfmm::update_charges_device(c, r));
+        if (kept) {
+            HFMM_TRY(hfmm::combine_device(c, c->d_M.as<double>(), kept, exp_words, false));
+            HFMM_TRY(hfmm::combine_device(c, c->d_L.as<double>(), kept + exp_words, exp_words, false));
+        }
         HFMM_TRY(hfmm::evaluate_passes(c, ops, exp_bytes));
+        if (kept) {
+            HFMM_TRY(hfmm::combine_device(c, kept, c->d_M.as<double>(), exp_words, false));
+            HFMM_TRY(hfmm::combine_device(c, kept + exp_words, c->d_L.as<double>(), exp_words, false));
+        }
         HFMM_TRY(hfmm::scatter_results(c, r));
     }
     c->evaluated = true;
@@ -711,6 +757,10 @@ int hfmm_get_expansions(hfmm_ctx* h, int32_t level, double* M, double* L) {
     for (int w = 0; w < 2; ++w) {
         double* out = w == 0 ? M : L;
         if (!out) continue;
+        if (c->keep_cells && c->cells_dirty) {  // reset, not yet evaluated: the initial state
+            std::memset(out, 0, sizeof(double) * static_cast<size_t>(nc) * nn);
+            continue;
+        }
         const hfmm::DevBuf& b = w == 0 ? c->d_M : c->d_L;
         HFMM_HIP(c, hipMemcpy(tmp.data(), b.as<double>() + static_cast<size_t>(c->lvl_off[level]) * c->LD,
                               sizeof(double) * tmp.size(), hipMemcpyDeviceToHost));
@@ -718,6 +768,22 @@ int hfmm_get_expansions(hfmm_ctx* h, int32_t level, double* M, double* L) {
             std::memcpy(out + static_cast<size_t>(i) * nn, tmp.data() + static_cast<size_t>(i) * c->LD,
                         sizeof(double) * nn);
     }
+    return HFMM_OK;
+}
+
+int hfmm_set_cell_mode(hfmm_ctx* h, int32_t mode) {
+    CTX_CHECK(h);
+    Context* c = &h->c;
+    if (mode != HFMM_CELLS_FRESH && mode != HFMM_CELLS_KEPT) return hfmm::fail(c, HFMM_ERR_ARG, "unknown cell mode");
+    const bool keep = mode == HFMM_CELLS_KEPT;
+    if (keep != c->keep_cells) c->cells_dirty = true;  // kept cells start from their initial state
+    c->keep_cells = keep;
+    return HFMM_OK;
+}
+
+int hfmm_reset_cells(hfmm_ctx* h) {
+    CTX_CHECK(h);
+    h->c.cells_dirty = true;
     return HFMM_OK;
 }
 

@@ -119,6 +119,13 @@ struct Context {
     DevBuf d_res_sorted;  // pot, grad(phi) x, y, z (sorted order; the charge is applied by the scatter)
     DevBuf d_res;         // pot, fx, fy, fz (input order)
     DevBuf d_field;       // grad(phi) x, y, z (input order): the force without the target charge
+    // Cell state between evaluations: fresh (every evaluation starts from zero
+    // expansions) or kept (the passes add into the expansions of earlier ones)
+    bool keep_cells = false;
+    bool cells_dirty = true;  // kept mode: zero M and L before the next evaluation
+    DevBuf d_keep;            // kept mode: snapshot of the expansions a pass overwrites
+    DevBuf d_kept;            // kept mode, nrhs > 1: M and L of every charge vector
+    std::vector<double> block_xyz;  // positions of the last particle-block evaluation (tree reuse)
 
     // ---- profiling
     bool profiling = false;
@@ -187,6 +194,8 @@ int zero_results(Context* c);
 int scatter_results(Context* c, int rhs = 0);
 // clears device memory with a fill kernel (capturable, unlike hipMemsetAsync)
 int zero_device(Context* c, void* ptr, size_t bytes);
+// dst[i] = src[i] (add = false) or dst[i] += src[i] (add = true), i < count
+int combine_device(Context* c, double* dst, const double* src, int64_t count, bool add);
 int selftest_mfma_f64(Context* c, double* out);
 
 // Mixed-precision passes (fmm_mixed.hip)

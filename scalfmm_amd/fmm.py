@@ -160,6 +160,36 @@ class Engine:
         else:
             self._check(self._L.hfmm_evaluate_levels(self.h, int(ops), int(levels[0]), int(levels[1])))
 
+    def set_cell_mode(self, mode):
+        """_abi.CELLS_FRESH (each evaluation starts from zero expansions) or
+        _abi.CELLS_KEPT (the passes add into the expansions of earlier
+        evaluations, as the reference's tree cells keep them)."""
+        self._check(self._L.hfmm_set_cell_mode(self.h, int(mode)))
+
+    def reset_cells(self):
+        """tree.forEachCell(resetToInitialState) for the kept cells."""
+        self._check(self._L.hfmm_reset_cells(self.h))
+
+    def evaluate_particle_blocks(self, blocks, attributes, nvals, ops=NEAR_AND_FAR, upper=2, lower=None):
+        """hfmm_evaluate_particle_blocks: `blocks` are the raw particle-block
+        buffers of a group tree, `attributes` their attribute buffers (numpy
+        arrays; the results are added to the latter)."""
+        nb = len(blocks)
+        if len(attributes) != nb:
+            raise ValueError("one attributes buffer per block")
+        for a in attributes:
+            if a.dtype != np.float64 or not a.flags.c_contiguous or not a.flags.writeable:
+                raise ValueError("attribute buffers must be writeable contiguous float64 arrays")
+        bp = (C.c_void_p * max(nb, 1))(*[b.ctypes.data for b in blocks])
+        ap = (C.POINTER(C.c_double) * max(nb, 1))(*[_dp(a) for a in attributes])
+        lower = self.height if lower is None else int(lower)
+        rc = self._L.hfmm_evaluate_particle_blocks(self.h, nb, bp, ap, int(nvals), int(ops), int(upper), lower)
+        self._check(rc)
+        n = int(sum(int(b[56:64].view(np.int64)[0]) for b in blocks))
+        if n:
+            self.n, self.nrhs, self.nsets = n, int(nvals), 1
+            self.tree_stamp += 1
+
     def synchronize(self):
         self._check(self._L.hfmm_synchronize(self.h))
 

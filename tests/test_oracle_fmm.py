@@ -147,3 +147,43 @@ def test_fmm_working_levels(oracle):
         oracle.fmm(x, y, z, q, c, 1.0, h, p, levels=(1, h))
     with pytest.raises(RuntimeError):
         oracle.fmm(x, y, z, q, c, 1.0, h, p, levels=(3, 2))
+
+
+def test_fmm_executions_on_one_tree(oracle):
+    """Executions on one tree (the oracle's persistent run): every pass adds into
+    the cells and particle results left by earlier executions, as the
+    reference's kernel operators do (FChebInterpolator.hpp:627-700, 560-582;
+    FChebSymKernel.hpp:193-206, 210-328, 331-350).  Pins: split pass sequences
+    reproduce one execution bit for bit; after forEachCell(resetToInitialState)
+    a second execution adds exactly the first one's results; without the reset
+    the multipoles of the second execution are (L - l + 2) times the first's at
+    level l (leaf level L: P2M adds the leaf expansions again, each M2M adds the
+    children's new sums) and the level-2 locals (1 + L) times (the old locals
+    plus the M2L of L-fold multipoles)."""
+    from scalfmm_amd.distributions import uniform_cube
+    x, y, z, q = uniform_cube(4000, 33)
+    h, p, c = 4, 5, (0.5, 0.5, 0.5)
+    cheb = oracle.Cheb(p)
+    full = oracle.fmm(x, y, z, q, c, 1.0, h, p, cheb=cheb)
+    far = oracle.fmm(x, y, z, q, c, 1.0, h, p, ops=oracle.ALL & ~oracle.P2P, cheb=cheb)
+    P2P, P2M, M2M, M2L, L2L, L2P = (1, 2, 4, 8, 16, 32)
+    for seq, ref in (([P2M | M2M, M2L | L2L | L2P], far), ([P2M | M2M | M2L, L2L | L2P], far),
+                     ([P2M, M2M, M2L, L2L, L2P], far), ([P2P, P2M | M2M | M2L | L2L | L2P], full)):
+        run = oracle.Run(cheb, x, y, z, q, c, 1.0, h, ops=0)
+        for ops in seq:
+            got = run.execute(ops)
+        assert all(np.array_equal(a, b) for a, b in zip(got, ref[:4])), seq
+    run = oracle.Run(cheb, x, y, z, q, c, 1.0, h)  # one full execution
+    first = [run.expansions(l) for l in (2, 3)]
+    run.reset_cells()
+    again = run.execute()
+    # ((near + far) + near) + far: equal to 2 (near + far) up to the rounding of the sums
+    assert all(np.allclose(a, 2 * b, rtol=1e-14, atol=1e-14 * np.abs(b).max()) for a, b in zip(again, full[:4]))
+    run = oracle.Run(cheb, x, y, z, q, c, 1.0, h)
+    run.execute()
+    L = h - 1
+    for lvl, (M1, L1) in zip((2, 3), first):
+        M2, L2 = run.expansions(lvl)
+        assert np.allclose(M2, (L - lvl + 2) * M1, rtol=1e-12, atol=1e-12 * np.abs(M1).max())
+    L2_2 = run.expansions(2)[1]
+    assert np.allclose(L2_2, (1 + L) * first[0][1], rtol=1e-12, atol=1e-12 * np.abs(first[0][1]).max())
