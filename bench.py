@@ -172,10 +172,22 @@ def run_form(parts, h, p, precision, steps, warmup):
     return e, ms, {k: v[0] / steps for k, v in sorted(times.items())}, {k: v[1] / steps for k, v in times.items()}
 
 
-def roofline(engines, kernel_ms, launches, p, mixed, workload, nrhs=1):
+# M2L partial products per product of each form (bfloat16 MFMA in the mixed forms)
+PRODUCTS = {"mixed": 6, "hilo": 3, "fp64": 1}
+DTYPE = {"mixed": "bf16x3-mfma/f32 (M2L), f32 (P2P), f64 (rest)",
+         "hilo": "bf16x2-mfma/f32 (M2L), f32 (P2P), f64 (rest)", "fp64": "f64"}
+
+
+def precision_code(form):
+    from scalfmm_amd import _abi
+    return {"mixed": _abi.PRECISION_MIXED, "hilo": _abi.PRECISION_MIXED_HILO, "fp64": _abi.PRECISION_FP64}[form]
+
+
+def roofline(engines, kernel_ms, launches, p, form, workload, nrhs=1):
     """Roofline of the kernel with the largest time share; `engines` are the
     trees the kernel times cover (one, or the per-height trees of a batch),
-    each evaluated for `nrhs` charge vectors."""
+    each evaluated for `nrhs` charge vectors; `form` is mixed, hilo or fp64."""
+    mixed = form != "fp64"
     m2l_name = "m2l_mixed" if mixed else "m2l"
     p2p_small, p2p_big = ("p2p_mixed", "p2p_mixed_big") if mixed else ("p2p", "p2p_big")
     top = max(kernel_ms, key=kernel_ms.get)
@@ -185,11 +197,11 @@ def roofline(engines, kernel_ms, launches, p, mixed, workload, nrhs=1):
             f, c = algorithmic_m2l_flops(e, e.height, p)
             flops += f * nrhs
             n_int += c * nrhs
-        work = 6 * flops if mixed else flops
+        work = PRODUCTS[form] * flops
         peak = PEAK["bf16_mfma"] if mixed else PEAK["fp64_mfma"]
         kname, bound = ("k_m2l_mixed", "mfma") if mixed else ("k_m2l_f64", "mfma")
         extra = {"m2l_interactions": n_int, "fp64_equivalent_flops_per_step": flops,
-                 "work": "6 bf16 partial products per product" if mixed else "FP64 products"}
+                 "work": f"{PRODUCTS[form]} bf16 partial products per product" if mixed else "FP64 products"}
     elif top in (p2p_small, p2p_big):
         pairs = 0.0
         for e in engines:
@@ -250,22 +262,22 @@ def run_batch_form(sets, p, precision, steps, warmup):
 
 def run_batch(args):
     """The batched configuration: every set and charge vector per graph launch."""
-    from scalfmm_amd import Engine, PRECISION_FP64, PRECISION_MIXED
+    from scalfmm_amd import Engine
     from scalfmm_amd.distributions import batch_sets
     dist, nsets, _, p, (csets, _) = WORKLOADS[args.workload]
     print(f"[bench] generating {nsets} sets x {BATCH_RHS} charge vectors", file=sys.stderr, flush=True)
     sets = batch_sets(nsets, BATCH_RHS, SEED[dist])
     sizes, x, y, z, q = sets
     n = int(sizes.sum())
-    mixed = args.precision == "mixed"
+    mixed = args.precision != "fp64"
     forms = [args.precision] + (["fp64"] if mixed and not args.no_fp64_compare else [])
     res = {}
     for form in forms:
         print(f"[bench] {form} batch: build, capture, {args.warmup} warmup + {args.steps} timed graph launches",
               file=sys.stderr, flush=True)
         b, trees, ms, build_ms, kms, launches = run_batch_form(
-            sets, p, PRECISION_MIXED if form == "mixed" else PRECISION_FP64, args.steps, args.warmup)
-        rl = roofline(trees, kms, launches, p, form == "mixed", args.workload, nrhs=BATCH_RHS)
+            sets, p, precision_code(form), args.steps, args.warmup)
+        rl = roofline(trees, kms, launches, p, form, args.workload, nrhs=BATCH_RHS)
         acc = None
         if not args.no_accuracy:
             # FP64 direct sums on sampled sets, every charge vector
@@ -298,7 +310,7 @@ def run_batch(args):
         "value": main_form["value"], "unit": "Mparticle-RHS/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": main_form["ms_per_step"], "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None,
-        "dtype": "bf16x3-mfma/f32 (M2L), f32 (P2P), f64 (rest)" if mixed else "f64",
+        "dtype": DTYPE[args.precision],
         "data": "synthetic",
         "config": {"workload": args.workload, "sets": nsets, "particles": n, "rhs": BATCH_RHS, "order": p,
                    "seed": SEED[dist], "heights": "smallest h in [3, 20] with n <= 32 * 8^(h-1)",
@@ -336,7 +348,8 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="plummer10M-h8-p8", choices=sorted(WORKLOADS))
-    ap.add_argument("--precision", default="mixed", choices=["mixed", "fp64"])
+    ap.add_argument("--precision", default="mixed", choices=["mixed", "hilo", "fp64"],
+                    help="mixed: three-part bf16 splits in the M2L; hilo: two-part (hi/lo) splits; fp64")
     ap.add_argument("--no-fp64-compare", action="store_true", help="skip the FP64 form beside the mixed one")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-accuracy", action="store_true")
@@ -352,23 +365,20 @@ def main():
                                    "so --gpus must be 1"}))
         sys.exit(2)
 
-    from scalfmm_amd import PRECISION_FP64, PRECISION_MIXED
-
     if WORKLOADS[args.workload][0] == "batch":
         run_batch(args)
         return
     dist, n, h, p, (cn, ch) = WORKLOADS[args.workload]
     print(f"[bench] generating {n} {dist} particles", file=sys.stderr, flush=True)
     parts = make_particles(dist, n)
-    mixed = args.precision == "mixed"
+    mixed = args.precision != "fp64"
     print(f"[bench] {args.precision} form: {args.warmup} warmup + {args.steps} timed steps", file=sys.stderr,
           flush=True)
-    e, ms, kms, launches = run_form(parts, h, p, PRECISION_MIXED if mixed else PRECISION_FP64, args.steps,
-                                    args.warmup)
+    e, ms, kms, launches = run_form(parts, h, p, precision_code(args.precision), args.steps, args.warmup)
     m2l_variant = e.m2l_variant()
     setup_ms = e.setup_ms
     value = n / (ms / 1e3) / 1e6
-    rl = roofline([e], kms, launches, p, mixed, args.workload)
+    rl = roofline([e], kms, launches, p, args.precision, args.workload)
     got = e.results()
     direct = None
     if not args.no_accuracy:
@@ -381,9 +391,9 @@ def main():
     got64 = None
     if mixed and not args.no_fp64_compare:
         print("[bench] fp64 form on the same particles", file=sys.stderr, flush=True)
-        e64, ms64, kms64, launches64 = run_form(parts, h, p, PRECISION_FP64, args.steps, args.warmup)
+        e64, ms64, kms64, launches64 = run_form(parts, h, p, precision_code("fp64"), args.steps, args.warmup)
         fp64 = {"value": n / (ms64 / 1e3) / 1e6, "ms_per_step": ms64, "kernel_ms_per_step": kms64,
-                "roofline": roofline([e64], kms64, launches64, p, False, args.workload)}
+                "roofline": roofline([e64], kms64, launches64, p, "fp64", args.workload)}
         got64 = e64.results()
         e64.close()
 
@@ -403,7 +413,7 @@ def main():
         "metric": METRIC, "value": value, "unit": "Mparticles/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "bf16x3-mfma/f32 (M2L), f32 (P2P), f64 (rest)" if mixed else "f64",
+        "dtype": DTYPE[args.precision],
         "data": "synthetic",
         "config": {"workload": args.workload, "particles": n, "distribution": dist,
                    "seed": SEED[dist], "height": h, "order": p, "epsilon": 10.0 ** -p, "precision": args.precision},

@@ -60,6 +60,11 @@ extern "C" {
                                   FP64.  Relative error about 1e-7 on top of the
                                   FP64 path's; no counterpart in the reference (its
                                   FReal=float instantiation is the closest). */
+#define HFMM_PRECISION_MIXED_HILO 2 /* as HFMM_PRECISION_MIXED with two-part (hi/lo)
+                                  bfloat16 splits in the M2L: three partial
+                                  products instead of six, relative M2L error
+                                  about 1e-5; for orders whose interpolation
+                                  error is far above that (order 5: ~1e-4). */
 
 typedef struct hfmm_ctx hfmm_ctx;
 

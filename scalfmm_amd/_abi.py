@@ -24,6 +24,7 @@ CELLS_KEPT = 1
 
 PRECISION_FP64 = 0
 PRECISION_MIXED = 1
+PRECISION_MIXED_HILO = 2
 
 # every symbol the header declares
 EXPORTS = (

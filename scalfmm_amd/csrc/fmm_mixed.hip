@@ -49,6 +49,17 @@ __device__ __forceinline__ f4 mma6(bf8 ah, bf8 am, bf8 al, bf8 bh, bf8 bm, bf8 b
     return mfma_bf16(ah, bh, c);
 }
 
+// NPL = 3: the six leading partial products of a three-part split; NPL = 2:
+// the three of a two-part (hi/lo) split, (ah + am)(bh + bm) without am bm,
+// the third parts unused (their loads are dead code the compiler removes).
+template <int NPL>
+__device__ __forceinline__ f4 mmaN(bf8 ah, bf8 am, bf8 al, bf8 bh, bf8 bm, bf8 bl, f4 c) {
+    if constexpr (NPL == 3) return mma6(ah, am, al, bh, bm, bl, c);
+    c = mfma_bf16(ah, bm, c);
+    c = mfma_bf16(am, bh, c);
+    return mfma_bf16(ah, bh, c);
+}
+
 __device__ __forceinline__ uint16_t bf16_bits(__bf16 v) { return __builtin_bit_cast(uint16_t, v); }
 
 // x = hi + mid + lo, each part rounded to nearest bfloat16.
@@ -214,7 +225,7 @@ __device__ __forceinline__ void load_vfrag(bf8 (&a)[3], const uint16_t* VT, int 
 // GEMM1 on node block kb with the VT rows rolling one rank tile ahead: va[0]
 // holds rank tile 0 of this block on entry and rank tile 0 of block kb + 1 on
 // exit (when `more`).
-template <int COLS, int KT, int KP16>
+template <int COLS, int KT, int KP16, int NPL>
 __device__ __forceinline__ void g1_step(f4 (&c1)[KT][COLS], const bf8 (&b)[3][COLS], bf8 (&va)[2][3],
                                         const uint16_t* VT, int LD, int col, int kq, int kb, bool more) {
 #pragma unroll
@@ -226,7 +237,7 @@ __device__ __forceinline__ void g1_step(f4 (&c1)[KT][COLS], const bf8 (&b)[3][CO
             load_vfrag<KP16>(va[cur ^ 1], VT, LD, 0, col, kq, kb + 1);
 #pragma unroll
         for (int ct = 0; ct < COLS; ++ct)
-            c1[rt][ct] = mma6(va[cur][0], va[cur][1], va[cur][2], b[0][ct], b[1][ct], b[2][ct], c1[rt][ct]);
+            c1[rt][ct] = mmaN<NPL>(va[cur][0], va[cur][1], va[cur][2], b[0][ct], b[1][ct], b[2][ct], c1[rt][ct]);
         __builtin_amdgcn_sched_barrier(0);
     }
     if (KT & 1) {  // the next block's rank tile 0 landed in va[1]
@@ -237,7 +248,7 @@ __device__ __forceinline__ void g1_step(f4 (&c1)[KT][COLS], const bf8 (&b)[3][CO
 }
 
 // GEMM1 on node block kb: the KT rank rows of VT against the multipole fragments.
-template <int COLS, int KT, int KP16>
+template <int COLS, int KT, int KP16, int NPL>
 __device__ __forceinline__ void g1_block(f4 (&c1)[KT][COLS], const bf8 (&b)[3][COLS], const uint16_t* VT, int LD,
                                          int col, int kq, int kb) {
 #pragma unroll
@@ -245,7 +256,7 @@ __device__ __forceinline__ void g1_block(f4 (&c1)[KT][COLS], const bf8 (&b)[3][C
         const uint16_t* a = VT + static_cast<int64_t>(16 * rt + col) * LD + 32 * kb + 8 * kq;
         const bf8 ah = ld8(a), am = ld8(a + KP16 * LD), al = ld8(a + 2 * KP16 * LD);
 #pragma unroll
-        for (int ct = 0; ct < COLS; ++ct) c1[rt][ct] = mma6(ah, am, al, b[0][ct], b[1][ct], b[2][ct], c1[rt][ct]);
+        for (int ct = 0; ct < COLS; ++ct) c1[rt][ct] = mmaN<NPL>(ah, am, al, b[0][ct], b[1][ct], b[2][ct], c1[rt][ct]);
     }
 }
 
@@ -264,7 +275,7 @@ __device__ __forceinline__ void g1_block(f4 (&c1)[KT][COLS], const bf8 (&b)[3][C
 // barrier is passed (every wave has finished reducing), and the next
 // reduction writes the B fragments only after the next first barrier (every
 // wave has finished its GEMM2).
-template <int P, int COLS, int KT>
+template <int P, int COLS, int KT, int NPL>
 __device__ __forceinline__ void mix_segment(MixState<P, COLS>& st, int i0, int i1, const int16_t* s_slot,
                                             const int64_t* s_off, const int32_t* lslots, const int (&tgt)[COLS],
                                             int (&nsrc)[COLS], const uint16_t* __restrict__ Mbl,
@@ -315,10 +326,10 @@ __device__ __forceinline__ void mix_segment(MixState<P, COLS>& st, int i0, int i
             for (int kb = kb0; kb < kb1; kb += 2) {
                 const bool more = kb + 1 < kb1;
                 if (more) load_mfrag<COLS>(b3[1], Mrow, plane, kb + 1);
-                g1_step<COLS, KT, KP16>(c1, b3[0], va, VT, LD, col, kq, kb, more);
+                g1_step<COLS, KT, KP16, NPL>(c1, b3[0], va, VT, LD, col, kq, kb, more);
                 if (more) {
                     if (kb + 2 < kb1) load_mfrag<COLS>(b3[0], Mrow, plane, kb + 2);
-                    g1_step<COLS, KT, KP16>(c1, b3[1], va, VT, LD, col, kq, kb + 1, kb + 2 < kb1);
+                    g1_step<COLS, KT, KP16, NPL>(c1, b3[1], va, VT, LD, col, kq, kb + 1, kb + 2 < kb1);
                 }
             }
             // partial C as [wave][target][row]: lane (col, kq) holds rows
@@ -356,7 +367,7 @@ __device__ __forceinline__ void mix_segment(MixState<P, COLS>& st, int i0, int i
             bf8* b = Bs + ((ct * 2 + kb) * 3) * 64 + lane;
             b[0] = h8;
             b[64] = m8;
-            b[128] = l8;
+            if constexpr (NPL == 3) b[128] = l8;
         }
         __syncthreads();
         // The first row tile of every wave (modes 0..63: the mean potential and
@@ -386,9 +397,9 @@ __device__ __forceinline__ void mix_segment(MixState<P, COLS>& st, int i0, int i
 #pragma unroll
                 for (int ct = 0; ct < COLS; ++ct) {
                     if (t == 0)
-                        fresh[ct] = mma6(ah, am, al, bh[ct], bm[ct], bl[ct], fresh[ct]);
+                        fresh[ct] = mmaN<NPL>(ah, am, al, bh[ct], bm[ct], bl[ct], fresh[ct]);
                     else
-                        st.acc[t][ct] = mma6(ah, am, al, bh[ct], bm[ct], bl[ct], st.acc[t][ct]);
+                        st.acc[t][ct] = mmaN<NPL>(ah, am, al, bh[ct], bm[ct], bl[ct], st.acc[t][ct]);
                 }
             }
         }
@@ -407,7 +418,7 @@ __device__ __forceinline__ void mix_segment(MixState<P, COLS>& st, int i0, int i
     }
 }
 
-template <int P, int COLS>
+template <int P, int COLS, int NPL>
 __global__ __launch_bounds__(256) void k_m2l_mixed(M2LLevels lv, const int32_t* __restrict__ slots,
                                                   const int32_t* __restrict__ order, const int32_t* __restrict__ pcount,
                                                   const int16_t* __restrict__ slotord, const int32_t* __restrict__ slotseg,
@@ -477,13 +488,13 @@ __global__ __launch_bounds__(256) void k_m2l_mixed(M2LLevels lv, const int32_t* 
         for (int ct = 0; ct < COLS; ++ct) st.acc[t][ct] = f4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
     for (int ct = 0; ct < COLS; ++ct) st.comp[ct] = f4{0.f, 0.f, 0.f, 0.f};
-    mix_segment<P, COLS, 1>(st, seg[0], seg[1], s_slot, s_off, lslots, tgt, nsrc, Mbl, Mbz, plane, opm, fsm, Bs,
+    mix_segment<P, COLS, 1, NPL>(st, seg[0], seg[1], s_slot, s_off, lslots, tgt, nsrc, Mbl, Mbz, plane, opm, fsm, Bs,
                             LD, CR, wave, lane, col, kq, kb0, kb1);
-    mix_segment<P, COLS, 2>(st, seg[1], seg[2], s_slot, s_off, lslots, tgt, nsrc, Mbl, Mbz, plane, opm, fsm, Bs,
+    mix_segment<P, COLS, 2, NPL>(st, seg[1], seg[2], s_slot, s_off, lslots, tgt, nsrc, Mbl, Mbz, plane, opm, fsm, Bs,
                             LD, CR, wave, lane, col, kq, kb0, kb1);
-    mix_segment<P, COLS, 3>(st, seg[2], seg[3], s_slot, s_off, lslots, tgt, nsrc, Mbl, Mbz, plane, opm, fsm, Bs,
+    mix_segment<P, COLS, 3, NPL>(st, seg[2], seg[3], s_slot, s_off, lslots, tgt, nsrc, Mbl, Mbz, plane, opm, fsm, Bs,
                             LD, CR, wave, lane, col, kq, kb0, kb1);
-    mix_segment<P, COLS, 4>(st, seg[3], seg[4], s_slot, s_off, lslots, tgt, nsrc, Mbl, Mbz, plane, opm, fsm, Bs,
+    mix_segment<P, COLS, 4, NPL>(st, seg[3], seg[4], s_slot, s_off, lslots, tgt, nsrc, Mbl, Mbz, plane, opm, fsm, Bs,
                             LD, CR, wave, lane, col, kq, kb0, kb1);
     const double scale = lv.scale[li];
 #pragma unroll
@@ -528,7 +539,7 @@ struct WaveMixState {
     f4 comp[NC][CT];
 };
 
-template <int P, int CT, int KT>
+template <int P, int CT, int KT, int NPL>
 __device__ __forceinline__ void wave_segment(WaveMixState<P, CT>& st, int i0, int i1, const int16_t* s_slot,
                                              const int64_t* s_off, const int32_t* lslots, const int (&tgt)[CT],
                                              int (&nsrc)[CT], const uint16_t* __restrict__ Mbl,
@@ -575,8 +586,8 @@ __device__ __forceinline__ void wave_segment(WaveMixState<P, CT>& st, int i0, in
                 load_vfrag<KP16>(a, VT, LD, rt, col, kq, kb);
 #pragma unroll
                 for (int ct = 0; ct < CT; ++ct)
-                    c1[rt][ct] = mma6(a[0], a[1], a[2], b3[kb & 1][0][ct], b3[kb & 1][1][ct], b3[kb & 1][2][ct],
-                                      c1[rt][ct]);
+                    c1[rt][ct] = mmaN<NPL>(a[0], a[1], a[2], b3[kb & 1][0][ct], b3[kb & 1][1][ct],
+                                           b3[kb & 1][2][ct], c1[rt][ct]);
             }
         }
         // GEMM2's B fragments straight from the accumulators, split in three
@@ -610,9 +621,9 @@ __device__ __forceinline__ void wave_segment(WaveMixState<P, CT>& st, int i0, in
 #pragma unroll
                 for (int ct = 0; ct < CT; ++ct) {
                     if (t < NC)
-                        fresh[t][ct] = mma6(ah, am, al, bh[sb][ct], bm[sb][ct], bl[sb][ct], fresh[t][ct]);
+                        fresh[t][ct] = mmaN<NPL>(ah, am, al, bh[sb][ct], bm[sb][ct], bl[sb][ct], fresh[t][ct]);
                     else
-                        st.acc[t][ct] = mma6(ah, am, al, bh[sb][ct], bm[sb][ct], bl[sb][ct], st.acc[t][ct]);
+                        st.acc[t][ct] = mmaN<NPL>(ah, am, al, bh[sb][ct], bm[sb][ct], bl[sb][ct], st.acc[t][ct]);
                 }
             }
         }
@@ -631,7 +642,7 @@ __device__ __forceinline__ void wave_segment(WaveMixState<P, CT>& st, int i0, in
     }
 }
 
-template <int P, int CT>
+template <int P, int CT, int NPL>
 __global__ __launch_bounds__(256) void k_m2l_mixed_wave(M2LLevels lv, const int32_t* __restrict__ slots,
                                                        const int32_t* __restrict__ order,
                                                        const int32_t* __restrict__ pcount,
@@ -702,10 +713,10 @@ __global__ __launch_bounds__(256) void k_m2l_mixed_wave(M2LLevels lv, const int3
     for (int t = 0; t < NC; ++t)
 #pragma unroll
         for (int ct = 0; ct < CT; ++ct) st.comp[t][ct] = f4{0.f, 0.f, 0.f, 0.f};
-    wave_segment<P, CT, 1>(st, seg[0], seg[1], s_slot, s_off, lslots, tgt, nsrc, Mbl, Mbz, plane, opm, col, kq);
-    wave_segment<P, CT, 2>(st, seg[1], seg[2], s_slot, s_off, lslots, tgt, nsrc, Mbl, Mbz, plane, opm, col, kq);
-    wave_segment<P, CT, 3>(st, seg[2], seg[3], s_slot, s_off, lslots, tgt, nsrc, Mbl, Mbz, plane, opm, col, kq);
-    wave_segment<P, CT, 4>(st, seg[3], seg[4], s_slot, s_off, lslots, tgt, nsrc, Mbl, Mbz, plane, opm, col, kq);
+    wave_segment<P, CT, 1, NPL>(st, seg[0], seg[1], s_slot, s_off, lslots, tgt, nsrc, Mbl, Mbz, plane, opm, col, kq);
+    wave_segment<P, CT, 2, NPL>(st, seg[1], seg[2], s_slot, s_off, lslots, tgt, nsrc, Mbl, Mbz, plane, opm, col, kq);
+    wave_segment<P, CT, 3, NPL>(st, seg[2], seg[3], s_slot, s_off, lslots, tgt, nsrc, Mbl, Mbz, plane, opm, col, kq);
+    wave_segment<P, CT, 4, NPL>(st, seg[3], seg[4], s_slot, s_off, lslots, tgt, nsrc, Mbl, Mbz, plane, opm, col, kq);
     const double scale = lv.scale[li];
 #pragma unroll
     for (int ct = 0; ct < CT; ++ct) {
@@ -940,7 +951,7 @@ __global__ void k_mfma_probe_bf16(float* out) {
 
 inline unsigned blocks_of(int64_t n) { return static_cast<unsigned>((n + 255) / 256); }
 
-template <int P, int COLS>
+template <int P, int COLS, int NPL>
 int m2l_mixed_launch(Context* c) {
     int tiles = 0;
     const M2LLevels lv = make_levels(c, 16 * COLS, tiles);
@@ -948,9 +959,9 @@ int m2l_mixed_launch(Context* c) {
     const int CR = ((c->kpad_max + 31) & ~31) + 4;
     const size_t shmem = sizeof(float) * kM2LWaves * static_cast<size_t>(CR) * 16 * COLS + 16 * COLS * 2 * 3 * 64;
     static int lds_limit[64] = {0};
-    HFMM_TRY(raise_lds_limit(c, reinterpret_cast<const void*>(&k_m2l_mixed<P, COLS>), shmem, lds_limit));
+    HFMM_TRY(raise_lds_limit(c, reinterpret_cast<const void*>(&k_m2l_mixed<P, COLS, NPL>), shmem, lds_limit));
     const int64_t plane = (c->total_cells + 1) * static_cast<int64_t>(c->LD);
-    hipLaunchKernelGGL((k_m2l_mixed<P, COLS>), dim3(static_cast<unsigned>(tiles)), dim3(64 * kM2LWaves), shmem,
+    hipLaunchKernelGGL((k_m2l_mixed<P, COLS, NPL>), dim3(static_cast<unsigned>(tiles)), dim3(64 * kM2LWaves), shmem,
                        c->stream, lv, c->d_m2l.as<int32_t>(), c->d_order.as<int32_t>(), c->d_pcount.as<int32_t>(),
                        c->d_slotord.as<int16_t>(), c->d_slotseg.as<int32_t>(), c->d_posoff.as<int64_t>(),
                        c->d_opm.as<uint16_t>(), c->d_Mb.as<uint16_t>(), plane, c->total_cells, c->d_L.as<double>(),
@@ -958,14 +969,14 @@ int m2l_mixed_launch(Context* c) {
     return check_launch(c, "m2l_mixed");
 }
 
-template <int P, int CT>
+template <int P, int CT, int NPL>
 int m2l_mixed_wave_launch(Context* c) {
     if (c->LD != WaveMixState<P, CT>::LDC) return fail(c, HFMM_ERR_ARG, "wave-level M2L: padded node count");
     int tiles = 0;
     const M2LLevels lv = make_levels(c, 16 * CT * kM2LWaves, tiles);
     if (tiles == 0) return HFMM_OK;  // no M2L level in the working range
     const int64_t plane = (c->total_cells + 1) * static_cast<int64_t>(c->LD);
-    hipLaunchKernelGGL((k_m2l_mixed_wave<P, CT>), dim3(static_cast<unsigned>(tiles)), dim3(64 * kM2LWaves), 0,
+    hipLaunchKernelGGL((k_m2l_mixed_wave<P, CT, NPL>), dim3(static_cast<unsigned>(tiles)), dim3(64 * kM2LWaves), 0,
                        c->stream, lv, c->d_m2l.as<int32_t>(), c->d_order.as<int32_t>(), c->d_pcount.as<int32_t>(),
                        c->d_slotord.as<int16_t>(), c->d_slotseg.as<int32_t>(), c->d_posoff.as<int64_t>(),
                        c->d_opm2.as<uint16_t>(), c->d_Mb.as<uint16_t>(), plane, c->total_cells, c->d_L.as<double>());
@@ -978,16 +989,16 @@ int m2l_mixed_wave_launch(Context* c) {
 //   2  two 16-target tiles per wave (p = 5)
 //   3  four waves share 32 targets (every order): half the registers and LDS
 //      of shape 0 for twice its operator reads per target
-template <int P>
+template <int P, int NPL>
 int m2l_mixed_variant(Context* c, int v) {
     switch (v) {
-        case 0: return m2l_mixed_launch<P, 4>(c);
-        case 3: return m2l_mixed_launch<P, 2>(c);
+        case 0: return m2l_mixed_launch<P, 4, NPL>(c);
+        case 3: return m2l_mixed_launch<P, 2, NPL>(c);
         case 1:
-            if constexpr (P <= kWaveM2LMaxOrder) return m2l_mixed_wave_launch<P, 1>(c);
+            if constexpr (P <= kWaveM2LMaxOrder) return m2l_mixed_wave_launch<P, 1, NPL>(c);
             break;
         case 2:
-            if constexpr (P <= 5) return m2l_mixed_wave_launch<P, 2>(c);
+            if constexpr (P <= 5) return m2l_mixed_wave_launch<P, 2, NPL>(c);
             break;
         default: break;
     }
@@ -1002,7 +1013,7 @@ inline bool mixed_variant_ok(int p, int v) {
 // Times every available variant once on the current tree (each writes every
 // local of the M2L levels, so the runs are interchangeable) and keeps the
 // fastest; the choice holds for the context's later evaluations.
-template <int P>
+template <int P, int NPL>
 int m2l_mixed_autotune(Context* c) {
     hipEvent_t ev[2 * kMixedVariants];
     for (auto& e : ev) HFMM_HIP(c, hipEventCreate(&e));
@@ -1012,7 +1023,7 @@ int m2l_mixed_autotune(Context* c) {
     for (int v = 0; v < kMixedVariants && st == HFMM_OK; ++v) {
         if (!mixed_variant_ok(P, v)) continue;
         (void)hipEventRecord(ev[2 * v], c->stream);
-        st = m2l_mixed_variant<P>(c, v);
+        st = m2l_mixed_variant<P, NPL>(c, v);
         (void)hipEventRecord(ev[2 * v + 1], c->stream);
         if (st != HFMM_OK) break;
         if (hipEventSynchronize(ev[2 * v + 1]) != hipSuccess) {
@@ -1033,17 +1044,17 @@ int m2l_mixed_autotune(Context* c) {
     return HFMM_OK;
 }
 
-template <int P>
+template <int P, int NPL>
 int m2l_mixed_impl(Context* c) {
     if (c->m2l_variant < 0) {
-        HFMM_TRY(m2l_mixed_autotune<P>(c));
+        HFMM_TRY((m2l_mixed_autotune<P, NPL>(c)));
     } else if (!mixed_variant_ok(P, c->m2l_variant)) {
         return fail(c, HFMM_ERR_ARG, "mixed M2L variant not available at this order");
     }
     // (the autotuned run already wrote the locals; the chosen variant runs again
     // so that the profile and the results come from one launch of it)
     prof_begin(c, "m2l_mixed");
-    HFMM_TRY(m2l_mixed_variant<P>(c, c->m2l_variant));
+    HFMM_TRY((m2l_mixed_variant<P, NPL>(c, c->m2l_variant)));
     prof_end(c);
     // the M2L targets: every cell of levels top..h-1, contiguous in the cell arrays
     const int64_t first = c->lvl_off[c->top()], ncells = c->lvl_off[c->height()] - first;
@@ -1124,18 +1135,21 @@ int run_split_multipoles(Context* c) {
 
 int run_m2l_mixed(Context* c) {
     if (c->kpad_max > 64) return fail(c, HFMM_ERR_ARG, "operator rank above 64 is not supported");
+    // three-part splits (six products) or, in HFMM_PRECISION_MIXED_HILO, two-part
+    // hi/lo splits (three products)
+    const bool hilo = c->P.precision == HFMM_PRECISION_MIXED_HILO;
     switch (c->order()) {
-        case 5: return m2l_mixed_impl<5>(c);
-        case 6: return m2l_mixed_impl<6>(c);
-        case 7: return m2l_mixed_impl<7>(c);
-        case 8: return m2l_mixed_impl<8>(c);
+        case 5: return hilo ? m2l_mixed_impl<5, 2>(c) : m2l_mixed_impl<5, 3>(c);
+        case 6: return hilo ? m2l_mixed_impl<6, 2>(c) : m2l_mixed_impl<6, 3>(c);
+        case 7: return hilo ? m2l_mixed_impl<7, 2>(c) : m2l_mixed_impl<7, 3>(c);
+        case 8: return hilo ? m2l_mixed_impl<8, 2>(c) : m2l_mixed_impl<8, 3>(c);
         default: return fail(c, HFMM_ERR_ARG, "order not compiled (5..8)");
     }
 }
 
-int m2l_variant_count(Context* c) { return c->P.precision == HFMM_PRECISION_MIXED ? kMixedVariants : 1; }
+int m2l_variant_count(Context* c) { return is_mixed(c) ? kMixedVariants : 1; }
 bool m2l_variant_available(Context* c, int v) {
-    if (c->P.precision != HFMM_PRECISION_MIXED) return v == 0;
+    if (!is_mixed(c)) return v == 0;
     return v >= 0 && v < kMixedVariants && mixed_variant_ok(c->order(), v);
 }
 

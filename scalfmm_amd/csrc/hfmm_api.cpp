@@ -233,7 +233,9 @@ int create_context(const hfmm_params* params, hipStream_t shared_stream, hfmm_ct
     if (params->order < 5 || params->order > 8) return HFMM_ERR_ARG;
     if (params->tree_height < 3 || params->tree_height > 20) return HFMM_ERR_ARG;
     if (!(params->box_width > 0) || !std::isfinite(params->box_width)) return HFMM_ERR_ARG;
-    if (params->precision != HFMM_PRECISION_FP64 && params->precision != HFMM_PRECISION_MIXED) return HFMM_ERR_ARG;
+    if (params->precision != HFMM_PRECISION_FP64 && params->precision != HFMM_PRECISION_MIXED &&
+        params->precision != HFMM_PRECISION_MIXED_HILO)
+        return HFMM_ERR_ARG;
     hfmm_ctx* h = new (std::nothrow) hfmm_ctx();
     if (!h) return HFMM_ERR_NOMEM;
     Context* c = &h->c;
@@ -253,7 +255,7 @@ int create_context(const hfmm_params* params, hipStream_t shared_stream, hfmm_ct
     int st = hfmm::build_cheb_operators(params->order, params->epsilon, c->ops, err);
     if (st == HFMM_OK) st = hfmm::compress_operators_device(c);
     if (st == HFMM_OK) st = hfmm::upload_operators(c);
-    if (st == HFMM_OK && c->P.precision == HFMM_PRECISION_MIXED) st = hfmm::prepare_mixed_operators(c);
+    if (st == HFMM_OK && hfmm::is_mixed(c)) st = hfmm::prepare_mixed_operators(c);
     if (st != HFMM_OK) {
         hfmm::free_all(c);
         if (c->own_stream) (void)hipStreamDestroy(c->stream);
@@ -301,7 +303,7 @@ static int evaluate_passes(Context* c, uint32_t ops, size_t exp_bytes) {
         for (int l = std::min(L - 1, lo - 1); l >= up; --l)
             HFMM_TRY(accumulating(c, M + c->lvl_off[l] * LD, c->ncell[l] * LD, [&] { return hfmm::run_m2m(c, l); }));
     // transfer pass: M2L writes every local of its levels; the others start at zero
-    const bool mixed = c->P.precision == HFMM_PRECISION_MIXED;
+    const bool mixed = hfmm::is_mixed(c);
     const bool all_levels = up == c->top() && lo == c->height();
     if (ops & HFMM_M2L) {
         HFMM_TRY(accumulating(c, Lx, static_cast<int64_t>(exp_bytes / sizeof(double)), [&] {
@@ -729,7 +731,7 @@ int hfmm_load_operators(hfmm_ctx* h, const char* path) {
     c->ops.rank = rank;
     c->ops.K = std::move(K);
     HFMM_TRY(hfmm::upload_operators(c));
-    if (c->P.precision == HFMM_PRECISION_MIXED) HFMM_TRY(hfmm::prepare_mixed_operators(c));
+    if (hfmm::is_mixed(c)) HFMM_TRY(hfmm::prepare_mixed_operators(c));
     c->m2l_variant = -1;  // other operators: tune again
     c->evaluated = false;
     return HFMM_OK;

@@ -140,6 +140,11 @@ struct Context {
     int order() const { return P.order; }
 };
 
+// M2L and P2P in the mixed forms (three-part or two-part bfloat16 splits)
+inline bool is_mixed(const Context* c) {
+    return c->P.precision == HFMM_PRECISION_MIXED || c->P.precision == HFMM_PRECISION_MIXED_HILO;
+}
+
 // Error helpers ---------------------------------------------------------------
 int fail(Context* c, int code, const std::string& msg);
 #define HFMM_HIP(ctx, call)                                                              \

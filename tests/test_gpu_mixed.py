@@ -206,3 +206,50 @@ def test_mixed_m2l_autotune():
     assert all(np.array_equal(u, w) for u, w in zip(a, e.results()))
     with pytest.raises(Exception):
         _mixed(7, 7).set_m2l_variant(1)  # the wave-level shapes stop at order 6
+
+
+# HFMM_PRECISION_MIXED_HILO: two-part (hi/lo) bfloat16 splits in the M2L, three
+# partial products; each product carries about 2^-16 relative error, so the bars
+# are those of 16-bit operands: the far field within 2e-5 (potential) and 6e-5
+# (forces) of the FP64 oracle, the full FMM within 1e-5 / 3e-5.
+HILO_FAR = (2e-5, 6e-5)
+HILO_ALL = (1e-5, 3e-5)
+
+
+@pytest.mark.parametrize("order,h,n,variant", [(5, 5, 20000, 0), (5, 5, 20000, 1), (5, 5, 20000, 2),
+                                               (5, 5, 20000, 3), (6, 4, 8000, 1), (8, 4, 6000, 0),
+                                               (8, 4, 6000, 3)])
+def test_hilo_m2l_variants(oracle, order, h, n, variant):
+    from scalfmm_amd import PRECISION_MIXED_HILO, Engine, FFmmFarField, FFmmNearAndFarFields
+    from scalfmm_amd.distributions import uniform_cube
+    x, y, z, q = uniform_cube(n, 300 + order)
+    _say(f"hi/lo M2L variant {variant}: n={n} h={h} p={order}")
+    e = Engine(h, 1.0, (0.5, 0.5, 0.5), order, 0.0, 0, PRECISION_MIXED_HILO)
+    e.set_m2l_variant(variant)
+    e.set_particles(x, y, z, q)
+    e.build_tree()
+    for ops, tol in ((FFmmFarField, HILO_FAR), (FFmmNearAndFarFields, HILO_ALL)):
+        e.evaluate(ops)
+        ref = oracle.fmm(x, y, z, q, (0.5, 0.5, 0.5), 1.0, h, order, ops=ops)
+        _check(e.results(), ref, *tol)
+    e.close()
+
+
+def test_hilo_plummer_order5_vs_direct(oracle, plummer100k):
+    """Order 5 on clustered particles: the hi/lo form's error against a direct
+    sum is the FP64 form's (the interpolation error, ~1e-4) to within 1%."""
+    from scalfmm_amd import PRECISION_FP64, PRECISION_MIXED_HILO, Engine
+    x, y, z, q = plummer100k["xyzq"]
+    t = np.sort(np.random.default_rng(8).choice(len(x), 2000, replace=False))
+    errs = {}
+    for name, prec in (("fp64", PRECISION_FP64), ("hilo", PRECISION_MIXED_HILO)):
+        e = Engine(6, 1.0, (0.5, 0.5, 0.5), 5, 0.0, 0, prec)
+        e.set_particles(x, y, z, q)
+        e.build_tree()
+        e.evaluate()
+        pot = e.results()[0]
+        d = e.direct(t)[0]
+        errs[name] = _rel(pot[t], d)
+        e.close()
+    _say(f"  pot rel L2 vs direct: fp64 {errs['fp64']:.3e} hilo {errs['hilo']:.3e}")
+    assert errs["hilo"] <= 1.01 * errs["fp64"]
