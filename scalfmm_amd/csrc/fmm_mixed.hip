@@ -35,6 +35,14 @@ typedef float f4 __attribute__((ext_vector_type(4)));
 
 __device__ __forceinline__ bf8 ld8(const uint16_t* p) { return *reinterpret_cast<const bf8*>(p); }
 
+typedef __bf16 bf4 __attribute__((ext_vector_type(4)));
+// Elements 0..3 from memory (8 bytes), 4..7 zero.
+__device__ __forceinline__ bf8 ld4z(const uint16_t* p) {
+    const bf4 v = *reinterpret_cast<const bf4*>(p);
+    const bf4 z = {};
+    return __builtin_shufflevector(v, z, 0, 1, 2, 3, 4, 5, 6, 7);
+}
+
 __device__ __forceinline__ f4 mfma_bf16(bf8 a, bf8 b, f4 c) {
     return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
 }
@@ -392,8 +400,16 @@ __device__ __forceinline__ void mix_segment(MixState<P, COLS>& st, int i0, int i
                 // a wave with fewer real row tiles computes throw-away ones on row 0
                 const int row = 16 * (wave + W * t) + col;
                 const uint16_t* a = U + static_cast<int64_t>(row < LD ? row : 0) * KP32 + 32 * kb + 8 * kq;
-                const bf8 ah = ld8(a), am = ld8(a + static_cast<int64_t>(LD) * KP32),
-                          al = ld8(a + 2 * static_cast<int64_t>(LD) * KP32);
+                // U's rank columns 16 KT .. KP32 are zero padding: the lanes that
+                // hold only padding skip the load (half the bytes of an odd KT's
+                // last block)
+                const bool live = 32 * kb + 8 * kq < KP16;
+                bf8 ah = {}, am = {}, al = {};
+                if (live) {
+                    ah = ld8(a);
+                    am = ld8(a + static_cast<int64_t>(LD) * KP32);
+                    if constexpr (NPL == 3) al = ld8(a + 2 * static_cast<int64_t>(LD) * KP32);
+                }
 #pragma unroll
                 for (int ct = 0; ct < COLS; ++ct) {
                     if (t == 0)
@@ -616,8 +632,20 @@ __device__ __forceinline__ void wave_segment(WaveMixState<P, CT>& st, int i0, in
 #pragma unroll
             for (int sb = 0; sb < KB32; ++sb) {
                 const uint16_t* a = U + static_cast<int64_t>(16 * t + col) * KP32 + 32 * sb + 8 * kq;
-                const bf8 ah = ld8(a), am = ld8(a + static_cast<int64_t>(LD) * KP32),
-                          al = ld8(a + 2 * static_cast<int64_t>(LD) * KP32);
+                // with an odd KT the last block's rank tile 2 sb + 1 is zero padding,
+                // stored (interleaved) in elements 4..7 of every lane: load only the
+                // first half
+                constexpr bool HALF = (KT & 1) != 0;
+                bf8 ah, am, al = {};
+                if (HALF && sb == KB32 - 1) {
+                    ah = ld4z(a);
+                    am = ld4z(a + static_cast<int64_t>(LD) * KP32);
+                    if constexpr (NPL == 3) al = ld4z(a + 2 * static_cast<int64_t>(LD) * KP32);
+                } else {
+                    ah = ld8(a);
+                    am = ld8(a + static_cast<int64_t>(LD) * KP32);
+                    if constexpr (NPL == 3) al = ld8(a + 2 * static_cast<int64_t>(LD) * KP32);
+                }
 #pragma unroll
                 for (int ct = 0; ct < CT; ++ct) {
                     if (t < NC)
