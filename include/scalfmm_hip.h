@@ -167,6 +167,17 @@ int hfmm_evaluate_levels(hfmm_ctx* ctx, uint32_t ops, int32_t upper, int32_t low
 int hfmm_evaluate_particle_blocks(hfmm_ctx* ctx, int32_t nblocks, const void* const* blocks, double* const* attributes,
                                   int32_t nvals, uint32_t ops, int32_t upper, int32_t lower);
 
+/* Near field beside the far field (default off): a context that owns its
+ * stream runs the P2P on a second stream of its own, concurrently with P2M,
+ * M2M, M2L and L2L, and the L2P waits for both (the reference runs these
+ * passes as independent tasks too, FGroupTaskAlgorithm.hpp:82-111).  Results
+ * are the same bit for bit either way; contexts of a batch and profiled
+ * evaluations run one kernel at a time.  Off by default: on MI355X the
+ * persistent P2P waves leave no register room for the M2L waves beside them,
+ * so the passes do not overlap and the M2L autotuner, timing under the P2P,
+ * picks worse tile shapes.  No counterpart in the reference. */
+int hfmm_set_stream_overlap(hfmm_ctx* ctx, int32_t enable);
+
 /* The state of the cells (multipoles and locals) between evaluations.
  * HFMM_CELLS_FRESH (the default): every evaluation starts from cells in their
  * initial state, as after tree.forEachCell(resetToInitialState)

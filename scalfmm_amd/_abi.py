@@ -41,7 +41,7 @@ EXPORTS = (
     "hfmm_batch_get_results", "hfmm_batch_info", "hfmm_batch_tree", "hfmm_get_m2l_counts",
     "hfmm_set_m2l_variant", "hfmm_get_m2l_variant", "hfmm_evaluate_levels",
     "hfmm_fma_read", "hfmm_fma_write", "hfmm_save_operators", "hfmm_load_operators", "hfmm_operator_file_name",
-    "hfmm_evaluate_particle_blocks", "hfmm_set_cell_mode", "hfmm_reset_cells",
+    "hfmm_evaluate_particle_blocks", "hfmm_set_cell_mode", "hfmm_reset_cells", "hfmm_set_stream_overlap",
 )
 MAX_RHS = 64
 _NON_STATUS = ("hfmm_last_error", "hfmm_version", "hfmm_batch_last_error", "hfmm_batch_default_height")
@@ -135,6 +135,7 @@ def lib(path: str | None = None):
     L.hfmm_operator_file_name.argtypes = [i32, d, C.c_char_p, i32]
     L.hfmm_get_m2l_variant.argtypes = [vp, C.POINTER(i32), C.POINTER(C.c_float)]
     L.hfmm_set_cell_mode.argtypes = [vp, i32]
+    L.hfmm_set_stream_overlap.argtypes = [vp, i32]
     L.hfmm_reset_cells.argtypes = [vp]
     L.hfmm_evaluate_particle_blocks.argtypes = [vp, i32, C.POINTER(vp), C.POINTER(dp), i32, C.c_uint32, i32, i32]
     for name in EXPORTS:

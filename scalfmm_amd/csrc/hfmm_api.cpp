@@ -192,6 +192,14 @@ static int upload_operators(Context* c) {
     return HFMM_OK;
 }
 
+static void destroy_side(Context* c) {
+    if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
+    if (c->ev_join) (void)hipEventDestroy(c->ev_join);
+    if (c->side) (void)hipStreamDestroy(c->side);
+    c->ev_fork = c->ev_join = nullptr;
+    c->side = nullptr;
+}
+
 static void free_all(Context* c) {
     DevBuf* bufs[] = {&c->d_Troots, &c->d_child, &c->d_pidx, &c->d_pvec, &c->d_rank, &c->d_opoff, &c->d_ops,
                       &c->d_slotpos, &c->d_slotord, &c->d_slotseg, &c->d_posoff, &c->d_opm, &c->d_opm2, &c->d_in, &c->d_keys, &c->d_keys2, &c->d_vals, &c->d_vals2, &c->d_hist,
@@ -251,6 +259,16 @@ int create_context(const hfmm_params* params, hipStream_t shared_stream, hfmm_ct
         delete h;
         return HFMM_ERR_DEVICE;
     }
+    // the side stream of the near field (default priority, as the main one)
+    if (c->own_stream &&
+        (hipStreamCreate(&c->side) != hipSuccess ||
+         hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming) != hipSuccess ||
+         hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming) != hipSuccess)) {
+        hfmm::destroy_side(c);
+        (void)hipStreamDestroy(c->stream);
+        delete h;
+        return HFMM_ERR_DEVICE;
+    }
     std::string err;
     int st = hfmm::build_cheb_operators(params->order, params->epsilon, c->ops, err);
     if (st == HFMM_OK) st = hfmm::compress_operators_device(c);
@@ -258,6 +276,7 @@ int create_context(const hfmm_params* params, hipStream_t shared_stream, hfmm_ct
     if (st == HFMM_OK && hfmm::is_mixed(c)) st = hfmm::prepare_mixed_operators(c);
     if (st != HFMM_OK) {
         hfmm::free_all(c);
+        hfmm::destroy_side(c);
         if (c->own_stream) (void)hipStreamDestroy(c->stream);
         delete h;
         return st;
@@ -294,6 +313,21 @@ static int evaluate_passes(Context* c, uint32_t ops, size_t exp_bytes) {
     double* M = c->d_M.as<double>();
     double* Lx = c->d_L.as<double>();
     const bool keep = c->keep_cells;
+    const bool mixed = hfmm::is_mixed(c);
+    // The P2P reads only the sorted particles and the leaf lists and writes
+    // only the results (and its own local coordinates): it runs on the side
+    // stream from here, and the L2P below waits for it.  Not when profiling
+    // (per-kernel times need one kernel at a time).
+    const bool side = c->side && c->overlap && !c->profiling && (ops & HFMM_P2P);
+    if (side) {
+        HFMM_HIP(c, hipEventRecord(c->ev_fork, c->stream));
+        HFMM_HIP(c, hipStreamWaitEvent(c->side, c->ev_fork, 0));
+        std::swap(c->stream, c->side);
+        const int st = mixed ? hfmm::run_p2p_mixed(c) : hfmm::run_p2p(c);
+        std::swap(c->stream, c->side);
+        HFMM_TRY(st);
+        HFMM_HIP(c, hipEventRecord(c->ev_join, c->side));
+    }
     // upward pass
     if (ops & HFMM_P2M)
         HFMM_TRY(accumulating(c, M + c->lvl_off[L] * LD, c->ncell[L] * LD, [&] { return hfmm::run_p2m(c); }));
@@ -303,7 +337,6 @@ static int evaluate_passes(Context* c, uint32_t ops, size_t exp_bytes) {
         for (int l = std::min(L - 1, lo - 1); l >= up; --l)
             HFMM_TRY(accumulating(c, M + c->lvl_off[l] * LD, c->ncell[l] * LD, [&] { return hfmm::run_m2m(c, l); }));
     // transfer pass: M2L writes every local of its levels; the others start at zero
-    const bool mixed = hfmm::is_mixed(c);
     const bool all_levels = up == c->top() && lo == c->height();
     if (ops & HFMM_M2L) {
         HFMM_TRY(accumulating(c, Lx, static_cast<int64_t>(exp_bytes / sizeof(double)), [&] {
@@ -318,7 +351,8 @@ static int evaluate_passes(Context* c, uint32_t ops, size_t exp_bytes) {
     if (ops & HFMM_L2L)
         for (int l = up; l < lo - 1; ++l) HFMM_TRY(hfmm::run_l2l(c, l));
     // near field initialises the results, the merge adds the far field
-    if (ops & HFMM_P2P) HFMM_TRY(mixed ? hfmm::run_p2p_mixed(c) : hfmm::run_p2p(c));
+    if (side) HFMM_HIP(c, hipStreamWaitEvent(c->stream, c->ev_join, 0));
+    else if (ops & HFMM_P2P) HFMM_TRY(mixed ? hfmm::run_p2p_mixed(c) : hfmm::run_p2p(c));
     else HFMM_TRY(hfmm::zero_results(c));
     if (ops & HFMM_L2P) HFMM_TRY(hfmm::run_l2p(c));
     return HFMM_OK;
@@ -333,12 +367,14 @@ int hfmm_destroy(hfmm_ctx* h) {
     Context* c = &h->c;
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
+    if (c->side) (void)hipStreamSynchronize(c->side);
     for (auto& t : c->pending) {
         (void)hipEventDestroy(t.start);
         (void)hipEventDestroy(t.stop);
     }
     for (auto e : c->event_pool) (void)hipEventDestroy(e);
     hfmm::free_all(c);
+    hfmm::destroy_side(c);
     if (c->stream && c->own_stream) (void)hipStreamDestroy(c->stream);
     delete h;
     return HFMM_OK;
@@ -770,6 +806,12 @@ int hfmm_get_expansions(hfmm_ctx* h, int32_t level, double* M, double* L) {
             std::memcpy(out + static_cast<size_t>(i) * nn, tmp.data() + static_cast<size_t>(i) * c->LD,
                         sizeof(double) * nn);
     }
+    return HFMM_OK;
+}
+
+int hfmm_set_stream_overlap(hfmm_ctx* h, int32_t enable) {
+    CTX_CHECK(h);
+    h->c.overlap = enable != 0;
     return HFMM_OK;
 }
 

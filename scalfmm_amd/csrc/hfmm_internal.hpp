@@ -41,6 +41,12 @@ struct Context {
     int device = 0;
     hipStream_t stream = nullptr;
     bool own_stream = true;  // false when the stream belongs to a batch
+    // Near field beside the far field (off by default): with its own stream a
+    // context can run the P2P on `side` while P2M..L2L run on `stream` (L2P
+    // waits for both)
+    hipStream_t side = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    bool overlap = false;
     std::string err;
 
     // ---- operators (host + device copies)

@@ -166,6 +166,10 @@ class Engine:
         evaluations, as the reference's tree cells keep them)."""
         self._check(self._L.hfmm_set_cell_mode(self.h, int(mode)))
 
+    def set_stream_overlap(self, on=True):
+        """Run the P2P on a second stream beside P2M..L2L (default off)."""
+        self._check(self._L.hfmm_set_stream_overlap(self.h, 1 if on else 0))
+
     def reset_cells(self):
         """tree.forEachCell(resetToInitialState) for the kept cells."""
         self._check(self._L.hfmm_reset_cells(self.h))

@@ -140,3 +140,22 @@ def test_several_charge_vectors_keep_their_cells(parts):
             np.testing.assert_array_equal(got[k][v], ref)
         f.close()
     e.close()
+
+
+@pytest.mark.parametrize("precision", [0, 1, 2], ids=["fp64", "mixed", "hilo"])
+def test_stream_overlap_same_results(parts, precision):
+    """The P2P on the side stream beside the far field gives the results of
+    the one-stream order bit for bit (it writes the results first, the L2P
+    waits for it and adds)."""
+    got = []
+    for overlap in (False, True):
+        e = _engine(parts, precision, keep=False)
+        if precision:
+            e.set_m2l_variant(0)
+        e.set_stream_overlap(overlap)
+        e.evaluate(ALL)
+        e.evaluate(ALL)  # a second evaluation: the side stream's P2P of the first has finished
+        got.append(e.results())
+        e.close()
+    for a, b in zip(*got):
+        np.testing.assert_array_equal(a, b)
