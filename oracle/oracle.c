@@ -642,6 +642,154 @@ void orc_cheb_destroy(orc_cheb* c) {
     free(c->child); free(c->pvectors); free(c);
 }
 
+/* ======================================================================== */
+/* FChebKernel's M2L: one global SVD of the 316 far-field operators         */
+/* (Kernels/Chebyshev/FChebM2LHandler.hpp:216-315 ComputeAndCompress,       */
+/* :437-523 Compress)                                                      */
+/* ======================================================================== */
+
+struct orc_chebg {
+    orc_cheb* c;  /* roots, interpolators (P2M, M2M, L2L, L2P are the symmetric kernel's) */
+    int k;        /* rank */
+    double* U;    /* nn x k, column-major, un-weighted (target side, applyU) */
+    double* B;    /* nn x k, column-major, un-weighted (source side, applyB) */
+    double* C;    /* [343][k x k] column-major: C_t[i][j] at i + j k; zero for near positions */
+    double* S1;   /* singular values of K_col (source side) */
+    double* S2;   /* singular values of K_row (target side) */
+};
+
+/* Right singular vectors and singular values of a tall m x n matrix (column-
+ * major): Householder QR, then the one-sided Jacobi SVD of R.  Stands in for
+ * dgesvd on K_col and dgesvd "SO" on K_row (FChebM2LHandler.hpp:459-484); the
+ * right singular vectors of K_row^T are the left ones of K_row. */
+static int tall_right_svd(int m, int n, const double* A, double* S, double* V /* n x n, column r = vector r */) {
+    double* R = (double*)malloc(sizeof(double) * (size_t)n * n);
+    double* VT = (double*)malloc(sizeof(double) * (size_t)n * n);
+    if (!R || !VT) { free(R); free(VT); return ORC_ERR_NOMEM; }
+    int st = orc_qr(m, n, A, NULL, R);
+    if (st == ORC_OK) st = orc_svd(n, R, NULL, S, VT);
+    if (st == ORC_OK)
+        for (int r = 0; r < n; ++r)
+            for (int c = 0; c < n; ++c) V[(size_t)r * n + c] = VT[(size_t)c * n + r];
+    free(R); free(VT);
+    return st;
+}
+
+int orc_chebg_create(int order, double epsilon, orc_chebg** out) {
+    if (!out) return ORC_ERR_ARG;
+    orc_chebg* g = (orc_chebg*)calloc(1, sizeof(orc_chebg));
+    if (!g) return ORC_ERR_NOMEM;
+    int st = orc_cheb_create(order, epsilon, &g->c);
+    if (st) { free(g); return st; }
+    const orc_cheb* c = g->c;
+    const int p = c->p, nn = c->nn, nint = 316;
+    const double eps = c->eps;
+    const size_t m1 = (size_t)nint * nn;
+    double* X = (double*)malloc(sizeof(double) * 3 * nn);
+    double* Y = (double*)malloc(sizeof(double) * 3 * nn);
+    double* w = (double*)malloc(sizeof(double) * nn);
+    double* Kw = (double*)malloc(sizeof(double) * m1 * nn);  /* [t][n][m] = w_n w_m K(X_m, Y_n) */
+    double* T = (double*)malloc(sizeof(double) * m1 * nn);
+    double* V1 = (double*)malloc(sizeof(double) * (size_t)nn * nn);
+    double* V2 = (double*)malloc(sizeof(double) * (size_t)nn * nn);
+    g->S1 = (double*)malloc(sizeof(double) * nn);
+    g->S2 = (double*)malloc(sizeof(double) * nn);
+    if (!X || !Y || !w || !Kw || !T || !V1 || !V2 || !g->S1 || !g->S2) st = ORC_ERR_NOMEM;
+    if (st == ORC_OK) {
+        const double zero[3] = {0, 0, 0};
+        set_roots(c, zero, 2.0, X);
+        double w1[ORC_MAX_ORDER];
+        for (int o = 0; o < p; ++o) w1[o] = M_PI / p * sqrt(1. - c->roots[o] * c->roots[o]);
+        for (int n = 0; n < nn; ++n) w[n] = sqrt(w1[n % p] * w1[(n / p) % p] * w1[n / (p * p)]);
+        /* the 316 operators in the loop order i, j, k (FChebM2LHandler.hpp:236-254) */
+        size_t t = 0;
+        for (int i = -3; i <= 3; ++i)
+            for (int j = -3; j <= 3; ++j)
+                for (int kk = -3; kk <= 3; ++kk) {
+                    if (abs(i) <= 1 && abs(j) <= 1 && abs(kk) <= 1) continue;
+                    const double cy[3] = {2. * i, 2. * j, 2. * kk};
+                    set_roots(c, cy, 2.0, Y);
+                    for (int n = 0; n < nn; ++n)
+                        for (int m = 0; m < nn; ++m)
+                            Kw[(t * nn + n) * nn + m] = w[n] * w[m] * kernel_r(X + 3 * m, Y + 3 * n);
+                    ++t;
+                }
+        /* K_col: rows (t, m), column n (FChebM2LHandler.hpp:448-456) -> B */
+        for (size_t tt = 0; tt < (size_t)nint; ++tt)
+            for (int n = 0; n < nn; ++n)
+                for (int m = 0; m < nn; ++m) T[(size_t)n * m1 + tt * nn + m] = Kw[(tt * nn + n) * nn + m];
+        st = tall_right_svd((int)m1, nn, T, g->S1, V1);
+    }
+    if (st == ORC_OK) {
+        /* K_row^T: rows (t, n), column m -> U */
+        for (size_t tt = 0; tt < (size_t)nint; ++tt)
+            for (int n = 0; n < nn; ++n)
+                for (int m = 0; m < nn; ++m) T[(size_t)m * m1 + tt * nn + n] = Kw[(tt * nn + n) * nn + m];
+        st = tall_right_svd((int)m1, nn, T, g->S2, V2);
+    }
+    if (st == ORC_OK) {
+        const int k1 = orc_svd_rank(g->S1, nn, eps), k2 = orc_svd_rank(g->S2, nn, eps);
+        if (k1 <= 0 || k2 <= 0) st = ORC_ERR_NUMERIC;
+        g->k = k1 < k2 ? k1 : k2;
+    }
+    if (st == ORC_OK) {
+        const int k = g->k;
+        g->U = (double*)malloc(sizeof(double) * (size_t)nn * k);
+        g->B = (double*)malloc(sizeof(double) * (size_t)nn * k);
+        g->C = (double*)calloc((size_t)343 * k * k, sizeof(double));
+        double* G = (double*)malloc(sizeof(double) * (size_t)nn * k);
+        if (!g->U || !g->B || !g->C || !G) st = ORC_ERR_NOMEM;
+        if (st == ORC_OK) {
+            memcpy(g->U, V2, sizeof(double) * (size_t)nn * k);
+            memcpy(g->B, V1, sizeof(double) * (size_t)nn * k);
+            /* C_t = U^T A_t B with A_t[m][n] = Kw[t][n][m] (the weighted operators) */
+            size_t t = 0;
+            for (int i = -3; i <= 3; ++i)
+                for (int j = -3; j <= 3; ++j)
+                    for (int kk = -3; kk <= 3; ++kk) {
+                        if (abs(i) <= 1 && abs(j) <= 1 && abs(kk) <= 1) continue;
+                        const double* Kt = Kw + t * nn * nn;
+                        for (int n = 0; n < nn; ++n)
+                            for (int a = 0; a < k; ++a) G[(size_t)a * nn + n] = dot(nn, Kt + (size_t)n * nn, g->U + (size_t)a * nn);
+                        double* Ct = g->C + (size_t)((i + 3) * 49 + (j + 3) * 7 + (kk + 3)) * k * k;
+                        for (int b = 0; b < k; ++b)
+                            for (int a = 0; a < k; ++a) Ct[a + (size_t)b * k] = dot(nn, G + (size_t)a * nn, g->B + (size_t)b * nn);
+                        ++t;
+                    }
+            /* un-weighting (FChebM2LHandler.hpp:299-303) */
+            for (int a = 0; a < k; ++a)
+                for (int n = 0; n < nn; ++n) {
+                    g->U[(size_t)a * nn + n] /= w[n];
+                    g->B[(size_t)a * nn + n] /= w[n];
+                }
+        }
+        free(G);
+    }
+    free(X); free(Y); free(w); free(Kw); free(T); free(V1); free(V2);
+    if (st != ORC_OK) { orc_chebg_destroy(g); return st; }
+    *out = g;
+    return ORC_OK;
+}
+
+void orc_chebg_destroy(orc_chebg* g) {
+    if (!g) return;
+    orc_cheb_destroy(g->c);
+    free(g->U); free(g->B); free(g->C); free(g->S1); free(g->S2);
+    free(g);
+}
+
+int orc_chebg_rank(const orc_chebg* g) { return g->k; }
+const orc_cheb* orc_chebg_cheb(const orc_chebg* g) { return g->c; }
+
+void orc_chebg_get(const orc_chebg* g, double* U, double* B, double* C, double* S1, double* S2) {
+    const int nn = g->c->nn, k = g->k;
+    if (U) memcpy(U, g->U, sizeof(double) * (size_t)nn * k);
+    if (B) memcpy(B, g->B, sizeof(double) * (size_t)nn * k);
+    if (C) memcpy(C, g->C, sizeof(double) * (size_t)343 * k * k);
+    if (S1) memcpy(S1, g->S1, sizeof(double) * nn);
+    if (S2) memcpy(S2, g->S2, sizeof(double) * nn);
+}
+
 int orc_cheb_rank(const orc_cheb* c, int idx) { return (idx >= 0 && idx < 343) ? c->rank[idx] : 0; }
 
 int orc_cheb_get_K(const orc_cheb* c, int idx, double* out) {
@@ -1044,7 +1192,38 @@ struct orc_run {
     double *x, *y, *z, *q, *pot, *fx, *fy, *fz;
     orc_stats st;
     double center[3], width;
+    /* FChebKernel: compressed multipoles / locals (k per cell) beside M and L */
+    const orc_chebg* g;
+    double* Mt[21];
+    double* Lt[21];
 };
+
+/* y = B^T x (applyB, FChebM2LHandler.hpp:194-197: overwrite) */
+static void chebg_apply_b(const orc_chebg* g, const double* x, double* y) {
+    const int nn = g->c->nn;
+    for (int a = 0; a < g->k; ++a) y[a] = dot(nn, g->B + (size_t)a * nn, x);
+}
+
+/* x += U X (applyU, FChebM2LHandler.hpp:150-153: gemva adds) */
+static void chebg_apply_u(const orc_chebg* g, const double* X, double* x) {
+    const int nn = g->c->nn;
+    for (int n = 0; n < nn; ++n) {
+        double s = 0;
+        for (int a = 0; a < g->k; ++a) s += g->U[(size_t)a * nn + n] * X[a];
+        x[n] += s;
+    }
+}
+
+/* X += scale C_t Y (applyC, FChebM2LHandler.hpp:166-173) */
+static void chebg_apply_c(const orc_chebg* g, int pos, double scale, const double* Y, double* X) {
+    const int k = g->k;
+    const double* Ct = g->C + (size_t)pos * k * k;
+    for (int i = 0; i < k; ++i) {
+        double s = 0;
+        for (int j = 0; j < k; ++j) s += Ct[i + (size_t)j * k] * Y[j];
+        X[i] += scale * s;
+    }
+}
 
 /* P2P for one target leaf against the particles of another leaf, one-way.
  * FP2P::GenericFullMutual / GenericInner (Kernels/P2P/FP2P.hpp:103-269)
@@ -1093,6 +1272,7 @@ static void run_passes(struct orc_run* r, unsigned ops, const double center[3], 
                                    corner[2] + ((double)c3[2] + .5) * leafw};
             const int64_t s = t->lstart[i];
             orc_p2m(c, ctr, leafw, t->lcount[i], r->x + s, r->y + s, r->z + s, r->q + s, r->M[L] + (size_t)i * nn);
+            if (r->g) chebg_apply_b(r->g, r->M[L] + (size_t)i * nn, r->Mt[L] + (size_t)i * r->g->k);
         }
     }
     r->st.t_p2m += wall_seconds() - t0;
@@ -1109,6 +1289,7 @@ static void run_passes(struct orc_run* r, unsigned ops, const double center[3], 
                 while (lo < hi) { const int64_t mid = (lo + hi) / 2; if ((ck[mid] >> 3) < pk) lo = mid + 1; else hi = mid; }
                 for (int64_t j = lo; j < nch && (ck[j] >> 3) == pk; ++j)
                     orc_m2m(c, (int)(ck[j] & 7), r->M[l + 1] + (size_t)j * nn, r->M[l] + (size_t)i * nn);
+                if (r->g) chebg_apply_b(r->g, r->M[l] + (size_t)i * nn, r->Mt[l] + (size_t)i * r->g->k);
             }
         }
     }
@@ -1124,8 +1305,12 @@ static void run_passes(struct orc_run* r, unsigned ops, const double center[3], 
 #pragma omp parallel for schedule(dynamic, 16)
             for (int64_t i = 0; i < nc; ++i)
                 for (int64_t e = t->m2l_off[l][i]; e < t->m2l_off[l][i + 1]; ++e)
-                    orc_cheb_apply_m2l(c, t->m2l_pos[l][e], scale, r->M[l] + (size_t)t->m2l_src[l][e] * nn,
-                                       r->L[l] + (size_t)i * nn);
+                    if (r->g)
+                        chebg_apply_c(r->g, t->m2l_pos[l][e], scale, r->Mt[l] + (size_t)t->m2l_src[l][e] * r->g->k,
+                                      r->Lt[l] + (size_t)i * r->g->k);
+                    else
+                        orc_cheb_apply_m2l(c, t->m2l_pos[l][e], scale, r->M[l] + (size_t)t->m2l_src[l][e] * nn,
+                                           r->L[l] + (size_t)i * nn);
         }
     }
     r->st.t_m2l += wall_seconds() - t0;
@@ -1140,6 +1325,7 @@ static void run_passes(struct orc_run* r, unsigned ops, const double center[3], 
                 const int64_t pk = t->keys[l][i];
                 int64_t lo = 0, hi = nch;
                 while (lo < hi) { const int64_t mid = (lo + hi) / 2; if ((ck[mid] >> 3) < pk) lo = mid + 1; else hi = mid; }
+                if (r->g) chebg_apply_u(r->g, r->Lt[l] + (size_t)i * r->g->k, r->L[l] + (size_t)i * nn);
                 for (int64_t j = lo; j < nch && (ck[j] >> 3) == pk; ++j)
                     orc_l2l(c, (int)(ck[j] & 7), r->L[l] + (size_t)i * nn, r->L[l + 1] + (size_t)j * nn);
             }
@@ -1174,6 +1360,7 @@ static void run_passes(struct orc_run* r, unsigned ops, const double center[3], 
             const double ctr[3] = {corner[0] + ((double)c3[0] + .5) * leafw, corner[1] + ((double)c3[1] + .5) * leafw,
                                    corner[2] + ((double)c3[2] + .5) * leafw};
             const int64_t s = t->lstart[i];
+            if (r->g) chebg_apply_u(r->g, r->Lt[L] + (size_t)i * r->g->k, r->L[L] + (size_t)i * nn);
             orc_l2p(c, ctr, leafw, r->L[L] + (size_t)i * nn, t->lcount[i], r->x + s, r->y + s, r->z + s, r->q + s,
                     r->pot + s, r->fx + s, r->fy + s, r->fz + s);
         }
@@ -1213,7 +1400,7 @@ void orc_run_destroy(orc_run* r) {
     if (!r) return;
     orc_tree_destroy(r->t);
     free(r->x); free(r->y); free(r->z); free(r->q); free(r->pot); free(r->fx); free(r->fy); free(r->fz);
-    for (int l = 0; l < 21; ++l) { free(r->M[l]); free(r->L[l]); }
+    for (int l = 0; l < 21; ++l) { free(r->M[l]); free(r->L[l]); free(r->Mt[l]); free(r->Lt[l]); }
     free(r);
 }
 
@@ -1291,6 +1478,42 @@ int orc_fmm_levels(const orc_cheb* c, int64_t n, const double* x, const double* 
     struct orc_run* r = NULL;
     int st = run_setup(c, n, x, y, z, q, center, width, height, &r);
     if (st == ORC_OK) {
+        run_passes(r, ops, center, width, upper, lower);
+        for (int64_t i = 0; i < n; ++i) {
+            const int64_t o = r->t->perm[i];
+            pot[o] = r->pot[i]; fx[o] = r->fx[i]; fy[o] = r->fy[i]; fz[o] = r->fz[i];
+        }
+        r->st.t_total = wall_seconds() - t0;
+        if (stats) *stats = r->st;
+        orc_run_destroy(r);
+    }
+#ifdef _OPENMP
+    omp_set_num_threads(saved);
+#endif
+    return st;
+}
+
+int orc_fmm_global(const orc_chebg* g, int64_t n, const double* x, const double* y, const double* z,
+                   const double* q, const double center[3], double width, int height, unsigned ops, int upper,
+                   int lower, int nthreads, double* pot, double* fx, double* fy, double* fz, orc_stats* stats) {
+    if (!g || n < 0) return ORC_ERR_ARG;
+    if (upper < 2 || upper > lower || lower > height) return ORC_ERR_ARG;
+#ifdef _OPENMP
+    const int saved = omp_get_max_threads();
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+#else
+    (void)nthreads;
+#endif
+    const double t0 = wall_seconds();
+    struct orc_run* r = NULL;
+    int st = run_setup(g->c, n, x, y, z, q, center, width, height, &r);
+    if (st == ORC_OK) {
+        r->g = g;
+        for (int l = 2; l < height; ++l) {
+            const size_t cells = (size_t)(r->t->ncell[l] > 0 ? r->t->ncell[l] : 1);
+            r->Mt[l] = (double*)calloc(cells * g->k, sizeof(double));
+            r->Lt[l] = (double*)calloc(cells * g->k, sizeof(double));
+        }
         run_passes(r, ops, center, width, upper, lower);
         for (int64_t i = 0; i < n; ++i) {
             const int64_t o = r->t->perm[i];

@@ -160,6 +160,26 @@ int orc_run_execute(orc_run* r, unsigned ops, int upper, int lower, double* pot,
 /* forEachCell(resetToInitialState): zero multipoles and locals. */
 void orc_run_reset_cells(orc_run* r);
 
+/* FChebKernel's M2L operators (Kernels/Chebyshev/FChebM2LHandler.hpp:216-523):
+ * the 316 weighted far-field operators compressed by one SVD of K_col (source
+ * side, B) and one of K_row (target side, U) to the smaller of the two ranks
+ * given by FSvd::getRank; C_t = U^T K_t B per far position (343 slots, zero for
+ * the near ones), then U and B un-weighted.  U, B: nn x k column-major; C:
+ * [343][k x k] column-major; S1 / S2: the singular values of K_col / K_row. */
+typedef struct orc_chebg orc_chebg;
+int orc_chebg_create(int order, double epsilon, orc_chebg** out);
+void orc_chebg_destroy(orc_chebg* g);
+int orc_chebg_rank(const orc_chebg* g);
+const orc_cheb* orc_chebg_cheb(const orc_chebg* g);
+void orc_chebg_get(const orc_chebg* g, double* U, double* B, double* C, double* S1, double* S2);
+/* The FMM with FChebKernel (Kernels/Chebyshev/FChebKernel.hpp:93-200): P2M and
+ * M2M followed by the compression M~ = B^T M, M2L on the compressed
+ * expansions (L~ += 2/w C_t M~), L2L and L2P preceded by L += U L~; the working
+ * levels as orc_fmm_levels. */
+int orc_fmm_global(const orc_chebg* g, int64_t n, const double* x, const double* y, const double* z,
+                   const double* q, const double center[3], double width, int height, unsigned ops, int upper,
+                   int lower, int nthreads, double* pot, double* fx, double* fy, double* fz, orc_stats* stats);
+
 /* O(N^2) direct sum (Kernels/P2P/FP2PR.hpp:14-42 conventions):
  * pot_i = sum_{j!=i} q_j/r_ij ; f_i = q_i grad_i(pot_i) = -q_i sum q_j (x_i-x_j)/r^3.
  * Targets: all particles when targets==NULL (nt ignored), else the listed indices;

@@ -90,6 +90,12 @@ def lib():
     L.orc_run_tree.restype = vp
     L.orc_run_tree.argtypes = [vp]
     L.orc_run_expansions.argtypes = [vp, C.c_int, _f64p, _f64p]
+    L.orc_chebg_create.argtypes = [C.c_int, C.c_double, C.POINTER(vp)]
+    L.orc_chebg_destroy.argtypes = [vp]
+    L.orc_chebg_rank.argtypes = [vp]
+    L.orc_chebg_get.argtypes = [vp, _f64p, _f64p, _f64p, _f64p, _f64p]
+    L.orc_fmm_global.argtypes = [vp, C.c_int64, _f64p, _f64p, _f64p, _f64p, _f64p, C.c_double, C.c_int,
+                                 C.c_uint, C.c_int, C.c_int, C.c_int, _f64p, _f64p, _f64p, _f64p, C.POINTER(Stats)]
     L.orc_run_execute.argtypes = [vp, C.c_uint, C.c_int, C.c_int, _f64p, _f64p, _f64p, _f64p]
     L.orc_run_reset_cells.argtypes = [vp]
     L.orc_direct.argtypes = [C.c_int64, _f64p, _f64p, _f64p, _f64p, C.c_int64, vp, C.c_int,
@@ -347,6 +353,47 @@ class Run:
         M = np.zeros(max(nc, 1) * self.cheb.nnodes); L = np.zeros(max(nc, 1) * self.cheb.nnodes)
         lib().orc_run_expansions(self.h, level, M, L)
         return M[: nc * self.cheb.nnodes].reshape(nc, -1), L[: nc * self.cheb.nnodes].reshape(nc, -1)
+
+
+class ChebGlobal:
+    """FChebKernel's operators: one global SVD of the 316 far-field operators."""
+
+    def __init__(self, order, epsilon=0.0):
+        self.order = order
+        self.nnodes = order ** 3
+        h = C.c_void_p()
+        st = lib().orc_chebg_create(order, float(epsilon), C.byref(h))
+        if st:
+            raise RuntimeError(f"orc_chebg_create failed: {st}")
+        self.h = h
+        self.rank = int(lib().orc_chebg_rank(h))
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.orc_chebg_destroy(self.h)
+            self.h = None
+
+    def operators(self):
+        """U, B (nnodes x k), C (343 x k x k with C[pos][i, j]), S1, S2."""
+        nn, k = self.nnodes, self.rank
+        U, B = np.zeros(nn * k), np.zeros(nn * k)
+        Cm = np.zeros(343 * k * k)
+        S1, S2 = np.zeros(nn), np.zeros(nn)
+        lib().orc_chebg_get(self.h, U, B, Cm, S1, S2)
+        return (U.reshape(k, nn).T, B.reshape(k, nn).T, Cm.reshape(343, k, k).transpose(0, 2, 1), S1, S2)
+
+
+def fmm_global(g, x, y, z, q, center, width, height, ops=ALL, levels=None, nthreads=0):
+    """The FMM with FChebKernel's compressed M2L over operators `g` (ChebGlobal)."""
+    n = len(x)
+    pot, fx, fy, fz = (np.zeros(max(n, 1)) for _ in range(4))
+    st = Stats()
+    up, lo = levels if levels is not None else (2, height)
+    rc = lib().orc_fmm_global(g.h, n, _f(x), _f(y), _f(z), _f(q), _center(center), float(width), height, ops,
+                              int(up), int(lo), nthreads, pot, fx, fy, fz, C.byref(st))
+    if rc:
+        raise RuntimeError(f"orc_fmm_global failed: {rc}")
+    return pot[:n], fx[:n], fy[:n], fz[:n], st
 
 
 def direct(x, y, z, q, targets=None, nthreads=0):

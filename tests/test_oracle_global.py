@@ -1,0 +1,95 @@
+"""The oracle's restatement of FChebKernel's M2L (one global SVD of the 316
+far-field operators, Kernels/Chebyshev/FChebM2LHandler.hpp:216-523, used by
+Kernels/Chebyshev/FChebKernel.hpp:93-200), pinned without the reference:
+
+- the weighted operators are reproduced by U C_t B^T within the truncation
+  the rank rule allows (FSvd::getRank: Frobenius tail below eps, per side);
+- weighted U and B have orthonormal columns;
+- K_col and K_row^T hold the same rows up to order (the node set and the 316
+  offsets are symmetric under reflection), so their singular values agree;
+- the rank is getRank of those singular values;
+- the FMM against the fixture's direct sums, against the symmetric kernel
+  (P2P identical; far fields within the two compressions' errors) and the
+  working levels (2, h) bit for bit.
+"""
+import os
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+@pytest.fixture(scope="module")
+def g5(oracle):
+    return oracle.ChebGlobal(5)
+
+
+def _nodes(oracle, p):
+    r = np.asarray(oracle.roots(p))
+    ids = np.array([[n % p, (n // p) % p, n // (p * p)] for n in range(p ** 3)])
+    return r[ids]
+
+
+def _weights(oracle, p):
+    r = np.asarray(oracle.roots(p))
+    w1 = np.pi / p * np.sqrt(1 - r * r)
+    return np.array([np.sqrt(w1[n % p] * w1[(n // p) % p] * w1[n // (p * p)]) for n in range(p ** 3)])
+
+
+def test_global_operators_reconstruct(oracle, g5):
+    p, eps = 5, 1e-5
+    U, B, Cm, S1, S2 = g5.operators()
+    X = _nodes(oracle, p)
+    w = _weights(oracle, p)
+    err2 = tot2 = 0.0
+    for i in range(-3, 4):
+        for j in range(-3, 4):
+            for k in range(-3, 4):
+                pos = (i + 3) * 49 + (j + 3) * 7 + (k + 3)
+                if max(abs(i), abs(j), abs(k)) <= 1:
+                    assert not Cm[pos].any()
+                    continue
+                Y = X + 2.0 * np.array([i, j, k])
+                A = 1.0 / np.linalg.norm(X[:, None, :] - Y[None, :, :], axis=2)  # A[m][n]: target m, source n
+                Aw = w[:, None] * A * w[None, :]
+                Rw = w[:, None] * (U @ Cm[pos] @ B.T) * w[None, :]
+                err2 += np.sum((Rw - Aw) ** 2)
+                tot2 += np.sum(Aw ** 2)
+    rel = np.sqrt(err2 / tot2)
+    assert rel < 2 * eps, rel  # two projections, each with a tail below eps
+
+
+def test_global_operators_orthonormal_and_ranked(oracle, g5):
+    p, eps = 5, 1e-5
+    U, B, Cm, S1, S2 = g5.operators()
+    w = _weights(oracle, p)
+    k = g5.rank
+    for M in (U, B):
+        G = (w[:, None] * M).T @ (w[:, None] * M)
+        assert np.abs(G - np.eye(k)).max() < 1e-12
+    assert np.allclose(S1, S2, rtol=1e-10, atol=1e-12 * S1[0])
+    assert np.all(np.diff(S1) <= 0)
+    # FSvd::getRank: the smallest k whose tail (k .. nn-1) stays within eps^2 of the total
+    tail = np.cumsum((S1 ** 2)[::-1])[::-1]
+    expect = int(np.max(np.nonzero(tail > eps * eps * np.sum(S1 ** 2))[0])) + 1
+    assert k == expect
+
+
+def test_global_fmm_accuracy_and_symmetric_kernel(oracle, g5, fixture20k):
+    d = fixture20k["data"]
+    c, w = fixture20k["center"], fixture20k["width"]
+    x, y, z, q = d[:, 0], d[:, 1], d[:, 2], d[:, 3]
+    got = oracle.fmm_global(g5, x, y, z, q, c, w, 5)
+    assert oracle.accuracy(d[:, 4], got[0])["rel_l2"] < 2e-6  # measured 1.0e-6
+    for k in range(3):
+        assert oracle.accuracy(d[:, 5 + k], got[1 + k])["rel_l2"] < 2e-4  # measured 8.6e-5
+    near_g = oracle.fmm_global(g5, x, y, z, q, c, w, 5, ops=oracle.P2P)
+    near_s = oracle.fmm(x, y, z, q, c, w, 5, 5, ops=oracle.P2P)
+    assert all(np.array_equal(a, b) for a, b in zip(near_g[:4], near_s[:4]))
+    far = oracle.ALL & ~oracle.P2P
+    far_g = oracle.fmm_global(g5, x, y, z, q, c, w, 5, ops=far)
+    far_s = oracle.fmm(x, y, z, q, c, w, 5, 5, ops=far)
+    assert oracle.accuracy(far_s[0], far_g[0])["rel_l2"] < 5e-3
+    same = oracle.fmm_global(g5, x, y, z, q, c, w, 5, levels=(2, 5))
+    assert all(np.array_equal(a, b) for a, b in zip(same[:4], got[:4]))
