@@ -120,10 +120,13 @@ def cpu_baseline(dist, n, h, p):
                       f"{st.t_total:.2f} s, M2L {st.t_m2l:.2f} s, P2P {st.t_p2p:.2f} s, wall {wall:.2f} s"}
 
 
-def profiled_traffic(kernel, workload):
+def profiled_traffic(kernel, workload, form=None):
     """HBM bytes per launch of `kernel` from its committed rocprofv3 summary
-    (FETCH_SIZE and WRITE_SIZE scaled by a copy-kernel calibration), or None."""
-    path = os.path.join(ROOT, "profiles", f"{kernel}_{workload}.md")
+    (FETCH_SIZE and WRITE_SIZE scaled by a copy-kernel calibration), or None.
+    A summary of one precision form is named ..._<workload>-<form>.md."""
+    path = os.path.join(ROOT, "profiles", f"{kernel}_{workload}-{form}.md")
+    if not os.path.exists(path):
+        path = os.path.join(ROOT, "profiles", f"{kernel}_{workload}.md")
     if not os.path.exists(path):
         return None
     for line in open(path):
@@ -218,7 +221,7 @@ def roofline(engines, kernel_ms, launches, p, form, workload, nrhs=1):
     ms = kernel_ms[top]
     achieved = work / (ms / 1e3) / 1e12
     out = {"bound": bound, "achieved": achieved, "peak": peak, "unit": "TFLOP/s", "frac": achieved / peak,
-           "traffic": profiled_traffic(kname, workload), "kernel": kname, "kernel_ms_per_step": ms,
+           "traffic": profiled_traffic(kname, workload, form), "kernel": kname, "kernel_ms_per_step": ms,
            "launches_per_step": launches.get(top), "algorithmic_flops_per_launch": work / max(launches.get(top, 1), 1)}
     out.update(extra)
     return out
