@@ -167,6 +167,25 @@ int hfmm_evaluate_levels(hfmm_ctx* ctx, uint32_t ops, int32_t upper, int32_t low
 int hfmm_evaluate_particle_blocks(hfmm_ctx* ctx, int32_t nblocks, const void* const* blocks, double* const* attributes,
                                   int32_t nvals, uint32_t ops, int32_t upper, int32_t lower);
 
+/* M2L scheme.  HFMM_M2L_SYMMETRIC (the default) is FChebSymKernel's: 16
+ * canonical operators compressed one by one, applied through the 343
+ * symmetries.  HFMM_M2L_GLOBAL_SVD is FChebKernel's
+ * (Kernels/Chebyshev/FChebKernel.hpp:93-200, FChebM2LHandler.hpp:216-523):
+ * the 316 far-field operators share one pair of bases U, B of rank k from one
+ * SVD of all of them, multipoles are compressed (B^T M) after P2M and M2M,
+ * the M2L works on the compressed expansions (L~ += 2/w C_t M~) and U L~ is
+ * added to the locals before L2L and L2P.  Selecting it builds the operators
+ * (host, FP64: a Householder QR and a Jacobi SVD of two 316 p^3 x p^3
+ * matrices; seconds at order 6, about a minute at order 8).  FP64 precision
+ * only; with kept cells, one charge vector.  hfmm_get_global_operators
+ * returns the rank, U and B (p^3 x k, column-major), C ([343][k x k],
+ * column-major, zero at the near positions) and the singular values of K_col
+ * and K_row (p^3 each); any pointer may be NULL. */
+#define HFMM_M2L_SYMMETRIC 0
+#define HFMM_M2L_GLOBAL_SVD 1
+int hfmm_set_m2l_scheme(hfmm_ctx* ctx, int32_t scheme);
+int hfmm_get_global_operators(hfmm_ctx* ctx, int32_t* rank, double* U, double* B, double* C, double* S1, double* S2);
+
 /* Near field beside the far field (default off): a context that owns its
  * stream runs the P2P on a second stream of its own, concurrently with P2M,
  * M2M, M2L and L2L, and the L2P waits for both (the reference runs these

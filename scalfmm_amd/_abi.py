@@ -19,6 +19,9 @@ NEAR_FIELD = P2P
 FAR_FIELD = P2M | M2M | M2L | L2L | L2P
 NEAR_AND_FAR = NEAR_FIELD | FAR_FIELD
 
+M2L_SYMMETRIC = 0
+M2L_GLOBAL_SVD = 1
+
 CELLS_FRESH = 0
 CELLS_KEPT = 1
 
@@ -42,6 +45,7 @@ EXPORTS = (
     "hfmm_set_m2l_variant", "hfmm_get_m2l_variant", "hfmm_evaluate_levels",
     "hfmm_fma_read", "hfmm_fma_write", "hfmm_save_operators", "hfmm_load_operators", "hfmm_operator_file_name",
     "hfmm_evaluate_particle_blocks", "hfmm_set_cell_mode", "hfmm_reset_cells", "hfmm_set_stream_overlap",
+    "hfmm_set_m2l_scheme", "hfmm_get_global_operators",
 )
 MAX_RHS = 64
 _NON_STATUS = ("hfmm_last_error", "hfmm_version", "hfmm_batch_last_error", "hfmm_batch_default_height")
@@ -136,6 +140,8 @@ def lib(path: str | None = None):
     L.hfmm_get_m2l_variant.argtypes = [vp, C.POINTER(i32), C.POINTER(C.c_float)]
     L.hfmm_set_cell_mode.argtypes = [vp, i32]
     L.hfmm_set_stream_overlap.argtypes = [vp, i32]
+    L.hfmm_set_m2l_scheme.argtypes = [vp, i32]
+    L.hfmm_get_global_operators.argtypes = [vp, C.POINTER(i32), dp, dp, dp, dp, dp]
     L.hfmm_reset_cells.argtypes = [vp]
     L.hfmm_evaluate_particle_blocks.argtypes = [vp, i32, C.POINTER(vp), C.POINTER(dp), i32, C.c_uint32, i32, i32]
     for name in EXPORTS:

@@ -206,7 +206,7 @@ static void free_all(Context* c) {
                       &c->d_scan_part, &c->d_qin, &c->d_flags, &c->d_scan, &c->d_sorted, &c->d_perm, &c->d_pleaf,
                       &c->d_count, &c->d_cellkey, &c->d_cellkey_tmp, &c->d_childstart, &c->d_leafstart,
                       &c->d_p2p, &c->d_p2pitems, &c->d_m2l, &c->d_order, &c->d_pcount, &c->d_dense, &c->d_M, &c->d_L, &c->d_Mb, &c->d_loc, &c->d_res_sorted,
-                      &c->d_res, &c->d_field, &c->d_setoff, &c->d_keep, &c->d_kept};
+                      &c->d_res, &c->d_field, &c->d_setoff, &c->d_keep, &c->d_kept, &c->d_gBt, &c->d_gU, &c->d_gC, &c->d_Mt, &c->d_Lt};
     for (DevBuf* b : bufs) release(*b);
 }
 
@@ -328,17 +328,32 @@ static int evaluate_passes(Context* c, uint32_t ops, size_t exp_bytes) {
         HFMM_TRY(st);
         HFMM_HIP(c, hipEventRecord(c->ev_join, c->side));
     }
-    // upward pass
-    if (ops & HFMM_P2M)
+    // FChebKernel's scheme: compressed expansions beside M and L
+    const bool glob = c->m2l_scheme == HFMM_M2L_GLOBAL_SVD;
+    const size_t cexp_bytes = sizeof(double) * static_cast<size_t>(c->total_cells) * c->glob.k;
+    // upward pass (fresh cells: multipoles that no pass of this evaluation
+    // writes read as zero)
+    if (!keep && (ops & HFMM_P2M) && !(ops & HFMM_M2M)) HFMM_TRY(hfmm::zero_device(c, c->d_M.ptr, exp_bytes));
+    if (glob && !keep) HFMM_TRY(hfmm::zero_device(c, c->d_Mt.ptr, cexp_bytes));
+    if (ops & HFMM_P2M) {
         HFMM_TRY(accumulating(c, M + c->lvl_off[L] * LD, c->ncell[L] * LD, [&] { return hfmm::run_p2m(c); }));
-    else if (!keep)
+        if (glob) HFMM_TRY(hfmm::run_global_compress(c, L));
+    } else if (!keep)
         HFMM_TRY(hfmm::zero_device(c, c->d_M.ptr, exp_bytes));
     if (ops & HFMM_M2M)
-        for (int l = std::min(L - 1, lo - 1); l >= up; --l)
+        for (int l = std::min(L - 1, lo - 1); l >= up; --l) {
             HFMM_TRY(accumulating(c, M + c->lvl_off[l] * LD, c->ncell[l] * LD, [&] { return hfmm::run_m2m(c, l); }));
+            if (glob) HFMM_TRY(hfmm::run_global_compress(c, l));
+        }
     // transfer pass: M2L writes every local of its levels; the others start at zero
     const bool all_levels = up == c->top() && lo == c->height();
-    if (ops & HFMM_M2L) {
+    if (glob) {  // the compressed M2L adds into L~; L gets U L~ before L2L / L2P
+        if (!keep) {
+            HFMM_TRY(hfmm::zero_device(c, c->d_L.ptr, exp_bytes));
+            HFMM_TRY(hfmm::zero_device(c, c->d_Lt.ptr, cexp_bytes));
+        }
+        if (ops & HFMM_M2L) HFMM_TRY(hfmm::run_global_m2l(c));
+    } else if (ops & HFMM_M2L) {
         HFMM_TRY(accumulating(c, Lx, static_cast<int64_t>(exp_bytes / sizeof(double)), [&] {
             if (keep || !all_levels) HFMM_TRY(hfmm::zero_device(c, c->d_L.ptr, exp_bytes));
             if (!mixed) return hfmm::run_m2l(c);
@@ -349,12 +364,18 @@ static int evaluate_passes(Context* c, uint32_t ops, size_t exp_bytes) {
         HFMM_TRY(hfmm::zero_device(c, c->d_L.ptr, exp_bytes));
     // downward pass (L2L adds into the children)
     if (ops & HFMM_L2L)
-        for (int l = up; l < lo - 1; ++l) HFMM_TRY(hfmm::run_l2l(c, l));
+        for (int l = up; l < lo - 1; ++l) {
+            if (glob) HFMM_TRY(hfmm::run_global_decompress(c, l));
+            HFMM_TRY(hfmm::run_l2l(c, l));
+        }
     // near field initialises the results, the merge adds the far field
     if (side) HFMM_HIP(c, hipStreamWaitEvent(c->stream, c->ev_join, 0));
     else if (ops & HFMM_P2P) HFMM_TRY(mixed ? hfmm::run_p2p_mixed(c) : hfmm::run_p2p(c));
     else HFMM_TRY(hfmm::zero_results(c));
-    if (ops & HFMM_L2P) HFMM_TRY(hfmm::run_l2p(c));
+    if (ops & HFMM_L2P) {
+        if (glob) HFMM_TRY(hfmm::run_global_decompress(c, L));
+        HFMM_TRY(hfmm::run_l2p(c));
+    }
     return HFMM_OK;
 }
 
@@ -515,6 +536,18 @@ int hfmm_evaluate_levels(hfmm_ctx* h, uint32_t ops, int32_t upper, int32_t lower
     // d_kept + 2 r exp (M, then L) between evaluations
     const int64_t exp_words = static_cast<int64_t>(exp_bytes / sizeof(double));
     const bool kept_many = c->keep_cells && c->nrhs > 1;
+    const bool glob = c->m2l_scheme == HFMM_M2L_GLOBAL_SVD;
+    if (glob) {
+        const size_t cexp = sizeof(double) * static_cast<size_t>(c->total_cells) * c->glob.k;
+        HFMM_TRY(hfmm::ensure(c, c->d_Mt, cexp));
+        HFMM_TRY(hfmm::ensure(c, c->d_Lt, cexp));
+        if (c->keep_cells && c->nrhs > 1)
+            return hfmm::fail(c, HFMM_ERR_STATE, "kept cells of the global M2L scheme hold one charge vector");
+        if (c->keep_cells && c->cells_dirty) {
+            HFMM_TRY(hfmm::zero_device(c, c->d_Mt.ptr, cexp));
+            HFMM_TRY(hfmm::zero_device(c, c->d_Lt.ptr, cexp));
+        }
+    }
     if (c->keep_cells) {
         HFMM_TRY(hfmm::ensure(c, c->d_keep, exp_bytes));
         if (kept_many) HFMM_TRY(hfmm::ensure(c, c->d_kept, 2 * exp_bytes * c->nrhs));
@@ -806,6 +839,53 @@ int hfmm_get_expansions(hfmm_ctx* h, int32_t level, double* M, double* L) {
             std::memcpy(out + static_cast<size_t>(i) * nn, tmp.data() + static_cast<size_t>(i) * c->LD,
                         sizeof(double) * nn);
     }
+    return HFMM_OK;
+}
+
+int hfmm_set_m2l_scheme(hfmm_ctx* h, int32_t scheme) {
+    CTX_CHECK(h);
+    Context* c = &h->c;
+    if (scheme != HFMM_M2L_SYMMETRIC && scheme != HFMM_M2L_GLOBAL_SVD)
+        return hfmm::fail(c, HFMM_ERR_ARG, "unknown M2L scheme");
+    if (scheme == HFMM_M2L_GLOBAL_SVD) {
+        if (c->P.precision != HFMM_PRECISION_FP64)
+            return hfmm::fail(c, HFMM_ERR_ARG, "the global-SVD M2L scheme runs in FP64 only");
+        if (c->glob.k == 0) {
+            std::string err;
+            hfmm::GlobalOperators g;
+            const int st = hfmm::build_global_operators(c->ops, g, err);
+            if (st != HFMM_OK) return hfmm::fail(c, st, err);
+            if (g.k > 256) return hfmm::fail(c, HFMM_ERR_ARG, "global M2L rank above 256");
+            const int nn = c->ops.nn, k = g.k;
+            std::vector<double> bt(static_cast<size_t>(nn) * k);
+            for (int a = 0; a < k; ++a)
+                for (int n = 0; n < nn; ++n) bt[static_cast<size_t>(n) * k + a] = g.B[static_cast<size_t>(a) * nn + n];
+            HFMM_HIP(c, hipSetDevice(c->device));
+            HFMM_TRY(hfmm::ensure(c, c->d_gBt, sizeof(double) * bt.size()));
+            HFMM_TRY(hfmm::ensure(c, c->d_gU, sizeof(double) * g.U.size()));
+            HFMM_TRY(hfmm::ensure(c, c->d_gC, sizeof(double) * g.C.size()));
+            HFMM_HIP(c, hipMemcpy(c->d_gBt.ptr, bt.data(), sizeof(double) * bt.size(), hipMemcpyHostToDevice));
+            HFMM_HIP(c, hipMemcpy(c->d_gU.ptr, g.U.data(), sizeof(double) * g.U.size(), hipMemcpyHostToDevice));
+            HFMM_HIP(c, hipMemcpy(c->d_gC.ptr, g.C.data(), sizeof(double) * g.C.size(), hipMemcpyHostToDevice));
+            c->glob = std::move(g);
+        }
+    }
+    if (scheme != c->m2l_scheme) c->cells_dirty = true;
+    c->m2l_scheme = scheme;
+    c->evaluated = false;
+    return HFMM_OK;
+}
+
+int hfmm_get_global_operators(hfmm_ctx* h, int32_t* rank, double* U, double* B, double* C, double* S1, double* S2) {
+    CTX_CHECK(h);
+    Context* c = &h->c;
+    if (c->glob.k == 0) return hfmm::fail(c, HFMM_ERR_STATE, "the global-SVD M2L operators are not built");
+    if (rank) *rank = c->glob.k;
+    if (U) std::memcpy(U, c->glob.U.data(), sizeof(double) * c->glob.U.size());
+    if (B) std::memcpy(B, c->glob.B.data(), sizeof(double) * c->glob.B.size());
+    if (C) std::memcpy(C, c->glob.C.data(), sizeof(double) * c->glob.C.size());
+    if (S1) std::memcpy(S1, c->glob.S1.data(), sizeof(double) * c->glob.S1.size());
+    if (S2) std::memcpy(S2, c->glob.S2.data(), sizeof(double) * c->glob.S2.size());
     return HFMM_OK;
 }
 

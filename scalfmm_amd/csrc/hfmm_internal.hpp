@@ -36,6 +36,15 @@ struct KernelTimer {
     hipEvent_t start, stop;
 };
 
+// FChebKernel's M2L operators (one global SVD; cheb_global.cpp): U and B
+// nn x k column-major (un-weighted), C [343][k x k] column-major, S1 / S2 the
+// singular values of K_col / K_row.
+struct GlobalOperators {
+    int k = 0;
+    std::vector<double> U, B, C, S1, S2;
+};
+int build_global_operators(const ChebOperators& op, GlobalOperators& g, std::string& err);
+
 struct Context {
     hfmm_params P{};
     int device = 0;
@@ -129,6 +138,14 @@ struct Context {
     // expansions) or kept (the passes add into the expansions of earlier ones)
     bool keep_cells = false;
     bool cells_dirty = true;  // kept mode: zero M and L before the next evaluation
+    // M2L scheme: HFMM_M2L_SYMMETRIC (FChebSymKernel) or HFMM_M2L_GLOBAL_SVD
+    // (FChebKernel: compressed multipoles / locals of rank k beside M and L)
+    int m2l_scheme = 0;
+    GlobalOperators glob;
+    DevBuf d_gBt;             // B^T as [nn][k] (the compression reads a row per node)
+    DevBuf d_gU;              // U as [k][nn]
+    DevBuf d_gC;              // C [343][k x k]
+    DevBuf d_Mt, d_Lt;        // [total_cells][k]
     DevBuf d_keep;            // kept mode: snapshot of the expansions a pass overwrites
     DevBuf d_kept;            // kept mode, nrhs > 1: M and L of every charge vector
     std::vector<double> block_xyz;  // positions of the last particle-block evaluation (tree reuse)
@@ -145,6 +162,7 @@ struct Context {
     int top() const { return 2 + set_levels; }
     int order() const { return P.order; }
 };
+
 
 // M2L and P2P in the mixed forms (three-part or two-part bfloat16 splits)
 inline bool is_mixed(const Context* c) {
@@ -205,6 +223,12 @@ int zero_results(Context* c);
 int scatter_results(Context* c, int rhs = 0);
 // clears device memory with a fill kernel (capturable, unlike hipMemsetAsync)
 int zero_device(Context* c, void* ptr, size_t bytes);
+// FChebKernel passes (fmm_global.hip): M~ = B^T M for the cells of level l,
+// L += U L~ for the cells of level l, and the compressed M2L over the working
+// levels (L~ += 2/w C_t M~)
+int run_global_compress(Context* c, int l);
+int run_global_decompress(Context* c, int l);
+int run_global_m2l(Context* c);
 // dst[i] = src[i] (add = false) or dst[i] += src[i] (add = true), i < count
 int combine_device(Context* c, double* dst, const double* src, int64_t count, bool add);
 int selftest_mfma_f64(Context* c, double* out);

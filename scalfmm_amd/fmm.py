@@ -166,6 +166,24 @@ class Engine:
         evaluations, as the reference's tree cells keep them)."""
         self._check(self._L.hfmm_set_cell_mode(self.h, int(mode)))
 
+    def set_m2l_scheme(self, scheme):
+        """_abi.M2L_SYMMETRIC (FChebSymKernel) or _abi.M2L_GLOBAL_SVD (FChebKernel:
+        builds the global operators on first use)."""
+        self._check(self._L.hfmm_set_m2l_scheme(self.h, int(scheme)))
+
+    def global_operators(self):
+        """FChebKernel's operators: U, B (nnodes x k), C (343 x k x k with
+        C[pos][i, j]), and the singular values of K_col and K_row."""
+        k = C.c_int32()
+        self._check(self._L.hfmm_get_global_operators(self.h, C.byref(k), None, None, None, None, None))
+        k = k.value
+        nn = self.nnodes
+        U, B = np.empty(nn * k), np.empty(nn * k)
+        Cm = np.empty(343 * k * k)
+        S1, S2 = np.empty(nn), np.empty(nn)
+        self._check(self._L.hfmm_get_global_operators(self.h, None, _dp(U), _dp(B), _dp(Cm), _dp(S1), _dp(S2)))
+        return U.reshape(k, nn).T, B.reshape(k, nn).T, Cm.reshape(343, k, k).transpose(0, 2, 1), S1, S2
+
     def set_stream_overlap(self, on=True):
         """Run the P2P on a second stream beside P2M..L2L (default off)."""
         self._check(self._L.hfmm_set_stream_overlap(self.h, 1 if on else 0))
@@ -355,6 +373,16 @@ class FChebSymKernel:
         self.device = device
         self.precision = int(precision)
         self.engine = Engine(tree_height, box_width, box_center, order, self.epsilon, device, self.precision)
+
+
+class FChebKernel(FChebSymKernel):
+    """Kernel parameters of FChebKernel<double, ..., FInterpMatrixKernelR, ORDER>
+    (Src/Kernels/Chebyshev/FChebKernel.hpp:59-84): the same interpolation, the
+    M2L through one global SVD of the 316 far-field operators (FP64)."""
+
+    def __init__(self, tree_height, box_width, box_center, order, epsilon=None, device=0):
+        super().__init__(tree_height, box_width, box_center, order, epsilon, device, _abi.PRECISION_FP64)
+        self.engine.set_m2l_scheme(_abi.M2L_GLOBAL_SVD)
 
 
 class FGroupTree:
