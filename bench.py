@@ -49,7 +49,13 @@ WORKLOADS = {
     # 4,096 sets of log-uniform sizes in [1e3, 6.4e4], heights 3..5 by size,
     # 8 charge vectors; the CPU sample is the first 512 sets with one vector
     "batch4096-p5-r8": ("batch", 4096, None, 5, (512, None)),
+    # the reference's published kernel comparison (Data/ResultCompareKernels.txt:
+    # Examples/compareAllPoissonKernels.cpp on unitCubeRef20kDouble.bfma, order 7,
+    # height 5): kernel construction (operator set-up) plus one execution
+    "compare20k-h5-p7": ("compare", 20_000, 5, 7, None),
 }
+# the reference's published times of that comparison (4 threads, 3 GHz Core i7)
+PUBLISHED_COMPARE = {"symmetric": 1.50791, "global": 31.937}
 SEED = {"plummer": 3, "uniform": 2, "batch": 4}  # the configurations' documented seeds
 BATCH_RHS = 8
 
@@ -345,6 +351,72 @@ def run_batch(args):
     print(json.dumps(out))
 
 
+def run_compare(args):
+    """Kernel construction plus one FP64 execution on the reference's 20k
+    fixture, for FChebSymKernel (the symmetric M2L, operators compressed on
+    the device) and FChebKernel (one global SVD, operators built on the host),
+    timed end to end from the host as the reference's comparison times them."""
+    from scalfmm_amd import _abi
+    from scalfmm_amd.fma import FFmaGenericLoader
+    from scalfmm_amd.fmm import Engine
+    _, n, h, p, _ = WORKLOADS[args.workload]
+    f = FFmaGenericLoader(os.path.join(ROOT, "tests", "golden", "unitCubeRef20kDouble.bfma"))
+    d = f.data
+    x, y, z, q = (np.ascontiguousarray(d[:, i]) for i in range(4))
+    w, c = f.getBoxWidth(), f.getCenterOfBox()
+
+    def rel(a, b):
+        return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+    res = {}
+    for scheme in ("symmetric", "global"):
+        times = []
+        for it in range(args.warmup + args.steps):
+            print(f"[bench] {scheme} run {it}", file=sys.stderr, flush=True)
+            t0 = time.perf_counter()
+            e = Engine(h, w, c, p)
+            if scheme == "global":
+                e.set_m2l_scheme(_abi.M2L_GLOBAL_SVD)
+            e.set_particles(x, y, z, q)
+            e.build_tree()
+            e.evaluate()
+            got = e.results()
+            t1 = time.perf_counter()
+            if it >= args.warmup:
+                times.append(t1 - t0)
+            if it == args.warmup + args.steps - 1:
+                e.set_profiling(True)
+                e.evaluate()
+                kms = {k: v for k, v in e.kernel_times().items()}
+                e.set_profiling(False)
+            e.close()
+        sec = float(np.mean(times))
+        res[scheme] = {"seconds": sec, "published_seconds": PUBLISHED_COMPARE[scheme],
+                       "speedup_vs_published": PUBLISHED_COMPARE[scheme] / sec,
+                       "pot_rel_l2_vs_direct": rel(got[0], d[:, 4]),
+                       "force_rel_l2_vs_direct": [rel(got[1 + k], d[:, 5 + k]) for k in range(3)],
+                       "kernel_ms_one_evaluation": kms}
+    main_t = res["symmetric"]["seconds"]
+    out = {"metric": "s, FChebSymKernel construction + one execution, 20k uniform (unitCubeRef20kDouble), h5, p7",
+           "value": main_t, "unit": "s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+           "ms_per_step": main_t * 1e3, "higher_is_better": False, "scaling": "strong",
+           "vs_baseline": main_t / PUBLISHED_COMPARE["symmetric"], "dtype": "f64", "data": "reference fixture",
+           "config": {"workload": args.workload, "particles": n, "height": h, "order": p,
+                      "published": "Data/ResultCompareKernels.txt (4 threads, 3 GHz Core i7)"},
+           "schemes": res,
+           "roofline": None, "roofline_note": "the timed region is dominated by operator set-up, not by one kernel"}
+    if not args.no_cpu_baseline:
+        from oracle import pyoracle
+        cores = os.cpu_count() or 1
+        t0 = time.perf_counter()
+        cheb = pyoracle.Cheb(p)
+        st = pyoracle.fmm(x, y, z, q, c, w, h, p, cheb=cheb, nthreads=cores)[4]
+        t1 = time.perf_counter()
+        out["cpu_baseline"] = {"value": t1 - t0, "unit": "s", "cores": cores, "kind": "port",
+                               "sample": f"the oracle's FP64 OpenMP FMM with its operator set-up on the same 20k "
+                                         f"particles, h{h}, p{p} (execute {st.t_total:.3f} s)"}
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -370,6 +442,9 @@ def main():
 
     if WORKLOADS[args.workload][0] == "batch":
         run_batch(args)
+        return
+    if WORKLOADS[args.workload][0] == "compare":
+        run_compare(args)
         return
     dist, n, h, p, (cn, ch) = WORKLOADS[args.workload]
     print(f"[bench] generating {n} {dist} particles", file=sys.stderr, flush=True)

@@ -15,14 +15,33 @@
 // tall matrix is reduced by a Householder QR and its triangle by one-sided
 // Jacobi rotations, which gives the same singular values and vectors (up to
 // the signs of the vectors, which cancel in U C_t B^T).
+#include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <thread>
 #include <vector>
 
 #include "hfmm_internal.hpp"
 
 namespace hfmm {
 namespace {
+
+// Runs body(i) for i in [0, n) over the host's cores (each index on one thread;
+// the results do not depend on the split).
+template <class F>
+void parallel_for(int n, F body) {
+    const int nt = std::max(1, std::min<int>(n, static_cast<int>(std::thread::hardware_concurrency())));
+    if (nt <= 1 || n < 2) {
+        for (int i = 0; i < n; ++i) body(i);
+        return;
+    }
+    std::vector<std::thread> pool;
+    for (int t = 0; t < nt; ++t)
+        pool.emplace_back([&, t] {
+            for (int i = t; i < n; i += nt) body(i);
+        });
+    for (auto& th : pool) th.join();
+}
 
 // R of the Householder QR of an m x n column-major matrix (m >= n), in place.
 void householder_r(int64_t m, int n, std::vector<double>& A, std::vector<double>& R) {
@@ -37,14 +56,19 @@ void householder_r(int64_t m, int n, std::vector<double>& A, std::vector<double>
         for (int64_t i = j + 1; i < m; ++i) a[i] /= v0;  // v = (1, a[j+1..] / v0)
         const double tau = -v0 / beta;
         a[j] = beta;
-        for (int c = j + 1; c < n; ++c) {
-            double* b = A.data() + static_cast<size_t>(c) * m;
-            double d = b[j];
-            for (int64_t i = j + 1; i < m; ++i) d += a[i] * b[i];
-            d *= tau;
-            b[j] -= d;
-            for (int64_t i = j + 1; i < m; ++i) b[i] -= d * a[i];
-        }
+        // the trailing columns, each updated by one thread (blocks of 8 columns)
+        const int nb = (n - j - 1 + 7) / 8;
+        parallel_for(nb, [&](int blk) {
+            const int c0 = j + 1 + 8 * blk, c1 = std::min(n, c0 + 8);
+            for (int c = c0; c < c1; ++c) {
+                double* b = A.data() + static_cast<size_t>(c) * m;
+                double d = b[j];
+                for (int64_t i = j + 1; i < m; ++i) d += a[i] * b[i];
+                d *= tau;
+                b[j] -= d;
+                for (int64_t i = j + 1; i < m; ++i) b[i] -= d * a[i];
+            }
+        });
     }
     R.assign(static_cast<size_t>(n) * n, 0.);
     for (int c = 0; c < n; ++c)
@@ -187,8 +211,8 @@ int build_global_operators(const ChebOperators& op, GlobalOperators& g, std::str
     g.U.assign(V2.begin(), V2.begin() + static_cast<size_t>(nn) * k);
     g.B.assign(V1.begin(), V1.begin() + static_cast<size_t>(nn) * k);
     g.C.assign(static_cast<size_t>(343) * k * k, 0.);
-    std::vector<double> G(static_cast<size_t>(nn) * k);
-    for (int t = 0; t < nint; ++t) {
+    parallel_for(nint, [&](int t) {
+        std::vector<double> G(static_cast<size_t>(nn) * k);
         const double* Kt = Kw.data() + static_cast<size_t>(t) * nn * nn;
         for (int a = 0; a < k; ++a)  // G[:, a] = K_t^T-as-stored U[:, a]: sum over targets m
             for (int n = 0; n < nn; ++n) {
@@ -203,7 +227,7 @@ int build_global_operators(const ChebOperators& op, GlobalOperators& g, std::str
                 for (int n = 0; n < nn; ++n) s += G[static_cast<size_t>(a) * nn + n] * g.B[static_cast<size_t>(b) * nn + n];
                 Ct[a + static_cast<size_t>(b) * k] = s;
             }
-    }
+    });
     for (int a = 0; a < k; ++a)
         for (int n = 0; n < nn; ++n) {
             g.U[static_cast<size_t>(a) * nn + n] /= w[n];
