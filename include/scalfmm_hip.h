@@ -180,9 +180,13 @@ int hfmm_evaluate_particle_blocks(hfmm_ctx* ctx, int32_t nblocks, const void* co
  * only; with kept cells, one charge vector.  hfmm_get_global_operators
  * returns the rank, U and B (p^3 x k, column-major), C ([343][k x k],
  * column-major, zero at the near positions) and the singular values of K_col
- * and K_row (p^3 each); any pointer may be NULL. */
+ * and K_row (p^3 each; zero for the dense scheme) of the scheme selected
+ * last; any pointer may be NULL. */
 #define HFMM_M2L_SYMMETRIC 0
 #define HFMM_M2L_GLOBAL_SVD 1
+#define HFMM_M2L_DENSE 2 /* FChebDenseKernel (FChebDenseKernel.hpp:93-165,
+                            FChebDenseM2LHandler.hpp:180-215): the 316 operators
+                            uncompressed (rank p^3, identity bases) */
 int hfmm_set_m2l_scheme(hfmm_ctx* ctx, int32_t scheme);
 int hfmm_get_global_operators(hfmm_ctx* ctx, int32_t* rank, double* U, double* B, double* C, double* S1, double* S2);
 

@@ -358,6 +358,7 @@ int orc_svd_rank(const double* S, int n, double eps) {
 
 struct orc_cheb {
     int p, nn;
+    int basis;  /* 0: Chebyshev (FChebInterpolator), 1: Lagrange on equispaced roots (FUnifInterpolator) */
     double eps;
     double roots[ORC_MAX_ORDER];
     double Troots[ORC_MAX_ORDER][ORC_MAX_ORDER]; /* [o][j] = T_o(root_j), o>=1 */
@@ -771,6 +772,127 @@ int orc_chebg_create(int order, double epsilon, orc_chebg** out) {
     return ORC_OK;
 }
 
+/* FChebDenseKernel's operators (Kernels/Chebyshev/FChebDenseM2LHandler.hpp:180-215):
+ * K_t[m][n] = K(X_m, Y_n) uncompressed, as a "global" set of rank p^3 with
+ * identity bases, so the same passes apply them. */
+int orc_chebd_create(int order, orc_chebg** out) {
+    if (!out) return ORC_ERR_ARG;
+    orc_chebg* g = (orc_chebg*)calloc(1, sizeof(orc_chebg));
+    if (!g) return ORC_ERR_NOMEM;
+    int st = orc_cheb_create(order, 0.0, &g->c);
+    if (st) { free(g); return st; }
+    const orc_cheb* c = g->c;
+    const int nn = c->nn;
+    g->k = nn;
+    g->U = (double*)calloc((size_t)nn * nn, sizeof(double));
+    g->B = (double*)calloc((size_t)nn * nn, sizeof(double));
+    g->C = (double*)calloc((size_t)343 * nn * nn, sizeof(double));
+    g->S1 = (double*)calloc((size_t)nn, sizeof(double));
+    g->S2 = (double*)calloc((size_t)nn, sizeof(double));
+    double* X = (double*)malloc(sizeof(double) * 3 * nn);
+    double* Y = (double*)malloc(sizeof(double) * 3 * nn);
+    if (!g->U || !g->B || !g->C || !g->S1 || !g->S2 || !X || !Y) {
+        free(X); free(Y); orc_chebg_destroy(g); return ORC_ERR_NOMEM;
+    }
+    for (int n = 0; n < nn; ++n) g->U[(size_t)n * nn + n] = g->B[(size_t)n * nn + n] = 1.0;
+    const double zero[3] = {0, 0, 0};
+    set_roots(c, zero, 2.0, X);
+    for (int i = -3; i <= 3; ++i)
+        for (int j = -3; j <= 3; ++j)
+            for (int kk = -3; kk <= 3; ++kk) {
+                if (abs(i) <= 1 && abs(j) <= 1 && abs(kk) <= 1) continue;
+                const double cy[3] = {2. * i, 2. * j, 2. * kk};
+                set_roots(c, cy, 2.0, Y);
+                double* Ct = g->C + (size_t)((i + 3) * 49 + (j + 3) * 7 + (kk + 3)) * nn * nn;
+                for (int n = 0; n < nn; ++n)
+                    for (int m = 0; m < nn; ++m) Ct[m + (size_t)n * nn] = kernel_r(X + 3 * m, Y + 3 * n);
+            }
+    free(X); free(Y);
+    *out = g;
+    return ORC_OK;
+}
+
+static double unif_L(int p, int n, double x);
+
+/* The equispaced roots of FUnifRootsCore (Kernels/Uniform/FUnifRoots.hpp:
+ * 130-240), as tabulated there (decimal data). */
+static int unif_roots(int p, double* r) {
+    static const double r5[5] = {-1., -0.5, 0., 0.5, 1.};
+    static const double r6[6] = {-1., -0.6, -0.2, 0.2, 0.6, 1.};
+    static const double r7[7] = {-1., -0.666666666666666, -0.333333333333333, 0., 0.333333333333333,
+                                 0.666666666666666, 1.};
+    static const double r8[8] = {-1., -0.714285714285714, -0.428571428571429, -0.142857142857143,
+                                 0.142857142857143, 0.428571428571429, 0.714285714285714, 1.};
+    const double* t = p == 5 ? r5 : p == 6 ? r6 : p == 7 ? r7 : p == 8 ? r8 : NULL;
+    if (!t) return ORC_ERR_ARG;
+    memcpy(r, t, sizeof(double) * p);
+    return ORC_OK;
+}
+
+/* FUnifKernel (Kernels/Uniform/FUnifKernel.hpp:72-175): Lagrange interpolation
+ * on the equispaced roots (P2M, M2M, L2L, L2P: FUnifInterpolator.hpp), the
+ * M2L applying the kernel between the equispaced roots of target and source
+ * cell exactly (the reference does it by a zero-padded DFT,
+ * FUnifM2LHandler.hpp, which computes the same product); as a rank-p^3 set
+ * with identity bases. */
+int orc_unif_create(int order, orc_chebg** out) {
+    if (!out) return ORC_ERR_ARG;
+    orc_chebg* g = (orc_chebg*)calloc(1, sizeof(orc_chebg));
+    if (!g) return ORC_ERR_NOMEM;
+    orc_cheb* c = (orc_cheb*)calloc(1, sizeof(orc_cheb));
+    if (!c) { free(g); return ORC_ERR_NOMEM; }
+    g->c = c;
+    const int p = order;
+    c->p = p;
+    c->nn = p * p * p;
+    c->basis = 1;
+    c->eps = 0.;
+    if (unif_roots(p, c->roots)) { orc_chebg_destroy(g); return ORC_ERR_ARG; }
+    /* child/parent interpolators (FUnifInterpolator.hpp:100-123): child width 1,
+     * centre +-0.5, S[n][m] = L_n(child root m) */
+    c->child = (double*)malloc(sizeof(double) * 8 * 3 * p * p);
+    c->pvectors = (uint32_t*)calloc((size_t)343 * c->nn, sizeof(uint32_t));
+    static const int rel[8][3] = {{-1, -1, -1}, {-1, -1, 1}, {-1, 1, -1}, {-1, 1, 1},
+                                  {1, -1, -1},  {1, -1, 1},  {1, 1, -1},  {1, 1, 1}};
+    for (int ch = 0; ch < 8; ++ch)
+        for (int d = 0; d < 3; ++d) {
+            const double ctr = (double)rel[ch][d] * 0.5;
+            double* S = c->child + ((size_t)ch * 3 + d) * p * p;
+            for (int m = 0; m < p; ++m) {
+                const double xm = ctr + c->roots[m] / 2.;
+                for (int n = 0; n < p; ++n) S[n * p + m] = unif_L(p, n, xm);
+            }
+        }
+    const int nn = c->nn;
+    g->k = nn;
+    g->U = (double*)calloc((size_t)nn * nn, sizeof(double));
+    g->B = (double*)calloc((size_t)nn * nn, sizeof(double));
+    g->C = (double*)calloc((size_t)343 * nn * nn, sizeof(double));
+    g->S1 = (double*)calloc((size_t)nn, sizeof(double));
+    g->S2 = (double*)calloc((size_t)nn, sizeof(double));
+    double* X = (double*)malloc(sizeof(double) * 3 * nn);
+    double* Y = (double*)malloc(sizeof(double) * 3 * nn);
+    if (!c->child || !c->pvectors || !g->U || !g->B || !g->C || !g->S1 || !g->S2 || !X || !Y) {
+        free(X); free(Y); orc_chebg_destroy(g); return ORC_ERR_NOMEM;
+    }
+    for (int n = 0; n < nn; ++n) g->U[(size_t)n * nn + n] = g->B[(size_t)n * nn + n] = 1.0;
+    const double zero[3] = {0, 0, 0};
+    set_roots(c, zero, 2.0, X);
+    for (int i = -3; i <= 3; ++i)
+        for (int j = -3; j <= 3; ++j)
+            for (int kk = -3; kk <= 3; ++kk) {
+                if (abs(i) <= 1 && abs(j) <= 1 && abs(kk) <= 1) continue;
+                const double cy[3] = {2. * i, 2. * j, 2. * kk};
+                set_roots(c, cy, 2.0, Y);
+                double* Ct = g->C + (size_t)((i + 3) * 49 + (j + 3) * 7 + (kk + 3)) * nn * nn;
+                for (int n = 0; n < nn; ++n)
+                    for (int m = 0; m < nn; ++m) Ct[m + (size_t)n * nn] = kernel_r(X + 3 * m, Y + 3 * n);
+            }
+    free(X); free(Y);
+    *out = g;
+    return ORC_OK;
+}
+
 void orc_chebg_destroy(orc_chebg* g) {
     if (!g) return;
     orc_cheb_destroy(g->c);
@@ -857,8 +979,48 @@ int orc_cheb_dense_block(int order, int i, int j, int k, double* out) {
 /* S(x, xbar_j) = 1/p + 2/p sum_{o>=1} T_o(x) T_o(xbar_j), T_o(x) by the
  * three-term recurrence (FChebInterpolator.hpp:675-686).  dS = d/dx S using
  * T_o' = o U_{o-1} (FChebInterpolator.hpp:1306-1332). */
+/* FUnifRoots::L and ::dL (Kernels/Uniform/FUnifRoots.hpp:49-118): L_n by
+ * the scaled product, dL_n by the sum of products over the root table; the
+ * argument clamped to [-1, 1]. */
+static double unif_L(int p, int n, double x) {
+    if (x > 1.) x = 1.;
+    if (x < -1.) x = -1.;
+    const int omn = p - n - 1;
+    double fn = 1., fo = 1.;
+    for (int i = 2; i <= n; ++i) fn *= i;
+    for (int i = 2; i <= omn; ++i) fo *= i;
+    double scale = (omn % 2) ? -1. : 1.;
+    scale /= pow(2., (double)(p - 1)) * fn * fo;
+    double L = 1.;
+    for (int m = 0; m < p; ++m)
+        if (m != n) L *= (double)(p - 1) * (x + 1.) - 2. * m;
+    return L * scale;
+}
+
+static double unif_dL(const double* roots, int p, int n, double x) {
+    if (x > 1.) x = 1.;
+    if (x < -1.) x = -1.;
+    double num = 0., den = 1.;
+    for (int q = 0; q < p; ++q) {
+        if (q == n) continue;
+        double t = 1.;
+        for (int m = 0; m < p; ++m)
+            if (m != n && m != q) t *= x - roots[m];
+        num += t;
+        den *= roots[n] - roots[q];
+    }
+    return num / den;
+}
+
 static void interp_1d(const orc_cheb* c, double xi, double* S, double* dS) {
     const int p = c->p;
+    if (c->basis == 1) {
+        for (int j = 0; j < p; ++j) {
+            S[j] = unif_L(p, j, xi);
+            if (dS) dS[j] = unif_dL(c->roots, p, j, xi);
+        }
+        return;
+    }
     double T[ORC_MAX_ORDER], U[ORC_MAX_ORDER];
     T[0] = 1.; T[1] = xi;
     U[0] = 1.; U[1] = 2. * xi;

@@ -168,6 +168,13 @@ void orc_run_reset_cells(orc_run* r);
  * [343][k x k] column-major; S1 / S2: the singular values of K_col / K_row. */
 typedef struct orc_chebg orc_chebg;
 int orc_chebg_create(int order, double epsilon, orc_chebg** out);
+/* FChebDenseKernel's operators (FChebDenseM2LHandler.hpp:180-215): rank p^3,
+ * identity U and B, C_t the uncompressed K_t; used with orc_fmm_global. */
+int orc_chebd_create(int order, orc_chebg** out);
+/* FUnifKernel (Kernels/Uniform/FUnifKernel.hpp): Lagrange interpolation on the
+ * reference's equispaced roots, the M2L applying the kernel exactly between
+ * the equispaced nodes (rank p^3, identity bases); used with orc_fmm_global. */
+int orc_unif_create(int order, orc_chebg** out);
 void orc_chebg_destroy(orc_chebg* g);
 int orc_chebg_rank(const orc_chebg* g);
 const orc_cheb* orc_chebg_cheb(const orc_chebg* g);

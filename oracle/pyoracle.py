@@ -92,6 +92,8 @@ def lib():
     L.orc_run_expansions.argtypes = [vp, C.c_int, _f64p, _f64p]
     L.orc_chebg_create.argtypes = [C.c_int, C.c_double, C.POINTER(vp)]
     L.orc_chebg_destroy.argtypes = [vp]
+    L.orc_chebd_create.argtypes = [C.c_int, C.POINTER(vp)]
+    L.orc_unif_create.argtypes = [C.c_int, C.POINTER(vp)]
     L.orc_chebg_rank.argtypes = [vp]
     L.orc_chebg_get.argtypes = [vp, _f64p, _f64p, _f64p, _f64p, _f64p]
     L.orc_fmm_global.argtypes = [vp, C.c_int64, _f64p, _f64p, _f64p, _f64p, _f64p, C.c_double, C.c_int,
@@ -358,11 +360,16 @@ class Run:
 class ChebGlobal:
     """FChebKernel's operators: one global SVD of the 316 far-field operators."""
 
-    def __init__(self, order, epsilon=0.0):
+    def __init__(self, order, epsilon=0.0, dense=False, uniform=False):
         self.order = order
         self.nnodes = order ** 3
         h = C.c_void_p()
-        st = lib().orc_chebg_create(order, float(epsilon), C.byref(h))
+        if uniform:  # FUnifKernel: Lagrange on equispaced roots, the kernel applied exactly
+            st = lib().orc_unif_create(order, C.byref(h))
+        elif dense:  # FChebDenseKernel: the operators uncompressed, identity bases
+            st = lib().orc_chebd_create(order, C.byref(h))
+        else:
+            st = lib().orc_chebg_create(order, float(epsilon), C.byref(h))
         if st:
             raise RuntimeError(f"orc_chebg_create failed: {st}")
         self.h = h

@@ -21,6 +21,7 @@ NEAR_AND_FAR = NEAR_FIELD | FAR_FIELD
 
 M2L_SYMMETRIC = 0
 M2L_GLOBAL_SVD = 1
+M2L_DENSE = 2
 
 CELLS_FRESH = 0
 CELLS_KEPT = 1

@@ -238,4 +238,43 @@ int build_global_operators(const ChebOperators& op, GlobalOperators& g, std::str
     return HFMM_OK;
 }
 
+int build_dense_operators(const ChebOperators& op, GlobalOperators& g) {
+    // FChebDenseM2LHandler (Src/Kernels/Chebyshev/FChebDenseM2LHandler.hpp:180-215):
+    // the 316 operators K_t[m][n] = K(X_m, Y_n) uncompressed, rank = p^3, so
+    // that the global scheme's passes apply them with identity bases
+    const int p = op.p, nn = op.nn;
+    std::vector<double> X(3 * nn);
+    for (int n = 0; n < nn; ++n) {
+        const int id[3] = {n % p, (n / p) % p, n / (p * p)};
+        for (int d = 0; d < 3; ++d) X[3 * n + d] = op.roots[id[d]];
+    }
+    g.k = nn;
+    g.U.assign(static_cast<size_t>(nn) * nn, 0.);
+    for (int n = 0; n < nn; ++n) g.U[static_cast<size_t>(n) * nn + n] = 1.;
+    g.B = g.U;
+    g.S1.assign(nn, 0.);
+    g.S2.assign(nn, 0.);
+    g.C.assign(static_cast<size_t>(343) * nn * nn, 0.);
+    std::vector<int> tpos;
+    for (int i = -3; i <= 3; ++i)
+        for (int j = -3; j <= 3; ++j)
+            for (int k = -3; k <= 3; ++k)
+                if (std::abs(i) > 1 || std::abs(j) > 1 || std::abs(k) > 1) tpos.push_back((i + 3) * 49 + (j + 3) * 7 + (k + 3));
+    parallel_for(static_cast<int>(tpos.size()), [&](int t) {
+        const int pos = tpos[t];
+        const int off[3] = {2 * (pos / 49 - 3), 2 * ((pos / 7) % 7 - 3), 2 * (pos % 7 - 3)};
+        double* Ct = g.C.data() + static_cast<size_t>(pos) * nn * nn;
+        for (int n = 0; n < nn; ++n)
+            for (int m = 0; m < nn; ++m) {
+                double r2 = 0.;
+                for (int d = 0; d < 3; ++d) {
+                    const double dd = X[3 * m + d] - (X[3 * n + d] + off[d]);
+                    r2 += dd * dd;
+                }
+                Ct[m + static_cast<size_t>(n) * nn] = 1. / std::sqrt(r2);
+            }
+    });
+    return HFMM_OK;
+}
+
 }  // namespace hfmm

@@ -14,7 +14,7 @@
 namespace hfmm {
 namespace {
 
-constexpr int kMaxRankRows = 4;  // ranks up to 256
+constexpr int kMaxRankRows = 8;  // ranks up to 512 (the dense scheme's p^3 at order 8)
 
 // M~[cell][a] = sum_n B[n][a] M[cell][n]; Bt is B as [nn][k].
 template <int R>
@@ -108,12 +108,14 @@ int run_global_compress(Context* c, int l) {
     const double* M = c->d_M.as<double>() + c->lvl_off[l] * c->LD;
     double* Mt = c->d_Mt.as<double>() + c->lvl_off[l] * k;
     prof_begin(c, "global_compress");
+    const double* Bt = c->d_gBt.as<double>();
+    const int nn = c->ops.nn;
+    const dim3 grid(static_cast<unsigned>(nc));
     switch (R) {
-        case 1: hipLaunchKernelGGL(k_global_compress<1>, dim3(static_cast<unsigned>(nc)), dim3(64), 0, c->stream, nc, M, c->LD, c->d_gBt.as<double>(), c->ops.nn, k, Mt); break;
-        case 2: hipLaunchKernelGGL(k_global_compress<2>, dim3(static_cast<unsigned>(nc)), dim3(64), 0, c->stream, nc, M, c->LD, c->d_gBt.as<double>(), c->ops.nn, k, Mt); break;
-        case 3: hipLaunchKernelGGL(k_global_compress<3>, dim3(static_cast<unsigned>(nc)), dim3(64), 0, c->stream, nc, M, c->LD, c->d_gBt.as<double>(), c->ops.nn, k, Mt); break;
-        case 4: hipLaunchKernelGGL(k_global_compress<4>, dim3(static_cast<unsigned>(nc)), dim3(64), 0, c->stream, nc, M, c->LD, c->d_gBt.as<double>(), c->ops.nn, k, Mt); break;
-        default: return fail(c, HFMM_ERR_ARG, "global M2L rank above 256");
+#define CASE(RR) case RR: hipLaunchKernelGGL(k_global_compress<RR>, grid, dim3(64), 0, c->stream, nc, M, c->LD, Bt, nn, k, Mt); break;
+        CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7) CASE(8)
+#undef CASE
+        default: return fail(c, HFMM_ERR_ARG, "global M2L rank above 512");
     }
     prof_end(c);
     return check_launch(c, "global_compress");
@@ -133,7 +135,7 @@ int run_global_decompress(Context* c, int l) {
 
 int run_global_m2l(Context* c) {
     const int k = c->glob.k, R = (k + 63) / 64;
-    if (R > kMaxRankRows) return fail(c, HFMM_ERR_ARG, "global M2L rank above 256");
+    if (R > kMaxRankRows) return fail(c, HFMM_ERR_ARG, "global M2L rank above 512");
     const int L = c->height() - 1;
     // the M2L levels of the working range, from the top
     const int lo = c->work_upper > c->top() ? c->work_upper : c->top();
@@ -149,11 +151,12 @@ int run_global_m2l(Context* c) {
         double* Lt = c->d_Lt.as<double>() + c->lvl_off[l] * k;
         const int16_t* sp = c->d_slotpos.as<int16_t>();
         const double* Cg = c->d_gC.as<double>();
+        const dim3 grid(static_cast<unsigned>(nc));
         switch (R) {
-            case 1: hipLaunchKernelGGL(k_global_m2l<1>, dim3(static_cast<unsigned>(nc)), dim3(64), 0, c->stream, nc, slots, keys, sp, Cg, Mt, k, scale, Lt); break;
-            case 2: hipLaunchKernelGGL(k_global_m2l<2>, dim3(static_cast<unsigned>(nc)), dim3(64), 0, c->stream, nc, slots, keys, sp, Cg, Mt, k, scale, Lt); break;
-            case 3: hipLaunchKernelGGL(k_global_m2l<3>, dim3(static_cast<unsigned>(nc)), dim3(64), 0, c->stream, nc, slots, keys, sp, Cg, Mt, k, scale, Lt); break;
-            default: hipLaunchKernelGGL(k_global_m2l<4>, dim3(static_cast<unsigned>(nc)), dim3(64), 0, c->stream, nc, slots, keys, sp, Cg, Mt, k, scale, Lt); break;
+#define CASE(RR) case RR: hipLaunchKernelGGL(k_global_m2l<RR>, grid, dim3(64), 0, c->stream, nc, slots, keys, sp, Cg, Mt, k, scale, Lt); break;
+            CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7) CASE(8)
+#undef CASE
+            default: break;
         }
         HFMM_TRY(check_launch(c, "global_m2l"));
     }

@@ -329,7 +329,7 @@ static int evaluate_passes(Context* c, uint32_t ops, size_t exp_bytes) {
         HFMM_HIP(c, hipEventRecord(c->ev_join, c->side));
     }
     // FChebKernel's scheme: compressed expansions beside M and L
-    const bool glob = c->m2l_scheme == HFMM_M2L_GLOBAL_SVD;
+    const bool glob = c->m2l_scheme != HFMM_M2L_SYMMETRIC;
     const size_t cexp_bytes = sizeof(double) * static_cast<size_t>(c->total_cells) * c->glob.k;
     // upward pass (fresh cells: multipoles that no pass of this evaluation
     // writes read as zero)
@@ -536,7 +536,7 @@ int hfmm_evaluate_levels(hfmm_ctx* h, uint32_t ops, int32_t upper, int32_t lower
     // d_kept + 2 r exp (M, then L) between evaluations
     const int64_t exp_words = static_cast<int64_t>(exp_bytes / sizeof(double));
     const bool kept_many = c->keep_cells && c->nrhs > 1;
-    const bool glob = c->m2l_scheme == HFMM_M2L_GLOBAL_SVD;
+    const bool glob = c->m2l_scheme != HFMM_M2L_SYMMETRIC;
     if (glob) {
         const size_t cexp = sizeof(double) * static_cast<size_t>(c->total_cells) * c->glob.k;
         HFMM_TRY(hfmm::ensure(c, c->d_Mt, cexp));
@@ -845,17 +845,18 @@ int hfmm_get_expansions(hfmm_ctx* h, int32_t level, double* M, double* L) {
 int hfmm_set_m2l_scheme(hfmm_ctx* h, int32_t scheme) {
     CTX_CHECK(h);
     Context* c = &h->c;
-    if (scheme != HFMM_M2L_SYMMETRIC && scheme != HFMM_M2L_GLOBAL_SVD)
+    if (scheme != HFMM_M2L_SYMMETRIC && scheme != HFMM_M2L_GLOBAL_SVD && scheme != HFMM_M2L_DENSE)
         return hfmm::fail(c, HFMM_ERR_ARG, "unknown M2L scheme");
-    if (scheme == HFMM_M2L_GLOBAL_SVD) {
+    if (scheme != HFMM_M2L_SYMMETRIC) {
         if (c->P.precision != HFMM_PRECISION_FP64)
-            return hfmm::fail(c, HFMM_ERR_ARG, "the global-SVD M2L scheme runs in FP64 only");
-        if (c->glob.k == 0) {
+            return hfmm::fail(c, HFMM_ERR_ARG, "the global-SVD and dense M2L schemes run in FP64 only");
+        if (c->glob.k == 0 || scheme != c->m2l_scheme) {
             std::string err;
             hfmm::GlobalOperators g;
-            const int st = hfmm::build_global_operators(c->ops, g, err);
+            const int st = scheme == HFMM_M2L_DENSE ? hfmm::build_dense_operators(c->ops, g)
+                                                    : hfmm::build_global_operators(c->ops, g, err);
             if (st != HFMM_OK) return hfmm::fail(c, st, err);
-            if (g.k > 256) return hfmm::fail(c, HFMM_ERR_ARG, "global M2L rank above 256");
+            if (g.k > 512) return hfmm::fail(c, HFMM_ERR_ARG, "global M2L rank above 512");
             const int nn = c->ops.nn, k = g.k;
             std::vector<double> bt(static_cast<size_t>(nn) * k);
             for (int a = 0; a < k; ++a)
@@ -879,7 +880,7 @@ int hfmm_set_m2l_scheme(hfmm_ctx* h, int32_t scheme) {
 int hfmm_get_global_operators(hfmm_ctx* h, int32_t* rank, double* U, double* B, double* C, double* S1, double* S2) {
     CTX_CHECK(h);
     Context* c = &h->c;
-    if (c->glob.k == 0) return hfmm::fail(c, HFMM_ERR_STATE, "the global-SVD M2L operators are not built");
+    if (c->glob.k == 0) return hfmm::fail(c, HFMM_ERR_STATE, "no global-SVD or dense M2L operators are built");
     if (rank) *rank = c->glob.k;
     if (U) std::memcpy(U, c->glob.U.data(), sizeof(double) * c->glob.U.size());
     if (B) std::memcpy(B, c->glob.B.data(), sizeof(double) * c->glob.B.size());

@@ -44,6 +44,8 @@ struct GlobalOperators {
     std::vector<double> U, B, C, S1, S2;
 };
 int build_global_operators(const ChebOperators& op, GlobalOperators& g, std::string& err);
+// FChebDenseKernel's operators: rank p^3, identity bases, C_t = K_t
+int build_dense_operators(const ChebOperators& op, GlobalOperators& g);
 
 struct Context {
     hfmm_params P{};

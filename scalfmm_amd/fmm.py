@@ -385,6 +385,16 @@ class FChebKernel(FChebSymKernel):
         self.engine.set_m2l_scheme(_abi.M2L_GLOBAL_SVD)
 
 
+class FChebDenseKernel(FChebSymKernel):
+    """Kernel parameters of FChebDenseKernel<double, ..., FInterpMatrixKernelR, ORDER>
+    (Src/Kernels/Chebyshev/FChebDenseKernel.hpp): the 316 M2L operators
+    applied uncompressed (FP64)."""
+
+    def __init__(self, tree_height, box_width, box_center, order, device=0):
+        super().__init__(tree_height, box_width, box_center, order, None, device, _abi.PRECISION_FP64)
+        self.engine.set_m2l_scheme(_abi.M2L_DENSE)
+
+
 class FGroupTree:
     """Particles of one simulation box; the octree itself lives on the device."""
 

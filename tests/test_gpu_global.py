@@ -128,3 +128,49 @@ def test_utest_chebyshev_cheb_kernel(oracle):
     assert _rel(pot, ref[0]) < 1e-12
     for k in range(3):
         assert _rel(forces[:, k], ref[1 + k]) < 1e-12
+
+
+def test_dense_fmm_matches_oracle(oracle, fixture20k):
+    """FChebDenseKernel (HFMM_M2L_DENSE) against the oracle's dense operators."""
+    from scalfmm_amd import _abi
+    from scalfmm_amd.fmm import Engine
+    d = fixture20k["data"]
+    c, w = fixture20k["center"], fixture20k["width"]
+    x, y, z, q = d[:, 0], d[:, 1], d[:, 2], d[:, 3]
+    e = Engine(5, w, c, 5)
+    e.set_m2l_scheme(_abi.M2L_DENSE)
+    e.set_particles(x, y, z, q)
+    e.build_tree()
+    e.evaluate()
+    got = e.results()
+    ref = oracle.fmm_global(oracle.ChebGlobal(5, dense=True), x, y, z, q, c, w, 5)
+    errs = [_rel(a, b) for a, b in zip(got, ref[:4])]
+    print(f"  dense: rel L2 vs oracle {max(errs):.2e}", flush=True)
+    assert max(errs) < 1e-12
+    e.close()
+
+
+def test_utest_chebyshev_dense_kernel(oracle):
+    """TestChebDenseKernel (UTests/utestChebyshev.cpp:33-48): order 6, the
+    classic octree of 4 levels, DirectDouble.bfma, the FUKernelTester gates."""
+    from scalfmm_amd import FChebDenseKernel, octree
+    from scalfmm_amd.fma import FFmaGenericLoader
+    loader = FFmaGenericLoader(os.path.join(ROOT, "tests", "golden", "DirectDouble.bfma"))
+    d = loader.data
+    n = loader.getNumberOfParticles()
+    tree = octree.FOctree(4, 2, loader.getBoxWidth(), loader.getCenterOfBox())
+    tree.insert_many(d[:, 0], d[:, 1], d[:, 2], d[:, 3], np.arange(n))
+    octree.FFmmAlgorithm(tree, FChebDenseKernel(4, loader.getBoxWidth(), loader.getCenterOfBox(), 6)).execute()
+    pot = np.zeros(n)
+    fx = np.zeros(n)
+
+    def visit(leaf):
+        t = leaf.getTargets()
+        pot[t.getIndexes()] = t.getPotentials()
+        fx[t.getIndexes()] = t.getForcesX()
+    tree.forEachLeaf(visit)
+    assert oracle.accuracy(d[:, 4], pot)["rel_l2"] < 9e-3
+    assert oracle.accuracy(d[:, 5], fx)["rel_l2"] < 9e-2
+    ref = oracle.fmm_global(oracle.ChebGlobal(6, dense=True), d[:, 0], d[:, 1], d[:, 2], d[:, 3],
+                            loader.getCenterOfBox(), loader.getBoxWidth(), 4)
+    assert _rel(pot, ref[0]) < 1e-12 and _rel(fx, ref[1]) < 1e-12
