@@ -187,6 +187,13 @@ int hfmm_evaluate_particle_blocks(hfmm_ctx* ctx, int32_t nblocks, const void* co
 #define HFMM_M2L_DENSE 2 /* FChebDenseKernel (FChebDenseKernel.hpp:93-165,
                             FChebDenseM2LHandler.hpp:180-215): the 316 operators
                             uncompressed (rank p^3, identity bases) */
+#define HFMM_M2L_UNIFORM 3 /* FUnifKernel (Kernels/Uniform/FUnifKernel.hpp:72-175):
+                              P2M, M2M, L2L and L2P interpolate on the equispaced
+                              roots of FUnifRoots with Lagrange polynomials; the M2L
+                              applies the kernel between the equispaced nodes of
+                              target and source cells (the product the reference
+                              computes by a zero-padded DFT, FUnifM2LHandler.hpp),
+                              as dense rank-p^3 operators */
 int hfmm_set_m2l_scheme(hfmm_ctx* ctx, int32_t scheme);
 int hfmm_get_global_operators(hfmm_ctx* ctx, int32_t* rank, double* U, double* B, double* C, double* S1, double* S2);
 

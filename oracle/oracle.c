@@ -817,13 +817,16 @@ static double unif_L(int p, int n, double x);
 /* The equispaced roots of FUnifRootsCore (Kernels/Uniform/FUnifRoots.hpp:
  * 130-240), as tabulated there (decimal data). */
 static int unif_roots(int p, double* r) {
+    static const double r2[2] = {-1., 1.};
+    static const double r3[3] = {-1., 0., 1.};
+    static const double r4[4] = {-1., -0.333333333333333, 0.333333333333333, 1.};
     static const double r5[5] = {-1., -0.5, 0., 0.5, 1.};
     static const double r6[6] = {-1., -0.6, -0.2, 0.2, 0.6, 1.};
     static const double r7[7] = {-1., -0.666666666666666, -0.333333333333333, 0., 0.333333333333333,
                                  0.666666666666666, 1.};
     static const double r8[8] = {-1., -0.714285714285714, -0.428571428571429, -0.142857142857143,
                                  0.142857142857143, 0.428571428571429, 0.714285714285714, 1.};
-    const double* t = p == 5 ? r5 : p == 6 ? r6 : p == 7 ? r7 : p == 8 ? r8 : NULL;
+    const double* t = p == 2 ? r2 : p == 3 ? r3 : p == 4 ? r4 : p == 5 ? r5 : p == 6 ? r6 : p == 7 ? r7 : p == 8 ? r8 : NULL;
     if (!t) return ORC_ERR_ARG;
     memcpy(r, t, sizeof(double) * p);
     return ORC_OK;

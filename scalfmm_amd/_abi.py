@@ -22,6 +22,7 @@ NEAR_AND_FAR = NEAR_FIELD | FAR_FIELD
 M2L_SYMMETRIC = 0
 M2L_GLOBAL_SVD = 1
 M2L_DENSE = 2
+M2L_UNIFORM = 3
 
 CELLS_FRESH = 0
 CELLS_KEPT = 1

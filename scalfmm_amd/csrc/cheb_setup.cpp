@@ -146,6 +146,50 @@ int build_cheb_operators(int order, double eps, ChebOperators& op, std::string& 
     return HFMM_OK;
 }
 
+double unif_lagrange(int p, int n, double x) {
+    // FUnifRoots::L (Src/Kernels/Uniform/FUnifRoots.hpp:49-83): the scaled product
+    if (x > 1.) x = 1.;
+    if (x < -1.) x = -1.;
+    const int omn = p - n - 1;
+    double fn = 1., fo = 1.;
+    for (int i = 2; i <= n; ++i) fn *= i;
+    for (int i = 2; i <= omn; ++i) fo *= i;
+    double scale = (omn % 2) ? -1. : 1.;
+    scale /= std::pow(2., static_cast<double>(p - 1)) * fn * fo;
+    double L = 1.;
+    for (int m = 0; m < p; ++m)
+        if (m != n) L *= static_cast<double>(p - 1) * (x + 1.) - 2. * m;
+    return L * scale;
+}
+
+int make_uniform_tables(ChebOperators& op) {
+    // the equispaced roots as FUnifRootsCore tabulates them (FUnifRoots.hpp:130-240)
+    static const double r2[2] = {-1., 1.};
+    static const double r3[3] = {-1., 0., 1.};
+    static const double r4[4] = {-1., -0.333333333333333, 0.333333333333333, 1.};
+    static const double r5[5] = {-1., -0.5, 0., 0.5, 1.};
+    static const double r6[6] = {-1., -0.6, -0.2, 0.2, 0.6, 1.};
+    static const double r7[7] = {-1., -0.666666666666666, -0.333333333333333, 0., 0.333333333333333,
+                                 0.666666666666666, 1.};
+    static const double r8[8] = {-1., -0.714285714285714, -0.428571428571429, -0.142857142857143,
+                                 0.142857142857143, 0.428571428571429, 0.714285714285714, 1.};
+    const int p = op.p;
+    const double* r = p == 2 ? r2 : p == 3 ? r3 : p == 4 ? r4 : p == 5 ? r5 : p == 6 ? r6 : p == 7 ? r7 : p == 8 ? r8 : nullptr;
+    if (!r) return HFMM_ERR_ARG;
+    for (int i = 0; i < p; ++i) op.roots[i] = r[i];
+    // child/parent interpolators (FUnifInterpolator.hpp:100-123): child width 1,
+    // centre +-0.5, S[n][m] = L_n(child root m)
+    for (int ch = 0; ch < 8; ++ch)
+        for (int d = 0; d < 3; ++d) {
+            const int bit = (ch >> (2 - d)) & 1;
+            const double ctr = bit ? 0.5 : -0.5;
+            double* S = &op.child[(static_cast<size_t>(ch) * 3 + d) * p * p];
+            for (int m = 0; m < p; ++m)
+                for (int n = 0; n < p; ++n) S[n * p + m] = unif_lagrange(p, n, ctr + op.roots[m] / 2.);
+        }
+    return HFMM_OK;
+}
+
 void far_slots(int parity, FarSlot out[189]) {
     const int bx = (parity >> 2) & 1, by = (parity >> 1) & 1, bz = parity & 1;
     int s = 0;

@@ -34,6 +34,12 @@ bool cheb_roots(int p, double* out);
 // operators: compress_operators_device); returns an HFMM_* status.
 int build_cheb_operators(int order, double eps, ChebOperators& ops, std::string& err);
 
+// The uniform (Lagrange) interpolation of FUnifKernel: the equispaced roots of
+// FUnifRootsCore and the child/parent tables built with FUnifRoots::L; returns
+// an HFMM_* status (orders 2..8).
+double unif_lagrange(int p, int n, double x);
+int make_uniform_tables(ChebOperators& op);
+
 // The 189 far-field slots of a cell of parity e (= Morton child index 0..7):
 // slot j <-> (parent-neighbour offset, cousin) in the reference enumeration
 // order of FTreeCoordinate::getInteractionNeighbors (FTreeCoordinate.hpp:219-264).

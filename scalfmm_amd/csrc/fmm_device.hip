@@ -49,13 +49,53 @@ __device__ __forceinline__ void interp_weights(double xi, const double* __restri
     }
 }
 
+constexpr int kUniformMaxOrder = 8;  // the uniform scheme's dense M2L holds p^3 <= 512 ranks
+
+// FUnifRoots::L and ::dL (Src/Kernels/Uniform/FUnifRoots.hpp:49-118) on the
+// equispaced roots r: L_n by the scaled product, dL_n by the sum of products
+// over the root table; the argument clamped to [-1, 1].
+template <int P, bool GRAD>
+__device__ __forceinline__ void lagrange_weights(double xi, const double* __restrict__ r, double* S, double* dS) {
+    const double x = fmin(fmax(xi, -1.), 1.);
+#pragma unroll
+    for (int n = 0; n < P; ++n) {
+        const int omn = P - n - 1;
+        double fn = 1., fo = 1.;
+        for (int i = 2; i <= n; ++i) fn *= i;
+        for (int i = 2; i <= omn; ++i) fo *= i;
+        double scale = (omn % 2) ? -1. : 1.;
+        double p2 = 1.;
+        for (int i = 0; i < P - 1; ++i) p2 *= 2.;
+        scale /= p2 * fn * fo;
+        double L = 1.;
+#pragma unroll
+        for (int m = 0; m < P; ++m)
+            if (m != n) L *= static_cast<double>(P - 1) * (x + 1.) - 2. * m;
+        S[n] = L * scale;
+        if (GRAD) {
+            double num = 0., den = 1.;
+#pragma unroll
+            for (int q = 0; q < P; ++q) {
+                if (q == n) continue;
+                double t = 1.;
+#pragma unroll
+                for (int m = 0; m < P; ++m)
+                    if (m != n && m != q) t *= x - r[m];
+                num += t;
+                den *= r[n] - r[q];
+            }
+            dS[n] = num / den;
+        }
+    }
+}
+
 // ---------------------------------------------------------------------------
 // P2M: one wave per leaf, 64 particles per round staged in LDS.
-template <int P>
+template <int P, int BASIS>
 __global__ __launch_bounds__(64) void k_p2m(int nleaf, const uint64_t* __restrict__ lkeys,
                                            const int32_t* __restrict__ lstart, const double* __restrict__ sp,
                                            int64_t n, Box bx, const double* __restrict__ Troots,
-                                           double* __restrict__ M, int LD) {
+                                           const double* __restrict__ uroots, double* __restrict__ M, int LD) {
     constexpr int NN = P * P * P;
     constexpr int PER = (NN + 63) / 64;
     __shared__ double sS[64][3 * P];
@@ -79,7 +119,10 @@ __global__ __launch_bounds__(64) void k_p2m(int nleaf, const uint64_t* __restric
 #pragma unroll
             for (int d = 0; d < 3; ++d) {
                 const double xi = (2. * pos[d] - b[d] - a[d]) / (b[d] - a[d]);
-                interp_weights<P, false>(xi, sT, &sS[lane][d * P], nullptr);
+                if constexpr (BASIS == 1)
+                    lagrange_weights<P, false>(xi, uroots, &sS[lane][d * P], nullptr);
+                else
+                    interp_weights<P, false>(xi, sT, &sS[lane][d * P], nullptr);
             }
             sQ[lane] = sp[3 * n + e];
         }
@@ -280,6 +323,55 @@ __global__ __launch_bounds__(64) void k_l2p(int nleaf, const uint64_t* __restric
             gy += ty * sz;
             gz += tp * dsz;
         }
+        res[e] += ph;
+        res[n + e] += gx * (2. / (b[0] - a[0]));
+        res[2 * n + e] += gy * (2. / (b[1] - a[1]));
+        res[3 * n + e] += gz * (2. / (b[2] - a[2]));
+    }
+}
+
+// L2P on the uniform (Lagrange) basis (FUnifInterpolator.hpp:476-640,
+// applyL2P + applyL2PGradient): one wave per leaf, one lane per particle.
+template <int P>
+__global__ __launch_bounds__(64) void k_l2p_lagrange(int nleaf, const uint64_t* __restrict__ lkeys,
+                                                    const int32_t* __restrict__ lstart,
+                                                    const double* __restrict__ sp, int64_t n, Box bx,
+                                                    const double* __restrict__ uroots, const double* __restrict__ L,
+                                                    int LD, double* __restrict__ res) {
+    constexpr int NN = P * P * P;
+    __shared__ double sL[NN];
+    __shared__ double sR[P];
+    const int leaf = blockIdx.x, lane = threadIdx.x;
+    if (lane < P) sR[lane] = uroots[lane];
+    for (int i = lane; i < NN; i += 64) sL[i] = L[static_cast<int64_t>(leaf) * LD + i];
+    double a[3], b[3];
+    leaf_bounds(bx, lkeys[leaf], a, b);
+    __syncthreads();
+    const int s0 = lstart[leaf], s1 = lstart[leaf + 1];
+    for (int e = s0 + lane; e < s1; e += 64) {
+        const double pos[3] = {sp[e], sp[n + e], sp[2 * n + e]};
+        double S[3][P], dS[3][P];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            const double xi = (2. * pos[d] - b[d] - a[d]) / (b[d] - a[d]);
+            lagrange_weights<P, true>(xi, sR, S[d], dS[d]);
+        }
+        double ph = 0., gx = 0., gy = 0., gz = 0.;
+        for (int k = 0; k < P; ++k)
+#pragma unroll
+            for (int j = 0; j < P; ++j) {
+                double va = 0., vb = 0.;
+#pragma unroll
+                for (int i = 0; i < P; ++i) {
+                    const double l = sL[(k * P + j) * P + i];
+                    va += l * S[0][i];
+                    vb += l * dS[0][i];
+                }
+                ph += va * S[1][j] * S[2][k];
+                gx += vb * S[1][j] * S[2][k];
+                gy += va * dS[1][j] * S[2][k];
+                gz += va * S[1][j] * dS[2][k];
+            }
         res[e] += ph;
         res[n + e] += gx * (2. / (b[0] - a[0]));
         res[2 * n + e] += gy * (2. / (b[1] - a[1]));
@@ -772,10 +864,14 @@ int p2m_impl(Context* c) {
     const int L = c->height() - 1;
     const int64_t nl = c->ncell[L];
     prof_begin(c, "p2m");
-    hipLaunchKernelGGL(k_p2m<P>, dim3(static_cast<unsigned>(nl)), dim3(64), 0, c->stream, static_cast<int>(nl),
+    auto kern = k_p2m<P, 0>;
+    if constexpr (P <= kUniformMaxOrder)
+        if (c->basis == 1) kern = k_p2m<P, 1>;
+    hipLaunchKernelGGL(kern, dim3(static_cast<unsigned>(nl)), dim3(64), 0, c->stream, static_cast<int>(nl),
                        c->d_cellkey.as<uint64_t>() + c->lvl_off[L], c->d_leafstart.as<int32_t>(),
                        c->d_sorted.as<double>(), c->n, make_box(c), c->d_Troots.as<double>(),
-                       c->d_M.as<double>() + static_cast<size_t>(c->lvl_off[L]) * c->LD, c->LD);
+                       c->d_uroots.as<double>(), c->d_M.as<double>() + static_cast<size_t>(c->lvl_off[L]) * c->LD,
+                       c->LD);
     prof_end(c);
     return check_launch(c, "p2m");
 }
@@ -809,6 +905,18 @@ int l2p_impl(Context* c) {
     const int L = c->height() - 1;
     const int64_t nl = c->ncell[L];
     prof_begin(c, "l2p");
+    if constexpr (P <= kUniformMaxOrder) {
+        if (c->basis == 1) {
+            hipLaunchKernelGGL(k_l2p_lagrange<P>, dim3(static_cast<unsigned>(nl)), dim3(64), 0, c->stream,
+                               static_cast<int>(nl), c->d_cellkey.as<uint64_t>() + c->lvl_off[L],
+                               c->d_leafstart.as<int32_t>(), c->d_sorted.as<double>(), c->n, make_box(c),
+                               c->d_uroots.as<double>(),
+                               c->d_L.as<double>() + static_cast<size_t>(c->lvl_off[L]) * c->LD, c->LD,
+                               c->d_res_sorted.as<double>());
+            prof_end(c);
+            return check_launch(c, "l2p");
+        }
+    }
     hipLaunchKernelGGL(k_l2p<P>, dim3(static_cast<unsigned>(nl)), dim3(64), 0, c->stream, static_cast<int>(nl),
                        c->d_cellkey.as<uint64_t>() + c->lvl_off[L], c->d_leafstart.as<int32_t>(),
                        c->d_sorted.as<double>(), c->n, make_box(c), c->d_Troots.as<double>(),

@@ -206,7 +206,7 @@ static void free_all(Context* c) {
                       &c->d_scan_part, &c->d_qin, &c->d_flags, &c->d_scan, &c->d_sorted, &c->d_perm, &c->d_pleaf,
                       &c->d_count, &c->d_cellkey, &c->d_cellkey_tmp, &c->d_childstart, &c->d_leafstart,
                       &c->d_p2p, &c->d_p2pitems, &c->d_m2l, &c->d_order, &c->d_pcount, &c->d_dense, &c->d_M, &c->d_L, &c->d_Mb, &c->d_loc, &c->d_res_sorted,
-                      &c->d_res, &c->d_field, &c->d_setoff, &c->d_keep, &c->d_kept, &c->d_gBt, &c->d_gU, &c->d_gC, &c->d_Mt, &c->d_Lt};
+                      &c->d_res, &c->d_field, &c->d_setoff, &c->d_keep, &c->d_kept, &c->d_gBt, &c->d_gU, &c->d_gC, &c->d_Mt, &c->d_Lt, &c->d_uroots};
     for (DevBuf* b : bufs) release(*b);
 }
 
@@ -845,23 +845,35 @@ int hfmm_get_expansions(hfmm_ctx* h, int32_t level, double* M, double* L) {
 int hfmm_set_m2l_scheme(hfmm_ctx* h, int32_t scheme) {
     CTX_CHECK(h);
     Context* c = &h->c;
-    if (scheme != HFMM_M2L_SYMMETRIC && scheme != HFMM_M2L_GLOBAL_SVD && scheme != HFMM_M2L_DENSE)
+    if (scheme != HFMM_M2L_SYMMETRIC && scheme != HFMM_M2L_GLOBAL_SVD && scheme != HFMM_M2L_DENSE &&
+        scheme != HFMM_M2L_UNIFORM)
         return hfmm::fail(c, HFMM_ERR_ARG, "unknown M2L scheme");
+    HFMM_HIP(c, hipSetDevice(c->device));
     if (scheme != HFMM_M2L_SYMMETRIC) {
         if (c->P.precision != HFMM_PRECISION_FP64)
-            return hfmm::fail(c, HFMM_ERR_ARG, "the global-SVD and dense M2L schemes run in FP64 only");
+            return hfmm::fail(c, HFMM_ERR_ARG, "the global-SVD, dense and uniform M2L schemes run in FP64 only");
         if (c->glob.k == 0 || scheme != c->m2l_scheme) {
             std::string err;
             hfmm::GlobalOperators g;
-            const int st = scheme == HFMM_M2L_DENSE ? hfmm::build_dense_operators(c->ops, g)
-                                                    : hfmm::build_global_operators(c->ops, g, err);
+            hfmm::ChebOperators u;  // the uniform basis: equispaced roots, Lagrange child tables
+            int st = HFMM_OK;
+            if (scheme == HFMM_M2L_UNIFORM) {
+                u.p = c->ops.p;
+                u.nn = c->ops.nn;
+                u.child.assign(static_cast<size_t>(8) * 3 * u.p * u.p, 0.);
+                if (hfmm::make_uniform_tables(u) != HFMM_OK)
+                    return hfmm::fail(c, HFMM_ERR_ARG, "no equispaced roots for this order");
+                st = hfmm::build_dense_operators(u, g);
+            } else {
+                st = scheme == HFMM_M2L_DENSE ? hfmm::build_dense_operators(c->ops, g)
+                                              : hfmm::build_global_operators(c->ops, g, err);
+            }
             if (st != HFMM_OK) return hfmm::fail(c, st, err);
             if (g.k > 512) return hfmm::fail(c, HFMM_ERR_ARG, "global M2L rank above 512");
             const int nn = c->ops.nn, k = g.k;
             std::vector<double> bt(static_cast<size_t>(nn) * k);
             for (int a = 0; a < k; ++a)
                 for (int n = 0; n < nn; ++n) bt[static_cast<size_t>(n) * k + a] = g.B[static_cast<size_t>(a) * nn + n];
-            HFMM_HIP(c, hipSetDevice(c->device));
             HFMM_TRY(hfmm::ensure(c, c->d_gBt, sizeof(double) * bt.size()));
             HFMM_TRY(hfmm::ensure(c, c->d_gU, sizeof(double) * g.U.size()));
             HFMM_TRY(hfmm::ensure(c, c->d_gC, sizeof(double) * g.C.size()));
@@ -869,7 +881,20 @@ int hfmm_set_m2l_scheme(hfmm_ctx* h, int32_t scheme) {
             HFMM_HIP(c, hipMemcpy(c->d_gU.ptr, g.U.data(), sizeof(double) * g.U.size(), hipMemcpyHostToDevice));
             HFMM_HIP(c, hipMemcpy(c->d_gC.ptr, g.C.data(), sizeof(double) * g.C.size(), hipMemcpyHostToDevice));
             c->glob = std::move(g);
+            // the interpolation tables of the basis the scheme works in
+            const hfmm::ChebOperators& tab = scheme == HFMM_M2L_UNIFORM ? u : c->ops;
+            HFMM_HIP(c, hipMemcpy(c->d_child.ptr, tab.child.data(), sizeof(double) * tab.child.size(),
+                                  hipMemcpyHostToDevice));
+            if (scheme == HFMM_M2L_UNIFORM) {
+                HFMM_TRY(hfmm::ensure(c, c->d_uroots, sizeof(double) * u.p));
+                HFMM_HIP(c, hipMemcpy(c->d_uroots.ptr, u.roots.data(), sizeof(double) * u.p, hipMemcpyHostToDevice));
+            }
+            c->basis = scheme == HFMM_M2L_UNIFORM ? 1 : 0;
         }
+    } else if (c->basis != 0) {
+        HFMM_HIP(c, hipMemcpy(c->d_child.ptr, c->ops.child.data(), sizeof(double) * c->ops.child.size(),
+                              hipMemcpyHostToDevice));
+        c->basis = 0;
     }
     if (scheme != c->m2l_scheme) c->cells_dirty = true;
     c->m2l_scheme = scheme;

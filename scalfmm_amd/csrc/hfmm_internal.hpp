@@ -143,6 +143,11 @@ struct Context {
     // M2L scheme: HFMM_M2L_SYMMETRIC (FChebSymKernel) or HFMM_M2L_GLOBAL_SVD
     // (FChebKernel: compressed multipoles / locals of rank k beside M and L)
     int m2l_scheme = 0;
+    // interpolation basis of P2M, M2M, L2L and L2P: 0 = Chebyshev (the
+    // symmetric, global-SVD and dense schemes), 1 = uniform Lagrange
+    // (HFMM_M2L_UNIFORM: d_child then holds the Lagrange child tables)
+    int basis = 0;
+    DevBuf d_uroots;          // [p] the equispaced roots of the uniform basis
     GlobalOperators glob;
     DevBuf d_gBt;             // B^T as [nn][k] (the compression reads a row per node)
     DevBuf d_gU;              // U as [k][nn]

@@ -167,8 +167,10 @@ class Engine:
         self._check(self._L.hfmm_set_cell_mode(self.h, int(mode)))
 
     def set_m2l_scheme(self, scheme):
-        """_abi.M2L_SYMMETRIC (FChebSymKernel) or _abi.M2L_GLOBAL_SVD (FChebKernel:
-        builds the global operators on first use)."""
+        """_abi.M2L_SYMMETRIC (FChebSymKernel), _abi.M2L_GLOBAL_SVD (FChebKernel:
+        builds the global operators on first use), _abi.M2L_DENSE
+        (FChebDenseKernel) or _abi.M2L_UNIFORM (FUnifKernel: the Lagrange basis
+        on equispaced roots)."""
         self._check(self._L.hfmm_set_m2l_scheme(self.h, int(scheme)))
 
     def global_operators(self):
@@ -393,6 +395,17 @@ class FChebDenseKernel(FChebSymKernel):
     def __init__(self, tree_height, box_width, box_center, order, device=0):
         super().__init__(tree_height, box_width, box_center, order, None, device, _abi.PRECISION_FP64)
         self.engine.set_m2l_scheme(_abi.M2L_DENSE)
+
+
+class FUnifKernel(FChebSymKernel):
+    """Kernel parameters of FUnifKernel<double, ..., FInterpMatrixKernelR, ORDER>
+    (Src/Kernels/Uniform/FUnifKernel.hpp:56-69): Lagrange interpolation on
+    equispaced roots, the M2L applying the kernel between the equispaced
+    nodes of target and source cells (FP64)."""
+
+    def __init__(self, tree_height, box_width, box_center, order, device=0):
+        super().__init__(tree_height, box_width, box_center, order, None, device, _abi.PRECISION_FP64)
+        self.engine.set_m2l_scheme(_abi.M2L_UNIFORM)
 
 
 class FGroupTree:
