@@ -55,7 +55,7 @@ WORKLOADS = {
     "compare20k-h5-p7": ("compare", 20_000, 5, 7, None),
 }
 # the reference's published times of that comparison (4 threads, 3 GHz Core i7)
-PUBLISHED_COMPARE = {"symmetric": 1.50791, "global": 31.937}
+PUBLISHED_COMPARE = {"symmetric": 1.50791, "global": 31.937, "uniform": 1.27096}
 SEED = {"plummer": 3, "uniform": 2, "batch": 4}  # the configurations' documented seeds
 BATCH_RHS = 8
 
@@ -354,8 +354,9 @@ def run_batch(args):
 def run_compare(args):
     """Kernel construction plus one FP64 execution on the reference's 20k
     fixture, for FChebSymKernel (the symmetric M2L, operators compressed on
-    the device) and FChebKernel (one global SVD, operators built on the host),
-    timed end to end from the host as the reference's comparison times them."""
+    the device), FChebKernel (one global SVD, operators built on the host) and
+    FUnifKernel (Lagrange basis, dense M2L between equispaced nodes), timed
+    end to end from the host as the reference's comparison times them."""
     from scalfmm_amd import _abi
     from scalfmm_amd.fma import FFmaGenericLoader
     from scalfmm_amd.fmm import Engine
@@ -368,14 +369,14 @@ def run_compare(args):
     def rel(a, b):
         return float(np.linalg.norm(a - b) / np.linalg.norm(b))
     res = {}
-    for scheme in ("symmetric", "global"):
+    for scheme in ("symmetric", "global", "uniform"):
         times = []
         for it in range(args.warmup + args.steps):
             print(f"[bench] {scheme} run {it}", file=sys.stderr, flush=True)
             t0 = time.perf_counter()
             e = Engine(h, w, c, p)
-            if scheme == "global":
-                e.set_m2l_scheme(_abi.M2L_GLOBAL_SVD)
+            if scheme != "symmetric":
+                e.set_m2l_scheme(_abi.M2L_GLOBAL_SVD if scheme == "global" else _abi.M2L_UNIFORM)
             e.set_particles(x, y, z, q)
             e.build_tree()
             e.evaluate()

@@ -190,10 +190,10 @@ int hfmm_evaluate_particle_blocks(hfmm_ctx* ctx, int32_t nblocks, const void* co
 #define HFMM_M2L_UNIFORM 3 /* FUnifKernel (Kernels/Uniform/FUnifKernel.hpp:72-175):
                               P2M, M2M, L2L and L2P interpolate on the equispaced
                               roots of FUnifRoots with Lagrange polynomials; the M2L
-                              applies the kernel between the equispaced nodes of
-                              target and source cells (the product the reference
-                              computes by a zero-padded DFT, FUnifM2LHandler.hpp),
-                              as dense rank-p^3 operators */
+                              works in Fourier space (FUnifM2LHandler.hpp:26-297):
+                              expansions zero-padded to (2p-1)^3 and DFT'd after
+                              P2M / M2M, L^ += 2/w K^_t . M^ entrywise, the inverse
+                              DFT added to the locals before L2L / L2P */
 int hfmm_set_m2l_scheme(hfmm_ctx* ctx, int32_t scheme);
 int hfmm_get_global_operators(hfmm_ctx* ctx, int32_t* rank, double* U, double* B, double* C, double* S1, double* S2);
 

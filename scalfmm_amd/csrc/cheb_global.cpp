@@ -277,4 +277,95 @@ int build_dense_operators(const ChebOperators& op, GlobalOperators& g) {
     return HFMM_OK;
 }
 
+int build_uniform_fft_operators(const ChebOperators& op, GlobalOperators& g) {
+    const int p = op.p, N = 2 * p - 1, NF = p * N * N;
+    const double pi = 3.14159265358979323846;
+    g.k = 2 * NF;
+    g.U.clear();
+    g.B.clear();
+    g.C.clear();
+    g.S1.clear();
+    g.S2.clear();
+    g.tw.assign(2 * N, 0.);
+    for (int j = 0; j < N; ++j) {
+        g.tw[2 * j] = std::cos(2. * pi * j / N);
+        g.tw[2 * j + 1] = std::sin(2. * pi * j / N);
+    }
+    g.Khat.assign(static_cast<size_t>(343) * NF * 2, 0.);
+    std::vector<int> tpos;
+    for (int i = -3; i <= 3; ++i)
+        for (int j = -3; j <= 3; ++j)
+            for (int k = -3; k <= 3; ++k)
+                if (std::abs(i) > 1 || std::abs(j) > 1 || std::abs(k) > 1) tpos.push_back((i + 3) * 49 + (j + 3) * 7 + (k + 3));
+    parallel_for(static_cast<int>(tpos.size()), [&](int t) {
+        const int pos = tpos[t];
+        const int off[3] = {2 * (pos / 49 - 3), 2 * ((pos / 7) % 7 - 3), 2 * (pos % 7 - 3)};
+        // circulant c[(dx mod N) + N ((dy mod N) + N (dz mod N))] = K for the node
+        // index difference d = target - source: target node max(d, 0), source
+        // node target - d on every axis (reference cell width 2, source at 2 off)
+        std::vector<double> c(static_cast<size_t>(N) * N * N);
+        for (int dz = -(p - 1); dz < p; ++dz)
+            for (int dy = -(p - 1); dy < p; ++dy)
+                for (int dx = -(p - 1); dx < p; ++dx) {
+                    const int d[3] = {dx, dy, dz};
+                    double r2 = 0.;
+                    for (int a = 0; a < 3; ++a) {
+                        const int tn = d[a] > 0 ? d[a] : 0, sn = tn - d[a];
+                        const double diff = op.roots[tn] - (op.roots[sn] + off[a]);
+                        r2 += diff * diff;
+                    }
+                    const int cx = (dx + N) % N, cy = (dy + N) % N, cz = (dz + N) % N;
+                    c[cx + static_cast<size_t>(N) * (cy + static_cast<size_t>(N) * cz)] = 1. / std::sqrt(r2);
+                }
+        // forward 3D DFT (w^-(a kx + b ky + c kz)), one axis at a time, kx < p kept
+        std::vector<double> t1(static_cast<size_t>(p) * N * N * 2), t2(t1.size());
+        for (int cz = 0; cz < N; ++cz)
+            for (int cy = 0; cy < N; ++cy)
+                for (int kx = 0; kx < p; ++kx) {
+                    double re = 0., im = 0.;
+                    for (int a = 0; a < N; ++a) {
+                        const double v = c[a + static_cast<size_t>(N) * (cy + static_cast<size_t>(N) * cz)];
+                        const int w = (a * kx) % N;
+                        re += v * g.tw[2 * w];
+                        im -= v * g.tw[2 * w + 1];
+                    }
+                    const size_t o = 2 * (kx + static_cast<size_t>(p) * (cy + static_cast<size_t>(N) * cz));
+                    t1[o] = re;
+                    t1[o + 1] = im;
+                }
+        for (int cz = 0; cz < N; ++cz)
+            for (int ky = 0; ky < N; ++ky)
+                for (int kx = 0; kx < p; ++kx) {
+                    double re = 0., im = 0.;
+                    for (int b = 0; b < N; ++b) {
+                        const size_t o = 2 * (kx + static_cast<size_t>(p) * (b + static_cast<size_t>(N) * cz));
+                        const int w = (b * ky) % N;
+                        const double wr = g.tw[2 * w], wi = g.tw[2 * w + 1];
+                        re += t1[o] * wr + t1[o + 1] * wi;
+                        im += t1[o + 1] * wr - t1[o] * wi;
+                    }
+                    const size_t o = 2 * (kx + static_cast<size_t>(p) * (ky + static_cast<size_t>(N) * cz));
+                    t2[o] = re;
+                    t2[o + 1] = im;
+                }
+        double* K = g.Khat.data() + static_cast<size_t>(pos) * NF * 2;
+        for (int kz = 0; kz < N; ++kz)
+            for (int ky = 0; ky < N; ++ky)
+                for (int kx = 0; kx < p; ++kx) {
+                    double re = 0., im = 0.;
+                    for (int cc = 0; cc < N; ++cc) {
+                        const size_t o = 2 * (kx + static_cast<size_t>(p) * (ky + static_cast<size_t>(N) * cc));
+                        const int w = (cc * kz) % N;
+                        const double wr = g.tw[2 * w], wi = g.tw[2 * w + 1];
+                        re += t2[o] * wr + t2[o + 1] * wi;
+                        im += t2[o + 1] * wr - t2[o] * wi;
+                    }
+                    const size_t o = 2 * (kx + static_cast<size_t>(p) * (ky + static_cast<size_t>(N) * kz));
+                    K[o] = re;
+                    K[o + 1] = im;
+                }
+    });
+    return HFMM_OK;
+}
+
 }  // namespace hfmm

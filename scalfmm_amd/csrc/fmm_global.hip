@@ -99,9 +99,240 @@ __global__ __launch_bounds__(64) void k_global_m2l(int64_t ncell, const int32_t*
     }
 }
 
+// ---------------------------------------------------------------------------
+// The uniform scheme (FUnifKernel): expansions in Fourier space.  A cell's p^3
+// node values, zero-padded to N^3 (N = 2p-1), are 3D-DFT'd; real input, so
+// only kx < p is kept: NF = p N^2 complex entries, index kx + p (ky + N kz).
+// One 64-lane wave per cell, the three axes one after the other through LDS.
+// tw[j] = (cos 2 pi j/N, sin 2 pi j/N).
+
+// M^ = DFT(pad(M)) (FUnifM2LHandler::applyZeroPaddingAndDFT), overwrite.
+template <int P>
+__global__ __launch_bounds__(64) void k_unif_forward(int64_t ncell, const double* __restrict__ M, int LD,
+                                                     const double2* __restrict__ tw, double2* __restrict__ Mh) {
+    constexpr int N = 2 * P - 1, NN = P * P * P, NF = P * N * N;
+    __shared__ double sM[NN];
+    __shared__ double2 sA[NN];         // [kx][b][c]: kx + P (b + P c)
+    __shared__ double2 sB[P * N * P];  // [kx][ky][c]: kx + P (ky + N c)
+    __shared__ double2 sW[N];
+    const int64_t cell = blockIdx.x;
+    if (cell >= ncell) return;
+    const int lane = threadIdx.x;
+    if (lane < N) sW[lane] = tw[lane];
+    for (int i = lane; i < NN; i += 64) sM[i] = M[cell * LD + i];
+    __syncthreads();
+    for (int i = lane; i < NN; i += 64) {  // x: real -> complex
+        const int kx = i % P, bc = i / P;
+        double re = 0., im = 0.;
+#pragma unroll
+        for (int a = 0; a < P; ++a) {
+            const double m = sM[a + P * bc];
+            const double2 w = sW[(a * kx) % N];
+            re += m * w.x;
+            im -= m * w.y;
+        }
+        sA[i] = make_double2(re, im);
+    }
+    __syncthreads();
+    for (int i = lane; i < P * N * P; i += 64) {  // y
+        const int kx = i % P, ky = (i / P) % N, cc = i / (P * N);
+        double re = 0., im = 0.;
+#pragma unroll
+        for (int b = 0; b < P; ++b) {
+            const double2 v = sA[kx + P * (b + P * cc)];
+            const double2 w = sW[(b * ky) % N];
+            re += v.x * w.x + v.y * w.y;
+            im += v.y * w.x - v.x * w.y;
+        }
+        sB[i] = make_double2(re, im);
+    }
+    __syncthreads();
+    for (int i = lane; i < NF; i += 64) {  // z
+        const int kxy = i % (P * N), kz = i / (P * N);
+        double re = 0., im = 0.;
+#pragma unroll
+        for (int cc = 0; cc < P; ++cc) {
+            const double2 v = sB[kxy + P * N * cc];
+            const double2 w = sW[(cc * kz) % N];
+            re += v.x * w.x + v.y * w.y;
+            im += v.y * w.x - v.x * w.y;
+        }
+        Mh[cell * NF + i] = make_double2(re, im);
+    }
+}
+
+// L += IDFT(L^) restricted to the p^3 nodes, over N^3
+// (FUnifM2LHandler::unapplyZeroPaddingAndDFT + FBlas::add).
+template <int P>
+__global__ __launch_bounds__(64) void k_unif_inverse(int64_t ncell, const double2* __restrict__ Lh,
+                                                     const double2* __restrict__ tw, double* __restrict__ L, int LD) {
+    constexpr int N = 2 * P - 1, NN = P * P * P, NF = P * N * N;
+    __shared__ double2 sF[NF];
+    __shared__ double2 sH[P * N * P];  // [kx][ky][c]
+    __shared__ double2 sG[NN];         // [kx][b][c]
+    __shared__ double2 sW[N];
+    const int64_t cell = blockIdx.x;
+    if (cell >= ncell) return;
+    const int lane = threadIdx.x;
+    if (lane < N) sW[lane] = tw[lane];
+    for (int i = lane; i < NF; i += 64) sF[i] = Lh[cell * NF + i];
+    __syncthreads();
+    for (int i = lane; i < P * N * P; i += 64) {  // z, outputs c < p
+        const int kxy = i % (P * N), cc = i / (P * N);
+        double re = 0., im = 0.;
+#pragma unroll
+        for (int kz = 0; kz < N; ++kz) {
+            const double2 v = sF[kxy + P * N * kz];
+            const double2 w = sW[(cc * kz) % N];
+            re += v.x * w.x - v.y * w.y;
+            im += v.x * w.y + v.y * w.x;
+        }
+        sH[i] = make_double2(re, im);
+    }
+    __syncthreads();
+    for (int i = lane; i < NN; i += 64) {  // y, outputs b < p
+        const int kx = i % P, b = (i / P) % P, cc = i / (P * P);
+        double re = 0., im = 0.;
+#pragma unroll
+        for (int ky = 0; ky < N; ++ky) {
+            const double2 v = sH[kx + P * (ky + N * cc)];
+            const double2 w = sW[(b * ky) % N];
+            re += v.x * w.x - v.y * w.y;
+            im += v.x * w.y + v.y * w.x;
+        }
+        sG[i] = make_double2(re, im);
+    }
+    __syncthreads();
+    constexpr double inv = 1. / (static_cast<double>(N) * N * N);
+    for (int i = lane; i < NN; i += 64) {  // x: the Hermitian half back to real
+        const int a = i % P, bc = i / P;
+        double v = sG[P * bc].x;
+        double h = 0.;
+#pragma unroll
+        for (int kx = 1; kx < P; ++kx) {
+            const double2 g = sG[kx + P * bc];
+            const double2 w = sW[(a * kx) % N];
+            h += g.x * w.x - g.y * w.y;
+        }
+        v += 2. * h;
+        L[cell * LD + i] += v * inv;
+    }
+}
+
+// L^[cell] += scale K^_pos . M^[src] entrywise (FUnifM2LHandler::applyFC)
+// over the cell's 189 slots in slot order.
+template <int P>
+__global__ __launch_bounds__(64) void k_unif_m2l(int64_t ncell, const int32_t* __restrict__ slots,
+                                                 const uint64_t* __restrict__ keys,
+                                                 const int16_t* __restrict__ slotpos,
+                                                 const double2* __restrict__ K, const double2* __restrict__ Mh,
+                                                 double scale, double2* __restrict__ Lh) {
+    constexpr int N = 2 * P - 1, NF = P * N * N, R = (NF + 63) / 64;
+    const int64_t cell = blockIdx.x;
+    if (cell >= ncell) return;
+    const int lane = threadIdx.x;
+    const int par = static_cast<int>(keys[cell] & 7u);
+    double2 acc[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) acc[r] = make_double2(0., 0.);
+    for (int s = 0; s < kFarSlots; ++s) {
+        const int src = slots[cell * kFarSlots + s];
+        if (src < 0) continue;  // the same for every lane
+        const double2* Kp = K + static_cast<int64_t>(slotpos[par * kFarSlots + s]) * NF;
+        const double2* m = Mh + static_cast<int64_t>(src) * NF;
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const int f = lane + 64 * r;
+            if (f < NF) {
+                const double2 k = Kp[f], x = m[f];
+                acc[r].x += k.x * x.x - k.y * x.y;
+                acc[r].y += k.x * x.y + k.y * x.x;
+            }
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const int f = lane + 64 * r;
+        if (f < NF) {
+            double2 o = Lh[cell * NF + f];
+            o.x += scale * acc[r].x;
+            o.y += scale * acc[r].y;
+            Lh[cell * NF + f] = o;
+        }
+    }
+}
+
+#define HFMM_UNIF_ORDERS(X) X(5) X(6) X(7) X(8)
+
+int run_unif_forward(Context* c, int l) {
+    const int64_t nc = c->ncell[l];
+    if (nc == 0) return HFMM_OK;
+    const dim3 grid(static_cast<unsigned>(nc));
+    const double* M = c->d_M.as<double>() + c->lvl_off[l] * c->LD;
+    double2* Mh = reinterpret_cast<double2*>(c->d_Mt.as<double>() + c->lvl_off[l] * c->glob.k);
+    const double2* tw = reinterpret_cast<const double2*>(c->d_utw.as<double>());
+    prof_begin(c, "unif_forward");
+    switch (c->ops.p) {
+#define CASE(PP) case PP: hipLaunchKernelGGL(k_unif_forward<PP>, grid, dim3(64), 0, c->stream, nc, M, c->LD, tw, Mh); break;
+        HFMM_UNIF_ORDERS(CASE)
+#undef CASE
+        default: return fail(c, HFMM_ERR_ARG, "uniform M2L: unsupported order");
+    }
+    prof_end(c);
+    return check_launch(c, "unif_forward");
+}
+
+int run_unif_inverse(Context* c, int l) {
+    const int64_t nc = c->ncell[l];
+    if (nc == 0) return HFMM_OK;
+    const dim3 grid(static_cast<unsigned>(nc));
+    const double2* Lh = reinterpret_cast<const double2*>(c->d_Lt.as<double>() + c->lvl_off[l] * c->glob.k);
+    double* L = c->d_L.as<double>() + c->lvl_off[l] * c->LD;
+    const double2* tw = reinterpret_cast<const double2*>(c->d_utw.as<double>());
+    prof_begin(c, "unif_inverse");
+    switch (c->ops.p) {
+#define CASE(PP) case PP: hipLaunchKernelGGL(k_unif_inverse<PP>, grid, dim3(64), 0, c->stream, nc, Lh, tw, L, c->LD); break;
+        HFMM_UNIF_ORDERS(CASE)
+#undef CASE
+        default: return fail(c, HFMM_ERR_ARG, "uniform M2L: unsupported order");
+    }
+    prof_end(c);
+    return check_launch(c, "unif_inverse");
+}
+
+int run_unif_m2l(Context* c) {
+    const int L = c->height() - 1;
+    const int lo = c->work_upper > c->top() ? c->work_upper : c->top();
+    const int hi = c->work_lower - 1 < L ? c->work_lower - 1 : L;
+    const int k = c->glob.k;
+    prof_begin(c, "unif_m2l");
+    for (int l = lo; l <= hi; ++l) {
+        const int64_t nc = c->ncell[l];
+        if (nc == 0) continue;
+        const double scale = 2. / (c->P.box_width / static_cast<double>(int64_t(1) << (l - c->set_levels)));
+        const int32_t* slots = c->d_m2l.as<int32_t>() + static_cast<size_t>(c->m2l_off[l]) * kFarSlots;
+        const uint64_t* keys = c->d_cellkey.as<uint64_t>() + c->lvl_off[l];
+        const double2* Mh = reinterpret_cast<const double2*>(c->d_Mt.as<double>() + c->lvl_off[l] * k);
+        double2* Lh = reinterpret_cast<double2*>(c->d_Lt.as<double>() + c->lvl_off[l] * k);
+        const int16_t* sp = c->d_slotpos.as<int16_t>();
+        const double2* K = reinterpret_cast<const double2*>(c->d_uK.as<double>());
+        const dim3 grid(static_cast<unsigned>(nc));
+        switch (c->ops.p) {
+#define CASE(PP) case PP: hipLaunchKernelGGL(k_unif_m2l<PP>, grid, dim3(64), 0, c->stream, nc, slots, keys, sp, K, Mh, scale, Lh); break;
+            HFMM_UNIF_ORDERS(CASE)
+#undef CASE
+            default: return fail(c, HFMM_ERR_ARG, "uniform M2L: unsupported order");
+        }
+        HFMM_TRY(check_launch(c, "unif_m2l"));
+    }
+    prof_end(c);
+    return HFMM_OK;
+}
+
 }  // namespace
 
 int run_global_compress(Context* c, int l) {
+    if (c->m2l_scheme == HFMM_M2L_UNIFORM) return run_unif_forward(c, l);
     const int64_t nc = c->ncell[l];
     if (nc == 0) return HFMM_OK;
     const int k = c->glob.k, R = (k + 63) / 64;
@@ -122,6 +353,7 @@ int run_global_compress(Context* c, int l) {
 }
 
 int run_global_decompress(Context* c, int l) {
+    if (c->m2l_scheme == HFMM_M2L_UNIFORM) return run_unif_inverse(c, l);
     const int64_t nc = c->ncell[l];
     if (nc == 0) return HFMM_OK;
     const int k = c->glob.k;
@@ -134,6 +366,7 @@ int run_global_decompress(Context* c, int l) {
 }
 
 int run_global_m2l(Context* c) {
+    if (c->m2l_scheme == HFMM_M2L_UNIFORM) return run_unif_m2l(c);
     const int k = c->glob.k, R = (k + 63) / 64;
     if (R > kMaxRankRows) return fail(c, HFMM_ERR_ARG, "global M2L rank above 512");
     const int L = c->height() - 1;

@@ -206,7 +206,7 @@ static void free_all(Context* c) {
                       &c->d_scan_part, &c->d_qin, &c->d_flags, &c->d_scan, &c->d_sorted, &c->d_perm, &c->d_pleaf,
                       &c->d_count, &c->d_cellkey, &c->d_cellkey_tmp, &c->d_childstart, &c->d_leafstart,
                       &c->d_p2p, &c->d_p2pitems, &c->d_m2l, &c->d_order, &c->d_pcount, &c->d_dense, &c->d_M, &c->d_L, &c->d_Mb, &c->d_loc, &c->d_res_sorted,
-                      &c->d_res, &c->d_field, &c->d_setoff, &c->d_keep, &c->d_kept, &c->d_gBt, &c->d_gU, &c->d_gC, &c->d_Mt, &c->d_Lt, &c->d_uroots};
+                      &c->d_res, &c->d_field, &c->d_setoff, &c->d_keep, &c->d_kept, &c->d_gBt, &c->d_gU, &c->d_gC, &c->d_Mt, &c->d_Lt, &c->d_uroots, &c->d_uK, &c->d_utw};
     for (DevBuf* b : bufs) release(*b);
 }
 
@@ -863,23 +863,30 @@ int hfmm_set_m2l_scheme(hfmm_ctx* h, int32_t scheme) {
                 u.child.assign(static_cast<size_t>(8) * 3 * u.p * u.p, 0.);
                 if (hfmm::make_uniform_tables(u) != HFMM_OK)
                     return hfmm::fail(c, HFMM_ERR_ARG, "no equispaced roots for this order");
-                st = hfmm::build_dense_operators(u, g);
+                st = hfmm::build_uniform_fft_operators(u, g);
             } else {
                 st = scheme == HFMM_M2L_DENSE ? hfmm::build_dense_operators(c->ops, g)
                                               : hfmm::build_global_operators(c->ops, g, err);
             }
             if (st != HFMM_OK) return hfmm::fail(c, st, err);
-            if (g.k > 512) return hfmm::fail(c, HFMM_ERR_ARG, "global M2L rank above 512");
-            const int nn = c->ops.nn, k = g.k;
-            std::vector<double> bt(static_cast<size_t>(nn) * k);
-            for (int a = 0; a < k; ++a)
-                for (int n = 0; n < nn; ++n) bt[static_cast<size_t>(n) * k + a] = g.B[static_cast<size_t>(a) * nn + n];
-            HFMM_TRY(hfmm::ensure(c, c->d_gBt, sizeof(double) * bt.size()));
-            HFMM_TRY(hfmm::ensure(c, c->d_gU, sizeof(double) * g.U.size()));
-            HFMM_TRY(hfmm::ensure(c, c->d_gC, sizeof(double) * g.C.size()));
-            HFMM_HIP(c, hipMemcpy(c->d_gBt.ptr, bt.data(), sizeof(double) * bt.size(), hipMemcpyHostToDevice));
-            HFMM_HIP(c, hipMemcpy(c->d_gU.ptr, g.U.data(), sizeof(double) * g.U.size(), hipMemcpyHostToDevice));
-            HFMM_HIP(c, hipMemcpy(c->d_gC.ptr, g.C.data(), sizeof(double) * g.C.size(), hipMemcpyHostToDevice));
+            if (scheme == HFMM_M2L_UNIFORM) {
+                HFMM_TRY(hfmm::ensure(c, c->d_uK, sizeof(double) * g.Khat.size()));
+                HFMM_TRY(hfmm::ensure(c, c->d_utw, sizeof(double) * g.tw.size()));
+                HFMM_HIP(c, hipMemcpy(c->d_uK.ptr, g.Khat.data(), sizeof(double) * g.Khat.size(), hipMemcpyHostToDevice));
+                HFMM_HIP(c, hipMemcpy(c->d_utw.ptr, g.tw.data(), sizeof(double) * g.tw.size(), hipMemcpyHostToDevice));
+            } else {
+                if (g.k > 512) return hfmm::fail(c, HFMM_ERR_ARG, "global M2L rank above 512");
+                const int nn = c->ops.nn, k = g.k;
+                std::vector<double> bt(static_cast<size_t>(nn) * k);
+                for (int a = 0; a < k; ++a)
+                    for (int n = 0; n < nn; ++n) bt[static_cast<size_t>(n) * k + a] = g.B[static_cast<size_t>(a) * nn + n];
+                HFMM_TRY(hfmm::ensure(c, c->d_gBt, sizeof(double) * bt.size()));
+                HFMM_TRY(hfmm::ensure(c, c->d_gU, sizeof(double) * g.U.size()));
+                HFMM_TRY(hfmm::ensure(c, c->d_gC, sizeof(double) * g.C.size()));
+                HFMM_HIP(c, hipMemcpy(c->d_gBt.ptr, bt.data(), sizeof(double) * bt.size(), hipMemcpyHostToDevice));
+                HFMM_HIP(c, hipMemcpy(c->d_gU.ptr, g.U.data(), sizeof(double) * g.U.size(), hipMemcpyHostToDevice));
+                HFMM_HIP(c, hipMemcpy(c->d_gC.ptr, g.C.data(), sizeof(double) * g.C.size(), hipMemcpyHostToDevice));
+            }
             c->glob = std::move(g);
             // the interpolation tables of the basis the scheme works in
             const hfmm::ChebOperators& tab = scheme == HFMM_M2L_UNIFORM ? u : c->ops;
@@ -905,7 +912,8 @@ int hfmm_set_m2l_scheme(hfmm_ctx* h, int32_t scheme) {
 int hfmm_get_global_operators(hfmm_ctx* h, int32_t* rank, double* U, double* B, double* C, double* S1, double* S2) {
     CTX_CHECK(h);
     Context* c = &h->c;
-    if (c->glob.k == 0) return hfmm::fail(c, HFMM_ERR_STATE, "no global-SVD or dense M2L operators are built");
+    if (c->glob.k == 0 || c->m2l_scheme == HFMM_M2L_UNIFORM)
+        return hfmm::fail(c, HFMM_ERR_STATE, "no global-SVD or dense M2L operators are built");
     if (rank) *rank = c->glob.k;
     if (U) std::memcpy(U, c->glob.U.data(), sizeof(double) * c->glob.U.size());
     if (B) std::memcpy(B, c->glob.B.data(), sizeof(double) * c->glob.B.size());

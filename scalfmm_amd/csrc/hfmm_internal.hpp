@@ -40,12 +40,23 @@ struct KernelTimer {
 // nn x k column-major (un-weighted), C [343][k x k] column-major, S1 / S2 the
 // singular values of K_col / K_row.
 struct GlobalOperators {
-    int k = 0;
+    int k = 0;  // doubles per compressed expansion (the uniform scheme: 2 NF)
     std::vector<double> U, B, C, S1, S2;
+    // the uniform scheme: kernel spectra [343][NF] complex (zero at the near
+    // positions) and the twiddles w^j = (cos 2 pi j/N, sin 2 pi j/N), N = 2p-1
+    std::vector<double> Khat, tw;
 };
 int build_global_operators(const ChebOperators& op, GlobalOperators& g, std::string& err);
 // FChebDenseKernel's operators: rank p^3, identity bases, C_t = K_t
 int build_dense_operators(const ChebOperators& op, GlobalOperators& g);
+// FUnifKernel's M2L in Fourier space (Kernels/Uniform/FUnifM2LHandler.hpp:26-
+// 120, 244-297): the kernel between the equispaced nodes of two cells depends
+// on the node index difference only, so each far-field operator is a 3-level
+// Toeplitz matrix, embedded in a 3D circulant of side N = 2p-1 and applied as
+// a pointwise product of 3D DFTs.  Builds the kernel spectra (half spectrum:
+// kx < p, NF = p N^2 entries per position, index kx + p (ky + N kz)) of the
+// uniform basis op and the twiddle table; g.k = 2 NF.
+int build_uniform_fft_operators(const ChebOperators& op, GlobalOperators& g);
 
 struct Context {
     hfmm_params P{};
@@ -148,6 +159,8 @@ struct Context {
     // (HFMM_M2L_UNIFORM: d_child then holds the Lagrange child tables)
     int basis = 0;
     DevBuf d_uroots;          // [p] the equispaced roots of the uniform basis
+    DevBuf d_uK;              // the uniform scheme's kernel spectra [343][NF] complex
+    DevBuf d_utw;             // the uniform scheme's twiddles [N] complex
     GlobalOperators glob;
     DevBuf d_gBt;             // B^T as [nn][k] (the compression reads a row per node)
     DevBuf d_gU;              // U as [k][nn]
