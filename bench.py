@@ -375,9 +375,8 @@ def run_compare(args):
         for it in range(args.warmup + args.steps):
             print(f"[bench] {scheme} run {it}", file=sys.stderr, flush=True)
             t0 = time.perf_counter()
-            e = Engine(h, w, c, p)
-            if scheme != "symmetric":
-                e.set_m2l_scheme(_abi.M2L_GLOBAL_SVD if scheme == "global" else _abi.M2L_UNIFORM)
+            e = Engine(h, w, c, p, m2l_scheme={"symmetric": _abi.M2L_SYMMETRIC, "global": _abi.M2L_GLOBAL_SVD,
+                                                "uniform": _abi.M2L_UNIFORM}[scheme])
             e.set_particles(x, y, z, q)
             e.build_tree()
             e.evaluate()

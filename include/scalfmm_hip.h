@@ -79,12 +79,18 @@ typedef struct hfmm_params {
     double epsilon;        /* M2L compression accuracy; <= 0 selects 10^-order */
     int32_t precision;     /* HFMM_PRECISION_* */
     int32_t device;        /* HIP device ordinal */
+    int32_t m2l_scheme;    /* HFMM_M2L_* the context starts with (0, the symmetric
+                              scheme, when zero-initialised); see hfmm_set_m2l_scheme */
 } hfmm_params;
 
 /* Creates a context: builds the Chebyshev roots, child/parent interpolators,
  * the 343 symmetry permutations and the 16 compressed M2L operators
  * (replaces the FChebSymKernel constructor, FChebSymKernel.hpp:110-126 and
- * SymmetryHandler, FChebSymM2LHandler.hpp:321-344). */
+ * SymmetryHandler, FChebSymM2LHandler.hpp:321-344).  A context created with
+ * another M2L scheme builds that scheme's operators instead (the FChebKernel,
+ * FChebDenseKernel and FUnifKernel constructors) and compresses the 16
+ * symmetric operators only when it switches to the symmetric scheme or
+ * hfmm_get_operator / hfmm_save_operators ask for them. */
 int hfmm_create(const hfmm_params* params, hfmm_ctx** out);
 int hfmm_destroy(hfmm_ctx* ctx);
 /* Message of the last failure on this context (never NULL). */

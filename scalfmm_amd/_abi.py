@@ -56,7 +56,7 @@ _NON_STATUS = ("hfmm_last_error", "hfmm_version", "hfmm_batch_last_error", "hfmm
 class Params(C.Structure):
     _fields_ = [("tree_height", C.c_int32), ("order", C.c_int32), ("box_width", C.c_double),
                 ("box_center", C.c_double * 3), ("epsilon", C.c_double), ("precision", C.c_int32),
-                ("device", C.c_int32)]
+                ("device", C.c_int32), ("m2l_scheme", C.c_int32)]
 
 
 class TreeInfo(C.Structure):

@@ -88,6 +88,7 @@ int hfmm_batch_create(const hfmm_params* params, int32_t nsets, const int64_t* s
         if (set_heights && (set_heights[s] < 3 || set_heights[s] > 20)) return HFMM_ERR_ARG;
     }
     if (set_offsets[nsets] <= 0) return HFMM_ERR_ARG;
+    if (params->m2l_scheme != HFMM_M2L_SYMMETRIC) return HFMM_ERR_ARG;  // batches run the symmetric scheme
     hfmm_batch* b = new (std::nothrow) hfmm_batch();
     if (!b) return HFMM_ERR_NOMEM;
     b->P = *params;

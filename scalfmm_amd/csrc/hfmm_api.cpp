@@ -118,6 +118,16 @@ static int upload_operators(Context* c) {
         for (int n = 0; n < nn; ++n) pvp[static_cast<size_t>(pos) * c->LD + n] = op.pvec[static_cast<size_t>(pos) * nn + n];
     HFMM_TRY(ensure(c, c->d_pvec, sizeof(uint16_t) * pvp.size()));
     HFMM_HIP(c, hipMemcpy(c->d_pvec.ptr, pvp.data(), sizeof(uint16_t) * pvp.size(), hipMemcpyHostToDevice));
+    std::vector<int16_t> sp(8 * kFarSlots);
+    for (int e = 0; e < 8; ++e) {
+        FarSlot s[kFarSlots];
+        far_slots(e, s);
+        for (int j = 0; j < kFarSlots; ++j) sp[e * kFarSlots + j] = s[j].pos;
+    }
+    HFMM_TRY(ensure(c, c->d_slotpos, sizeof(int16_t) * sp.size()));
+    HFMM_HIP(c, hipMemcpy(c->d_slotpos.ptr, sp.data(), sizeof(int16_t) * sp.size(), hipMemcpyHostToDevice));
+    // the rest serves the symmetric scheme's compressed operators only
+    if (!c->sym_compressed) return HFMM_OK;
     std::vector<int32_t> rank(343, 0);
     std::vector<int64_t> off(343, 0);
     std::vector<double> packed;
@@ -146,14 +156,6 @@ static int upload_operators(Context* c) {
     HFMM_HIP(c, hipMemcpy(c->d_opoff.ptr, off.data(), sizeof(int64_t) * 343, hipMemcpyHostToDevice));
     HFMM_TRY(ensure(c, c->d_ops, sizeof(double) * packed.size()));
     HFMM_HIP(c, hipMemcpy(c->d_ops.ptr, packed.data(), sizeof(double) * packed.size(), hipMemcpyHostToDevice));
-    std::vector<int16_t> sp(8 * kFarSlots);
-    for (int e = 0; e < 8; ++e) {
-        FarSlot s[kFarSlots];
-        far_slots(e, s);
-        for (int j = 0; j < kFarSlots; ++j) sp[e * kFarSlots + j] = s[j].pos;
-    }
-    HFMM_TRY(ensure(c, c->d_slotpos, sizeof(int16_t) * sp.size()));
-    HFMM_HIP(c, hipMemcpy(c->d_slotpos.ptr, sp.data(), sizeof(int16_t) * sp.size(), hipMemcpyHostToDevice));
     // slots of every parity ordered by the rank tiles of their operator, so the
     // M2L kernels run one specialised loop per rank-tile count.  Inside a
     // rank-tile group the slots are ordered by the parity class of their
@@ -270,10 +272,17 @@ int create_context(const hfmm_params* params, hipStream_t shared_stream, hfmm_ct
         return HFMM_ERR_DEVICE;
     }
     std::string err;
+    const int scheme = params->m2l_scheme;
     int st = hfmm::build_cheb_operators(params->order, params->epsilon, c->ops, err);
-    if (st == HFMM_OK) st = hfmm::compress_operators_device(c);
-    if (st == HFMM_OK) st = hfmm::upload_operators(c);
+    if (st == HFMM_OK && scheme != HFMM_M2L_SYMMETRIC &&
+        (scheme < HFMM_M2L_GLOBAL_SVD || scheme > HFMM_M2L_UNIFORM || params->precision != HFMM_PRECISION_FP64))
+        st = HFMM_ERR_ARG;  // unknown scheme, or a non-symmetric one outside FP64
+    // the symmetric scheme's operators now, the others' below
+    if (st == HFMM_OK && scheme == HFMM_M2L_SYMMETRIC) st = hfmm::compress_operators_device(c);
+    c->sym_compressed = c->sym_uploaded = scheme == HFMM_M2L_SYMMETRIC;
+    if (st == HFMM_OK) st = hfmm::upload_operators(c);  // rank-0 operators until compressed
     if (st == HFMM_OK && hfmm::is_mixed(c)) st = hfmm::prepare_mixed_operators(c);
+    if (st == HFMM_OK && scheme != HFMM_M2L_SYMMETRIC) st = hfmm_set_m2l_scheme(h, scheme);
     if (st != HFMM_OK) {
         hfmm::free_all(c);
         hfmm::destroy_side(c);
@@ -780,9 +789,20 @@ int hfmm_get_m2l_counts(hfmm_ctx* h, int32_t level, int64_t* counts) {
     return HFMM_OK;
 }
 
+// The 16 symmetric operators on the host, compressed on first need by a
+// context created with another M2L scheme.
+static int ensure_symmetric_compressed(Context* c) {
+    if (c->sym_compressed) return HFMM_OK;
+    HFMM_HIP(c, hipSetDevice(c->device));
+    HFMM_TRY(hfmm::compress_operators_device(c));
+    c->sym_compressed = true;
+    return HFMM_OK;
+}
+
 int hfmm_save_operators(hfmm_ctx* h, const char* path) {
     CTX_CHECK(h);
     if (!path) return hfmm::fail(&h->c, HFMM_ERR_ARG, "save_operators: no path");
+    HFMM_TRY(ensure_symmetric_compressed(&h->c));
     return hfmm::save_operators(&h->c, path);
 }
 
@@ -799,7 +819,12 @@ int hfmm_load_operators(hfmm_ctx* h, const char* path) {
     HFMM_HIP(c, hipStreamSynchronize(c->stream));
     c->ops.rank = rank;
     c->ops.K = std::move(K);
-    HFMM_TRY(hfmm::upload_operators(c));
+    c->sym_compressed = true;
+    c->sym_uploaded = false;
+    if (c->m2l_scheme == HFMM_M2L_SYMMETRIC) {  // otherwise uploaded on a switch to it
+        HFMM_TRY(hfmm::upload_operators(c));
+        c->sym_uploaded = true;
+    }
     if (hfmm::is_mixed(c)) HFMM_TRY(hfmm::prepare_mixed_operators(c));
     c->m2l_variant = -1;  // other operators: tune again
     c->evaluated = false;
@@ -810,6 +835,7 @@ int hfmm_get_operator(hfmm_ctx* h, int32_t idx, int32_t* rank, double* K) {
     CTX_CHECK(h);
     Context* c = &h->c;
     if (idx < 0 || idx >= 343) return hfmm::fail(c, HFMM_ERR_ARG, "bad operator index");
+    HFMM_TRY(ensure_symmetric_compressed(c));
     const int k = c->ops.rank[idx];
     if (rank) *rank = k;
     if (K && k) std::memcpy(K, c->ops.K[idx].data(), sizeof(double) * c->ops.K[idx].size());
@@ -898,9 +924,16 @@ int hfmm_set_m2l_scheme(hfmm_ctx* h, int32_t scheme) {
             }
             c->basis = scheme == HFMM_M2L_UNIFORM ? 1 : 0;
         }
-    } else if (c->basis != 0) {
-        HFMM_HIP(c, hipMemcpy(c->d_child.ptr, c->ops.child.data(), sizeof(double) * c->ops.child.size(),
-                              hipMemcpyHostToDevice));
+    } else {
+        HFMM_TRY(ensure_symmetric_compressed(c));
+        if (!c->sym_uploaded) {  // also restores the Chebyshev tables
+            HFMM_HIP(c, hipStreamSynchronize(c->stream));
+            HFMM_TRY(hfmm::upload_operators(c));
+            c->sym_uploaded = true;
+        } else if (c->basis != 0) {
+            HFMM_HIP(c, hipMemcpy(c->d_child.ptr, c->ops.child.data(), sizeof(double) * c->ops.child.size(),
+                                  hipMemcpyHostToDevice));
+        }
         c->basis = 0;
     }
     if (scheme != c->m2l_scheme) c->cells_dirty = true;

@@ -154,6 +154,9 @@ struct Context {
     // M2L scheme: HFMM_M2L_SYMMETRIC (FChebSymKernel) or HFMM_M2L_GLOBAL_SVD
     // (FChebKernel: compressed multipoles / locals of rank k beside M and L)
     int m2l_scheme = 0;
+    // the 16 symmetric operators: compressed (host, c->ops.K) and uploaded
+    // (d_ops, d_rank, ...); a context created with another scheme defers both
+    bool sym_compressed = false, sym_uploaded = false;
     // interpolation basis of P2M, M2M, L2L and L2P: 0 = Chebyshev (the
     // symmetric, global-SVD and dense schemes), 1 = uniform Lagrange
     // (HFMM_M2L_UNIFORM: d_child then holds the Lagrange child tables)

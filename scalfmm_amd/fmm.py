@@ -50,7 +50,7 @@ class Engine:
     """One device context: operators, particles, tree and expansions."""
 
     def __init__(self, tree_height, box_width, box_center, order, epsilon=0.0, device=0,
-                 precision=_abi.PRECISION_FP64):
+                 precision=_abi.PRECISION_FP64, m2l_scheme=_abi.M2L_SYMMETRIC):
         self._L = _abi.lib()
         p = _abi.Params()
         p.tree_height = int(tree_height)
@@ -61,6 +61,7 @@ class Engine:
         p.epsilon = float(epsilon or 0.0)
         p.precision = int(precision)
         p.device = int(device)
+        p.m2l_scheme = int(m2l_scheme)
         h = C.c_void_p()
         rc = self._L.hfmm_create(C.byref(p), C.byref(h))
         if rc:
@@ -366,7 +367,7 @@ class FChebSymKernel:
     """Kernel parameters of FChebSymKernel<double, ..., FInterpMatrixKernelR, ORDER>."""
 
     def __init__(self, tree_height, box_width, box_center, order, epsilon=None, device=0,
-                 precision=_abi.PRECISION_FP64):
+                 precision=_abi.PRECISION_FP64, m2l_scheme=_abi.M2L_SYMMETRIC):
         self.tree_height = int(tree_height)
         self.box_width = float(box_width)
         self.box_center = tuple(float(c) for c in box_center)
@@ -374,7 +375,8 @@ class FChebSymKernel:
         self.epsilon = float(epsilon) if epsilon else 0.0
         self.device = device
         self.precision = int(precision)
-        self.engine = Engine(tree_height, box_width, box_center, order, self.epsilon, device, self.precision)
+        self.engine = Engine(tree_height, box_width, box_center, order, self.epsilon, device, self.precision,
+                             m2l_scheme)
 
 
 class FChebKernel(FChebSymKernel):
@@ -383,8 +385,8 @@ class FChebKernel(FChebSymKernel):
     M2L through one global SVD of the 316 far-field operators (FP64)."""
 
     def __init__(self, tree_height, box_width, box_center, order, epsilon=None, device=0):
-        super().__init__(tree_height, box_width, box_center, order, epsilon, device, _abi.PRECISION_FP64)
-        self.engine.set_m2l_scheme(_abi.M2L_GLOBAL_SVD)
+        super().__init__(tree_height, box_width, box_center, order, epsilon, device, _abi.PRECISION_FP64,
+                         _abi.M2L_GLOBAL_SVD)
 
 
 class FChebDenseKernel(FChebSymKernel):
@@ -393,8 +395,8 @@ class FChebDenseKernel(FChebSymKernel):
     applied uncompressed (FP64)."""
 
     def __init__(self, tree_height, box_width, box_center, order, device=0):
-        super().__init__(tree_height, box_width, box_center, order, None, device, _abi.PRECISION_FP64)
-        self.engine.set_m2l_scheme(_abi.M2L_DENSE)
+        super().__init__(tree_height, box_width, box_center, order, None, device, _abi.PRECISION_FP64,
+                         _abi.M2L_DENSE)
 
 
 class FUnifKernel(FChebSymKernel):
@@ -404,8 +406,8 @@ class FUnifKernel(FChebSymKernel):
     nodes of target and source cells (FP64)."""
 
     def __init__(self, tree_height, box_width, box_center, order, device=0):
-        super().__init__(tree_height, box_width, box_center, order, None, device, _abi.PRECISION_FP64)
-        self.engine.set_m2l_scheme(_abi.M2L_UNIFORM)
+        super().__init__(tree_height, box_width, box_center, order, None, device, _abi.PRECISION_FP64,
+                         _abi.M2L_UNIFORM)
 
 
 class FGroupTree:

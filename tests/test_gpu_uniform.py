@@ -177,3 +177,47 @@ def test_utest_lagrange_unif_kernel(oracle):
     assert _rel(pot, ref[0]) < 1e-12
     for k in range(3):
         assert _rel(forces[:, k], ref[1 + k]) < 1e-12
+
+
+def test_scheme_at_creation(fixture20k):
+    """A context created with the uniform scheme (hfmm_params.m2l_scheme) skips
+    the symmetric compression: its results equal a symmetric context switched
+    to the uniform scheme bit for bit; the 16 symmetric operators it compresses
+    on demand equal the symmetric context's; switched to the symmetric scheme
+    it gives that context's results bit for bit."""
+    from scalfmm_amd import _abi
+    from scalfmm_amd.fmm import Engine
+    d = fixture20k["data"]
+    c, w = fixture20k["center"], fixture20k["width"]
+    engines = []
+    for created in (_abi.M2L_SYMMETRIC, _abi.M2L_UNIFORM):
+        e = Engine(5, w, c, 5, m2l_scheme=created)
+        e.set_particles(d[:, 0], d[:, 1], d[:, 2], d[:, 3])
+        e.build_tree()
+        engines.append(e)
+    sym, uni = engines
+    sym.evaluate()
+    ref_sym = [a.copy() for a in sym.results()]
+    sym.set_m2l_scheme(_abi.M2L_UNIFORM)
+    sym.evaluate()
+    uni.evaluate()
+    for a, b in zip(uni.results(), sym.results()):
+        np.testing.assert_array_equal(a, b)
+    for idx in (269, 276, 342):  # canonical positions (2,0,0), (2,1,0), (3,3,3)
+        Ua, Va = sym.operator(idx)
+        Ub, Vb = uni.operator(idx)
+        np.testing.assert_array_equal(Ua, Ub)
+        np.testing.assert_array_equal(Va, Vb)
+    uni.set_m2l_scheme(_abi.M2L_SYMMETRIC)
+    uni.evaluate()
+    for a, b in zip(uni.results(), ref_sym):
+        np.testing.assert_array_equal(a, b)
+    for e in engines:
+        e.close()
+
+
+def test_scheme_at_creation_rejects_mixed():
+    from scalfmm_amd import FmmError, _abi
+    from scalfmm_amd.fmm import Engine
+    with pytest.raises(FmmError):
+        Engine(3, 1.0, (0.5, 0.5, 0.5), 5, precision=_abi.PRECISION_MIXED, m2l_scheme=_abi.M2L_UNIFORM)
