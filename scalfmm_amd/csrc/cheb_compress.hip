@@ -12,11 +12,13 @@
 //   3. k = the smallest rank whose tail energy is <= eps^2 of the total;
 //   4. K = [Q_U U_phi S_k | Q_V V_phi] with the rows divided by w (un-weighting).
 // The reference calls LAPACK for step 2; these kernels use their own QR and
-// SVD.  Every dot product and every accumulation runs sequentially in one
-// thread in index order, and the file is built without FMA contraction, so
-// the pivots (chosen by exact comparisons, ties included) and the ranks do not
-// depend on a reduction order.  Parallelism is over independent columns,
-// entries and the 16 operators.
+// SVD.  The pACA's dot products and accumulations run sequentially in one
+// thread in index order and the file is built without FMA contraction, so its
+// pivots (chosen by exact comparisons, ties included) do not depend on a
+// reduction order; the Jacobi SVD's column dot products are 64 lane-strided
+// partial sums combined by a fixed xor butterfly, the same bits on every run.
+// Parallelism is over lanes, independent columns, entries and the 16
+// operators.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -162,22 +164,40 @@ __global__ __launch_bounds__(kCT) void k_aca(int nn, int maxk, double eps, const
 // Thin Householder QR in place: A (m x n, column-major) becomes the reflectors
 // below the diagonal and R on and above it; Q (m x n) is formed explicitly and
 // R copied out (n x n).
+// Sum over the 64 lanes by a fixed xor butterfly: every lane ends with the
+// same bits, the same on every run.
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    return v;
+}
+
+// s + sum_{i = i0}^{m-1} x[i] y[i] over one wave (lane-strided partial sums).
+__device__ __forceinline__ double wave_dot_from(double s, int i0, int m, const double* x, const double* y, int lane) {
+    double v = 0.;
+    for (int i = i0 + lane; i < m; i += 64) v += x[i] * y[i];
+    return s + wave_sum(v);
+}
+
 __device__ void block_qr(int m, int n, double* A, double* tau, double* Q, double* R, double* sh) {
-    const int tid = threadIdx.x;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    constexpr int kWaves = kCT / 64;
     for (int j = 0; j < n; ++j) {
         double* cj = A + static_cast<size_t>(j) * m;
-        if (tid == 0) {
-            const double norm = sqrt(dotp(m - j, cj + j, cj + j));
-            if (norm == 0.0) {
-                sh[0] = 0.;
-                tau[j] = 0.;
-            } else {
-                const double beta = cj[j] >= 0 ? -norm : norm;
-                const double head = cj[j] - beta;
-                sh[0] = 1.;
-                sh[1] = beta;
-                sh[2] = head;
-                tau[j] = -head / beta;
+        if (wave == 0) {
+            const double norm = sqrt(wave_dot_from(0., j, m, cj, cj, lane));
+            if (lane == 0) {
+                if (norm == 0.0) {
+                    sh[0] = 0.;
+                    tau[j] = 0.;
+                } else {
+                    const double beta = cj[j] >= 0 ? -norm : norm;
+                    const double head = cj[j] - beta;
+                    sh[0] = 1.;
+                    sh[1] = beta;
+                    sh[2] = head;
+                    tau[j] = -head / beta;
+                }
             }
         }
         __syncthreads();
@@ -187,13 +207,11 @@ __device__ void block_qr(int m, int n, double* A, double* tau, double* Q, double
             __syncthreads();
             if (tid == 0) cj[j] = sh[1];
             const double tj = tau[j];
-            for (int c = j + 1 + tid; c < n; c += kCT) {
+            for (int c = j + 1 + wave; c < n; c += kWaves) {  // one wave per column
                 double* cc = A + static_cast<size_t>(c) * m;
-                double s = cc[j];
-                for (int i = j + 1; i < m; ++i) s += cj[i] * cc[i];
-                s *= tj;
-                cc[j] -= s;
-                for (int i = j + 1; i < m; ++i) cc[i] -= s * cj[i];
+                const double s = wave_dot_from(cc[j], j + 1, m, cj, cc, lane) * tj;
+                if (lane == 0) cc[j] -= s;
+                for (int i = j + 1 + lane; i < m; i += 64) cc[i] -= s * cj[i];
             }
         }
         __syncthreads();
@@ -211,13 +229,11 @@ __device__ void block_qr(int m, int n, double* A, double* tau, double* Q, double
         const double tj = tau[j];
         if (tj != 0.0) {
             const double* v = A + static_cast<size_t>(j) * m;
-            for (int c = tid; c < n; c += kCT) {
+            for (int c = wave; c < n; c += kWaves) {  // one wave per column
                 double* qc = Q + static_cast<size_t>(c) * m;
-                double s = qc[j];
-                for (int i = j + 1; i < m; ++i) s += v[i] * qc[i];
-                s *= tj;
-                qc[j] -= s;
-                for (int i = j + 1; i < m; ++i) qc[i] -= s * v[i];
+                const double s = wave_dot_from(qc[j], j + 1, m, v, qc, lane) * tj;
+                if (lane == 0) qc[j] -= s;
+                for (int i = j + 1 + lane; i < m; i += 64) qc[i] -= s * v[i];
             }
         }
         __syncthreads();
@@ -271,9 +287,18 @@ __global__ __launch_bounds__(kCT) void k_recompress(int nn, int maxk, double eps
             for (int b = a + 1; b < n; ++b) {
                 double* wa = W + static_cast<size_t>(a) * n;
                 double* wb = W + static_cast<size_t>(b) * n;
-                if (tid == 0) sdot[0] = dotp(n, wa, wa);
-                if (tid == 64) sdot[1] = dotp(n, wb, wb);
-                if (tid == 128) sdot[2] = dotp(n, wa, wb);
+                // waves 0, 1, 2: wa.wa, wb.wb, wa.wb as 64 lane-strided partial
+                // sums and an xor butterfly (every lane ends with the same bits)
+                if (tid < 192) {
+                    const int wv = tid >> 6, ln = tid & 63;
+                    const double* x = wv == 1 ? wb : wa;
+                    const double* y = wv == 0 ? wa : wb;
+                    double v = 0.;
+                    for (int i = ln; i < n; i += 64) v += x[i] * y[i];
+#pragma unroll
+                    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+                    if (ln == 0) sdot[wv] = v;
+                }
                 __syncthreads();
                 const double aa = sdot[0], bb = sdot[1], ab = sdot[2];
                 __syncthreads();
