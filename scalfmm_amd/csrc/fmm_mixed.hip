@@ -180,12 +180,13 @@ __global__ __launch_bounds__(256) void k_modal_to_nodal(int64_t ncells, const do
         for (int i = lane; i < NN; i += 64) Lc[i] = a[i];
 }
 
-// Multipoles (FP64, [cells + 1][LD]) -> three bfloat16 planes of the same shape.
+// Multipoles (FP64, [cells + 1][LD]) -> FP32 of the same shape; the M2L splits
+// each value into three bfloat16 parts in registers as it loads it.
 __global__ __launch_bounds__(256) void k_split_multipoles(int64_t nvals, const double* __restrict__ M,
-                                                         uint16_t* __restrict__ Mb) {
+                                                         float* __restrict__ Mf) {
     const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= nvals) return;
-    split3d(M[i], Mb[i], Mb[nvals + i], Mb[2 * nvals + i]);
+    Mf[i] = static_cast<float>(M[i]);
 }
 
 // ---------------------------------------------------------------------------
@@ -208,16 +209,28 @@ struct MixState {
     f4 comp[COLS];  // Kahan compensation of the first row tile (the lowest modes)
 };
 
-// Multipole fragments (three planes, COLS column tiles) of node block kb.
+// Multipole fragments (COLS column tiles) of node block kb: eight FP32 values
+// per lane (two 16-byte loads, 16-byte aligned: LD and 8 kq are multiples of
+// 8), split into the three bfloat16 parts in registers.
 template <int COLS>
-__device__ __forceinline__ void load_mfrag(bf8 (&b)[3][COLS], const uint16_t* const (&Mrow)[COLS], int64_t plane,
-                                           int kb) {
+__device__ __forceinline__ void load_mfrag(bf8 (&b)[3][COLS], const float* const (&Mrow)[COLS], int kb) {
 #pragma unroll
     for (int ct = 0; ct < COLS; ++ct) {
-        const uint16_t* p = Mrow[ct] + 32 * kb;
-        b[0][ct] = ld8(p);
-        b[1][ct] = ld8(p + plane);
-        b[2][ct] = ld8(p + 2 * plane);
+        const float* p = Mrow[ct] + 32 * kb;
+        const f4 x0 = *reinterpret_cast<const f4*>(p);
+        const f4 x1 = *reinterpret_cast<const f4*>(p + 4);
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            __bf16 h, m, l;
+            split3(x0[e], h, m, l);
+            b[0][ct][e] = h;
+            b[1][ct][e] = m;
+            b[2][ct][e] = l;
+            split3(x1[e], h, m, l);
+            b[0][ct][4 + e] = h;
+            b[1][ct][4 + e] = m;
+            b[2][ct][4 + e] = l;
+        }
     }
 }
 
@@ -286,8 +299,8 @@ __device__ __forceinline__ void g1_block(f4 (&c1)[KT][COLS], const bf8 (&b)[3][C
 template <int P, int COLS, int KT, int NPL>
 __device__ __forceinline__ void mix_segment(MixState<P, COLS>& st, int i0, int i1, const int16_t* s_slot,
                                             const int64_t* s_off, const int32_t* lslots, const int (&tgt)[COLS],
-                                            int (&nsrc)[COLS], const uint16_t* __restrict__ Mbl,
-                                            const uint16_t* __restrict__ Mbz, int64_t plane,
+                                            int (&nsrc)[COLS], const float* __restrict__ Mbl,
+                                            const float* __restrict__ Mbz, int64_t plane,
                                             const uint16_t* __restrict__ opm, float* Cp, bf8* Bs, int LD, int CR,
                                             int wave, int lane, int col, int kq, int kb0, int kb1) {
     constexpr int W = kM2LWaves;
@@ -298,7 +311,7 @@ __device__ __forceinline__ void mix_segment(MixState<P, COLS>& st, int i0, int i
     constexpr int KB32 = (KT + 1) / 2;  // GEMM2 K blocks: rank padded to 32
     constexpr int KP32 = 32 * KB32;
     for (int i = i0; i < i1; ++i) {
-        const uint16_t* Mrow[COLS];
+        const float* Mrow[COLS];
         bool any = false;
         bool has[COLS];  // this lane's target has a source in this slot
 #pragma unroll
@@ -328,15 +341,15 @@ __device__ __forceinline__ void mix_segment(MixState<P, COLS>& st, int i0, int i
             bf8 b3[2][3][COLS];
             bf8 va[2][3];
             if (kb0 < kb1) {
-                load_mfrag<COLS>(b3[0], Mrow, plane, kb0);
+                load_mfrag<COLS>(b3[0], Mrow, kb0);
                 load_vfrag<KP16>(va[0], VT, LD, 0, col, kq, kb0);
             }
             for (int kb = kb0; kb < kb1; kb += 2) {
                 const bool more = kb + 1 < kb1;
-                if (more) load_mfrag<COLS>(b3[1], Mrow, plane, kb + 1);
+                if (more) load_mfrag<COLS>(b3[1], Mrow, kb + 1);
                 g1_step<COLS, KT, KP16, NPL>(c1, b3[0], va, VT, LD, col, kq, kb, more);
                 if (more) {
-                    if (kb + 2 < kb1) load_mfrag<COLS>(b3[0], Mrow, plane, kb + 2);
+                    if (kb + 2 < kb1) load_mfrag<COLS>(b3[0], Mrow, kb + 2);
                     g1_step<COLS, KT, KP16, NPL>(c1, b3[1], va, VT, LD, col, kq, kb + 1, kb + 2 < kb1);
                 }
             }
@@ -439,7 +452,7 @@ __global__ __launch_bounds__(256) void k_m2l_mixed(M2LLevels lv, const int32_t* 
                                                   const int32_t* __restrict__ order, const int32_t* __restrict__ pcount,
                                                   const int16_t* __restrict__ slotord, const int32_t* __restrict__ slotseg,
                                                   const int64_t* __restrict__ posoff, const uint16_t* __restrict__ opm,
-                                                  const uint16_t* __restrict__ Mb, int64_t plane, int64_t zero_cell,
+                                                  const float* __restrict__ Mb, int64_t plane, int64_t zero_cell,
                                                   double* __restrict__ L, int LD, int CR) {
     constexpr int NN = P * P * P;
     constexpr int W = kM2LWaves;
@@ -483,8 +496,8 @@ __global__ __launch_bounds__(256) void k_m2l_mixed(M2LLevels lv, const int32_t* 
     for (int q = 0; q < 5; ++q) seg[q] = slotseg[e * 5 + q];
     const int64_t coff = lv.cell_off[li];
     const int32_t* lslots = slots + coff * kFarSlots;
-    const uint16_t* Mbl = Mb + coff * LD;
-    const uint16_t* Mbz = Mb + zero_cell * LD;
+    const float* Mbl = Mb + coff * LD;
+    const float* Mbz = Mb + zero_cell * LD;
     double* Ll = L + coff * LD;
     int tgt[COLS];
 #pragma unroll
@@ -558,8 +571,8 @@ struct WaveMixState {
 template <int P, int CT, int KT, int NPL>
 __device__ __forceinline__ void wave_segment(WaveMixState<P, CT>& st, int i0, int i1, const int16_t* s_slot,
                                              const int64_t* s_off, const int32_t* lslots, const int (&tgt)[CT],
-                                             int (&nsrc)[CT], const uint16_t* __restrict__ Mbl,
-                                             const uint16_t* __restrict__ Mbz, int64_t plane,
+                                             int (&nsrc)[CT], const float* __restrict__ Mbl,
+                                             const float* __restrict__ Mbz, int64_t plane,
                                              const uint16_t* __restrict__ opm, int col, int kq) {
     constexpr int NT = WaveMixState<P, CT>::NT;
     constexpr int NC = WaveMixState<P, CT>::NC;
@@ -569,7 +582,7 @@ __device__ __forceinline__ void wave_segment(WaveMixState<P, CT>& st, int i0, in
     constexpr int KB32 = (KT + 1) / 2;  // GEMM2 K blocks: rank padded to 32
     constexpr int KP32 = 32 * KB32;
     for (int i = i0; i < i1; ++i) {
-        const uint16_t* Mrow[CT];
+        const float* Mrow[CT];
         bool any = false;
         bool has[CT];  // this lane's target has a source in this slot
 #pragma unroll
@@ -592,10 +605,10 @@ __device__ __forceinline__ void wave_segment(WaveMixState<P, CT>& st, int i0, in
             for (int ct = 0; ct < CT; ++ct) c1[rt][ct] = f4{0.f, 0.f, 0.f, 0.f};
         // GEMM1: fragments of block kb + 1 are requested before block kb's MFMAs
         bf8 b3[2][3][CT];
-        load_mfrag<CT>(b3[0], Mrow, plane, 0);
+        load_mfrag<CT>(b3[0], Mrow, 0);
 #pragma unroll
         for (int kb = 0; kb < KB; ++kb) {
-            if (kb + 1 < KB) load_mfrag<CT>(b3[(kb + 1) & 1], Mrow, plane, kb + 1);
+            if (kb + 1 < KB) load_mfrag<CT>(b3[(kb + 1) & 1], Mrow, kb + 1);
 #pragma unroll
             for (int rt = 0; rt < KT; ++rt) {
                 bf8 a[3];
@@ -677,7 +690,7 @@ __global__ __launch_bounds__(256) void k_m2l_mixed_wave(M2LLevels lv, const int3
                                                        const int16_t* __restrict__ slotord,
                                                        const int32_t* __restrict__ slotseg,
                                                        const int64_t* __restrict__ posoff,
-                                                       const uint16_t* __restrict__ opm, const uint16_t* __restrict__ Mb,
+                                                       const uint16_t* __restrict__ opm, const float* __restrict__ Mb,
                                                        int64_t plane, int64_t zero_cell, double* __restrict__ L) {
     constexpr int NN = P * P * P;
     constexpr int LD = WaveMixState<P, CT>::LDC;
@@ -715,8 +728,8 @@ __global__ __launch_bounds__(256) void k_m2l_mixed_wave(M2LLevels lv, const int3
     for (int q = 0; q < 5; ++q) seg[q] = slotseg[e * 5 + q];
     const int64_t coff = lv.cell_off[li];
     const int32_t* lslots = slots + coff * kFarSlots;
-    const uint16_t* Mbl = Mb + coff * LD;
-    const uint16_t* Mbz = Mb + zero_cell * LD;
+    const float* Mbl = Mb + coff * LD;
+    const float* Mbz = Mb + zero_cell * LD;
     double* Ll = L + coff * LD;
     int tgt[CT];
     bool mine = false;
@@ -992,7 +1005,7 @@ int m2l_mixed_launch(Context* c) {
     hipLaunchKernelGGL((k_m2l_mixed<P, COLS, NPL>), dim3(static_cast<unsigned>(tiles)), dim3(64 * kM2LWaves), shmem,
                        c->stream, lv, c->d_m2l.as<int32_t>(), c->d_order.as<int32_t>(), c->d_pcount.as<int32_t>(),
                        c->d_slotord.as<int16_t>(), c->d_slotseg.as<int32_t>(), c->d_posoff.as<int64_t>(),
-                       c->d_opm.as<uint16_t>(), c->d_Mb.as<uint16_t>(), plane, c->total_cells, c->d_L.as<double>(),
+                       c->d_opm.as<uint16_t>(), c->d_Mb.as<float>(), plane, c->total_cells, c->d_L.as<double>(),
                        c->LD, CR);
     return check_launch(c, "m2l_mixed");
 }
@@ -1007,7 +1020,7 @@ int m2l_mixed_wave_launch(Context* c) {
     hipLaunchKernelGGL((k_m2l_mixed_wave<P, CT, NPL>), dim3(static_cast<unsigned>(tiles)), dim3(64 * kM2LWaves), 0,
                        c->stream, lv, c->d_m2l.as<int32_t>(), c->d_order.as<int32_t>(), c->d_pcount.as<int32_t>(),
                        c->d_slotord.as<int16_t>(), c->d_slotseg.as<int32_t>(), c->d_posoff.as<int64_t>(),
-                       c->d_opm2.as<uint16_t>(), c->d_Mb.as<uint16_t>(), plane, c->total_cells, c->d_L.as<double>());
+                       c->d_opm2.as<uint16_t>(), c->d_Mb.as<float>(), plane, c->total_cells, c->d_L.as<double>());
     return check_launch(c, "m2l_mixed");
 }
 
@@ -1153,10 +1166,10 @@ int prepare_mixed_operators(Context* c) {
 
 int run_split_multipoles(Context* c) {
     const int64_t nvals = (c->total_cells + 1) * static_cast<int64_t>(c->LD);
-    HFMM_TRY(ensure(c, c->d_Mb, sizeof(uint16_t) * 3 * static_cast<size_t>(nvals)));
+    HFMM_TRY(ensure(c, c->d_Mb, sizeof(float) * static_cast<size_t>(nvals)));
     prof_begin(c, "split_multipoles");
     hipLaunchKernelGGL(k_split_multipoles, dim3(blocks_of(nvals)), dim3(256), 0, c->stream, nvals,
-                       c->d_M.as<double>(), c->d_Mb.as<uint16_t>());
+                       c->d_M.as<double>(), c->d_Mb.as<float>());
     prof_end(c);
     return check_launch(c, "split_multipoles");
 }

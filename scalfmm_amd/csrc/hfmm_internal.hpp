@@ -142,7 +142,7 @@ struct Context {
 
     // ---- expansions and results
     DevBuf d_M, d_L;      // [total_cells][LD]
-    DevBuf d_Mb;          // mixed precision: multipoles as bf16 hi/mid/lo planes [3][total_cells + 1][LD]
+    DevBuf d_Mb;          // mixed precision: multipoles in FP32 [total_cells + 1][LD] (the M2L splits them)
     DevBuf d_loc;         // mixed precision: leaf-local fixed-point int4 (x, y, z, q bits) of the sorted particles
     DevBuf d_res_sorted;  // pot, grad(phi) x, y, z (sorted order; the charge is applied by the scatter)
     DevBuf d_res;         // pot, fx, fy, fz (input order)
