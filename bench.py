@@ -96,15 +96,23 @@ def algorithmic_m2l_flops(engine, height, order):
 
 
 def p2p_pairs(engine):
-    """Target-source pairs of the near field (self pairs excluded), split into
-    the leaves with at most 64 particles and the bigger ones."""
+    """Target-source pairs of the near field (self pairs excluded) by kernel:
+    'small' = targets in leaves of at most 64 particles (k_p2p_*), 'big' = the
+    other targets (the ordered big-leaf kernels); the mutual kernel's work is
+    'big_big' ordered pairs between two big leaves (each unordered pair once)
+    plus 'big_rest' = the big leaves' pairs with themselves and with small
+    neighbours (ordered)."""
     _, cnt = engine.leaf_ranges()
     off, nb, _ = engine.p2p_list()
     owner = np.repeat(np.arange(len(cnt)), np.diff(off))
-    nbr_sum = np.bincount(owner, weights=cnt[nb].astype(np.float64), minlength=len(cnt))
-    per_leaf = cnt * (cnt - 1) + cnt * nbr_sum
+    cnt = cnt.astype(np.float64)
     big = cnt > 64
-    return float(per_leaf[~big].sum()), float(per_leaf[big].sum())
+    nbr_sum = np.bincount(owner, weights=cnt[nb], minlength=len(cnt))
+    nbr_big = np.bincount(owner, weights=cnt[nb] * big[nb], minlength=len(cnt))
+    per_leaf = cnt * (cnt - 1) + cnt * nbr_sum
+    big_big = (cnt * nbr_big)[big].sum()
+    return {"small": float(per_leaf[~big].sum()), "big": float(per_leaf[big].sum()),
+            "big_big": float(big_big), "big_rest": float(per_leaf[big].sum() - big_big)}
 
 
 def make_particles(dist, n):
@@ -126,19 +134,44 @@ def cpu_baseline(dist, n, h, p):
                       f"{st.t_total:.2f} s, M2L {st.t_m2l:.2f} s, P2P {st.t_p2p:.2f} s, wall {wall:.2f} s"}
 
 
+# source file of each profiled kernel: a committed summary's traffic is used
+# only while that file is unchanged (its sha256 is recorded in the summary)
+KERNEL_SOURCE = {
+    "k_p2p_mixed_big": "fmm_mixed.hip", "k_p2p_mixed_mutual": "fmm_mixed.hip", "k_m2l_mixed": "fmm_mixed.hip",
+    "k_p2p_mixed": "fmm_mixed.hip", "k_m2l_f64": "fmm_device.hip", "k_p2p_f64_big": "fmm_device.hip",
+    "k_p2p_f64": "fmm_device.hip", "k_unif_m2l": "fmm_global.hip",
+}
+
+
+def source_sha256(kernel):
+    import hashlib
+    src = KERNEL_SOURCE.get(kernel)
+    if src is None:
+        return None
+    with open(os.path.join(ROOT, "scalfmm_amd", "csrc", src), "rb") as f:
+        return hashlib.sha256(f.read()).hexdigest()
+
+
 def profiled_traffic(kernel, workload, form=None):
     """HBM bytes per launch of `kernel` from its committed rocprofv3 summary
-    (FETCH_SIZE and WRITE_SIZE scaled by a copy-kernel calibration), or None.
-    A summary of one precision form is named ..._<workload>-<form>.md."""
+    (FETCH_SIZE and WRITE_SIZE scaled by a copy-kernel calibration) and a
+    note; None when there is no summary or when the kernel's source changed
+    after it was profiled.  A summary of one precision form is named
+    ..._<workload>-<form>.md."""
     path = os.path.join(ROOT, "profiles", f"{kernel}_{workload}-{form}.md")
     if not os.path.exists(path):
         path = os.path.join(ROOT, "profiles", f"{kernel}_{workload}.md")
     if not os.path.exists(path):
-        return None
+        return None, "no profile summary"
+    traffic, sha = None, None
     for line in open(path):
         if line.startswith("traffic_bytes_per_launch:"):
-            return float(line.split(":", 1)[1])
-    return None
+            traffic = float(line.split(":", 1)[1])
+        if line.startswith("kernel_source_sha256:"):
+            sha = line.split(":", 1)[1].strip()
+    if sha is None or sha != source_sha256(kernel):
+        return None, f"{os.path.basename(path)} was profiled on another version of the kernel source"
+    return traffic, os.path.basename(path)
 
 
 def copy_calibration(mbytes=512, reps=3):
@@ -155,8 +188,10 @@ def copy_calibration(mbytes=512, reps=3):
 
 
 def run_form(parts, h, p, precision, steps, warmup):
-    """Warm up, then time `steps` tree builds + evaluations; kernel times from
-    HIP events on the engine's stream over the same region."""
+    """Warm up, then time `steps` tree builds + evaluations.  The timed steps
+    record HIP events on the engine's stream around the near-field and M2L
+    kernels only (their rooflines); a separate profiled pass of the same
+    steps gives the per-kernel breakdown."""
     from scalfmm_amd import Engine
     x, y, z, q = parts
     t0 = time.perf_counter()
@@ -168,6 +203,7 @@ def run_form(parts, h, p, precision, steps, warmup):
         e.evaluate()
     e.synchronize()
     e.set_profiling(True)
+    e.set_profiling_filter(ROOFLINE_KERNELS)
     e.reset_kernel_times()
     t0 = time.perf_counter()
     for _ in range(steps):
@@ -175,10 +211,21 @@ def run_form(parts, h, p, precision, steps, warmup):
         e.evaluate()
     e.synchronize()
     t1 = time.perf_counter()
+    live = e.kernel_times()
+    e.set_profiling_filter(())
+    e.reset_kernel_times()
+    for _ in range(steps):
+        e.build_tree()
+        e.evaluate()
     times = e.kernel_times()
     e.set_profiling(False)
     ms = (t1 - t0) * 1e3 / steps
+    times.update(live)  # the roofline kernels as timed in the timed steps
     return e, ms, {k: v[0] / steps for k, v in sorted(times.items())}, {k: v[1] / steps for k, v in times.items()}
+
+
+# kernels whose HIP-event times the timed steps record (the roofline kernels)
+ROOFLINE_KERNELS = ("p2p_mixed_mutual", "p2p_mixed_big", "p2p_big", "m2l_mixed", "m2l")
 
 
 # M2L partial products per product of each form (bfloat16 MFMA in the mixed forms)
@@ -212,23 +259,35 @@ def roofline(engines, kernel_ms, launches, p, form, workload, nrhs=1, kernel=Non
         kname, bound = ("k_m2l_mixed", "mfma") if mixed else ("k_m2l_f64", "mfma")
         extra = {"m2l_interactions": n_int, "fp64_equivalent_flops_per_step": flops,
                  "work": f"{PRODUCTS[form]} bf16 partial products per product" if mixed else "FP64 products"}
-    elif top in (p2p_small, p2p_big):
-        pairs = 0.0
+    elif top in (p2p_small, p2p_big, "p2p_mixed_mutual"):
+        pairs = {"small": 0.0, "big": 0.0, "big_big": 0.0, "big_rest": 0.0}
         for e in engines:
-            small, big = p2p_pairs(e)
-            pairs += (big if top == p2p_big else small) * nrhs
-        work = 34.0 * pairs
+            for k, v in p2p_pairs(e).items():
+                pairs[k] += v * nrhs
         peak = PEAK["fp32_vector"] if mixed else PEAK["fp64_vector"]
-        kname = {"p2p_mixed": "k_p2p_mixed", "p2p_mixed_big": "k_p2p_mixed_big", "p2p": "k_p2p_f64",
-                 "p2p_big": "k_p2p_f64_big"}[top]
         bound = "valu"
-        extra = {"pairs": pairs, "work": "34 flops per target-source pair"}
+        if top == "p2p_mixed_mutual":
+            # each unordered big-big pair once (39 flops, the reference's mutual
+            # count), the other pairs of the big leaves one way (34 flops)
+            work = 39.0 * pairs["big_big"] / 2 + 34.0 * pairs["big_rest"]
+            kname = "k_p2p_mixed_mutual"
+            extra = {"pairs": pairs["big"], "unordered_big_big_pairs": pairs["big_big"] / 2,
+                     "ordered_other_pairs": pairs["big_rest"],
+                     "work": "39 flops per unordered pair of two big leaves, 34 per other target-source pair "
+                             "(FChebFlopsSymKernel.hpp:66-69)"}
+        else:
+            work = 34.0 * (pairs["big"] if top == p2p_big else pairs["small"])
+            kname = {"p2p_mixed": "k_p2p_mixed", "p2p_mixed_big": "k_p2p_mixed_big", "p2p": "k_p2p_f64",
+                     "p2p_big": "k_p2p_f64_big"}[top]
+            extra = {"pairs": pairs["big"] if top == p2p_big else pairs["small"],
+                     "work": "34 flops per target-source pair"}
     else:
         return {"kernel": top, "note": "no roofline model for this kernel"}
     ms = kernel_ms[top]
     achieved = work / (ms / 1e3) / 1e12
+    traffic, traffic_src = profiled_traffic(kname, workload, form)
     out = {"bound": bound, "achieved": achieved, "peak": peak, "unit": "TFLOP/s", "frac": achieved / peak,
-           "traffic": profiled_traffic(kname, workload, form), "kernel": kname, "kernel_ms_per_step": ms,
+           "traffic": traffic, "traffic_source": traffic_src, "kernel": kname, "kernel_ms_per_step": ms,
            "launches_per_step": launches.get(top), "algorithmic_flops_per_launch": work / max(launches.get(top, 1), 1)}
     out.update(extra)
     return out

@@ -173,6 +173,31 @@ int hfmm_evaluate_levels(hfmm_ctx* ctx, uint32_t ops, int32_t upper, int32_t low
 int hfmm_evaluate_particle_blocks(hfmm_ctx* ctx, int32_t nblocks, const void* const* blocks, double* const* attributes,
                                   int32_t nvals, uint32_t ops, int32_t upper, int32_t lower);
 
+/* The group tree's cell blocks of one level (2 .. height-1) as the boundary:
+ * blocks[b] is cell group b's getRawBuffer() (GroupTree/Core/FGroupOfCells.hpp:
+ * 189-197: a 24-byte header {startingIndex, endingIndex, numberOfCellsInBlock},
+ * the cells' Morton indexes, then one FBasicCellPOD {mortonIndex,
+ * coordinates[3], level} per cell), multipoles[b] its getRawMultipoleBuffer()
+ * and locals[b] its getRawLocalBuffer() (:199-242): per cell nvals x 2 p^3
+ * doubles, RHS v at offset 2 p^3 v, the FChebCellPODPole / FChebCellPODLocal
+ * layout (GroupTree/Chebyshev/FChebCellPOD.hpp:11-21, 85-100).
+ * hfmm_get_cell_blocks writes the context's multipole and local expansions of
+ * the cells of the blocks into the first p^3 values of each RHS slot (the
+ * global-SVD scheme also writes its compressed expansions, k values, at
+ * offset p^3, where FChebKernel keeps them, FChebKernel.hpp:103, 148); the
+ * other values are untouched.  hfmm_set_cell_blocks copies them into the
+ * context's cells (kept-cell mode, symmetric scheme): the next evaluation adds
+ * into them, as the reference's passes add into the tree's cells.  Every cell
+ * of the blocks must be a cell of the context's tree at that level (one
+ * particle set); nvals is the context's charge-vector count (above 1 with
+ * kept cells only); NULL buffers skip.  Fails with HFMM_ERR_ARG on a bad
+ * header, indexes out of order or outside the block's interval, or an unknown
+ * cell. */
+int hfmm_get_cell_blocks(hfmm_ctx* ctx, int32_t level, int32_t nblocks, const void* const* blocks,
+                         double* const* multipoles, double* const* locals, int32_t nvals);
+int hfmm_set_cell_blocks(hfmm_ctx* ctx, int32_t level, int32_t nblocks, const void* const* blocks,
+                         const double* const* multipoles, const double* const* locals, int32_t nvals);
+
 /* M2L scheme.  HFMM_M2L_SYMMETRIC (the default) is FChebSymKernel's: 16
  * canonical operators compressed one by one, applied through the 343
  * symmetries.  HFMM_M2L_GLOBAL_SVD is FChebKernel's
@@ -213,6 +238,18 @@ int hfmm_get_global_operators(hfmm_ctx* ctx, int32_t* rank, double* U, double* B
  * so the passes do not overlap and the M2L autotuner, timing under the P2P,
  * picks worse tile shapes.  No counterpart in the reference. */
 int hfmm_set_stream_overlap(hfmm_ctx* ctx, int32_t enable);
+
+/* Near field of the big leaves in the mixed forms (default on): each unordered
+ * pair of two neighbour leaves with more than 64 particles is evaluated once
+ * and added to both particles, as the reference's P2POuter and its algorithms'
+ * opposite-direction calls do (FChebSymKernel.hpp:468-478,
+ * FGroupSeqAlgorithm.hpp:343-394, FP2P.hpp:103-171).  The source-side sums of
+ * a pair go through per-(particle, neighbour work item) slots in device memory
+ * that a second kernel adds in a fixed order, so results do not depend on the
+ * launch.  Off: every ordered pair is evaluated by its target's kernel.  The
+ * FP64 form always evaluates ordered pairs.  A tree whose slots would need
+ * more than a quarter of the free device memory falls back to ordered pairs. */
+int hfmm_set_p2p_mutual(hfmm_ctx* ctx, int32_t enable);
 
 /* The state of the cells (multipoles and locals) between evaluations.
  * HFMM_CELLS_FRESH (the default): every evaluation starts from cells in their
@@ -309,6 +346,11 @@ typedef struct hfmm_kernel_time {
     int64_t launches;
 } hfmm_kernel_time;
 int hfmm_set_profiling(hfmm_ctx* ctx, int32_t enable);
+/* Restricts the timed kernels to a comma-separated list of the names
+ * hfmm_get_kernel_times reports (e.g. "p2p_mixed_big,m2l_mixed"); NULL or ""
+ * times every kernel.  With a short list a profiled evaluation records only a
+ * few events per step.  No counterpart in the reference. */
+int hfmm_set_profiling_filter(hfmm_ctx* ctx, const char* names);
 int hfmm_get_kernel_times(hfmm_ctx* ctx, hfmm_kernel_time* out, int32_t max, int32_t* count);
 int hfmm_reset_kernel_times(hfmm_ctx* ctx);
 

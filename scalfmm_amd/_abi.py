@@ -47,7 +47,8 @@ EXPORTS = (
     "hfmm_set_m2l_variant", "hfmm_get_m2l_variant", "hfmm_evaluate_levels",
     "hfmm_fma_read", "hfmm_fma_write", "hfmm_save_operators", "hfmm_load_operators", "hfmm_operator_file_name",
     "hfmm_evaluate_particle_blocks", "hfmm_set_cell_mode", "hfmm_reset_cells", "hfmm_set_stream_overlap",
-    "hfmm_set_m2l_scheme", "hfmm_get_global_operators",
+    "hfmm_set_m2l_scheme", "hfmm_get_global_operators", "hfmm_set_p2p_mutual", "hfmm_set_profiling_filter",
+    "hfmm_get_cell_blocks", "hfmm_set_cell_blocks",
 )
 MAX_RHS = 64
 _NON_STATUS = ("hfmm_last_error", "hfmm_version", "hfmm_batch_last_error", "hfmm_batch_default_height")
@@ -142,10 +143,14 @@ def lib(path: str | None = None):
     L.hfmm_get_m2l_variant.argtypes = [vp, C.POINTER(i32), C.POINTER(C.c_float)]
     L.hfmm_set_cell_mode.argtypes = [vp, i32]
     L.hfmm_set_stream_overlap.argtypes = [vp, i32]
+    L.hfmm_set_p2p_mutual.argtypes = [vp, i32]
+    L.hfmm_set_profiling_filter.argtypes = [vp, C.c_char_p]
     L.hfmm_set_m2l_scheme.argtypes = [vp, i32]
     L.hfmm_get_global_operators.argtypes = [vp, C.POINTER(i32), dp, dp, dp, dp, dp]
     L.hfmm_reset_cells.argtypes = [vp]
     L.hfmm_evaluate_particle_blocks.argtypes = [vp, i32, C.POINTER(vp), C.POINTER(dp), i32, C.c_uint32, i32, i32]
+    L.hfmm_get_cell_blocks.argtypes = [vp, i32, i32, C.POINTER(vp), C.POINTER(dp), C.POINTER(dp), i32]
+    L.hfmm_set_cell_blocks.argtypes = [vp, i32, i32, C.POINTER(vp), C.POINTER(dp), C.POINTER(dp), i32]
     for name in EXPORTS:
         if name not in _NON_STATUS:
             getattr(L, name).restype = C.c_int

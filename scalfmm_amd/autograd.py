@@ -11,14 +11,17 @@ loss L(phi) with g = dL/dphi:
             through source j.
 
 The forward pass keeps only grad(phi) (3N values).  The backward pass sets the
-charges g on the forward's tree (no rebuild) and recomputes every expansion
-instead of storing them.  With an engine in HFMM_PRECISION_MIXED the adjoint
-runs in the mixed form (FP32 P2P, split-bf16 M2L).
+charges g on the forward's tree (no rebuild), recomputes every expansion
+instead of storing them, and puts the forward's charges back (the engine's
+results() then hold the adjoint potential psi until its next evaluate()).
+With an engine in HFMM_PRECISION_MIXED the adjoint runs in the mixed form
+(FP32 P2P, split-bf16 M2L).
 
-Tensors cross the boundary through host memory (CPU tensors, float64): the
-engine owns its device memory and its own HIP runtime.  The reference has no
-counterpart (its kernels have no derivative with respect to the charges or
-the positions).
+FmmPotential takes CPU tensors (float64 host copies cross the boundary);
+fmm_potential() sends device tensors to the libtorch C++ autograd function
+(csrc/torch_potential.cpp) instead, which reads them in place.  The reference
+has no counterpart (its kernels have no derivative with respect to the
+charges or the positions).
 """
 from __future__ import annotations
 
@@ -55,10 +58,30 @@ class FmmPotential(torch.autograd.Function):
         e.evaluate()
         psi = e.results()[0]
         field_g = np.stack(e.field(), axis=1)
+        # the forward's charges back on the tree: a later evaluate() computes
+        # phi again (results() keep the adjoint potential psi until then)
+        e.set_charges(ctx.qn)
         dx = gn[:, None] * ctx.field + ctx.qn[:, None] * field_g
         return torch.from_numpy(dx).to(ctx.dtype), torch.from_numpy(psi).to(ctx.dtype), None
 
 
+def _device_function():
+    """The libtorch C++ autograd function (scalfmm_amd/_lib/_torch_fmm): device
+    tensors in and out, backward written in C++ (csrc/torch_potential.cpp)."""
+    from ._lib import _torch_fmm
+    if not _torch_fmm.same_runtime():
+        raise RuntimeError("libscalfmm_hip.so was loaded before torch: the engine and torch use two HIP "
+                           "runtimes and cannot share device tensors; import torch first")
+    return _torch_fmm.potential
+
+
 def fmm_potential(x: torch.Tensor, q: torch.Tensor, engine) -> torch.Tensor:
-    """Differentiable potential of charges q at positions x (one engine per problem size)."""
+    """Differentiable potential of charges q at positions x (one engine per
+    problem size).  Device tensors go through the C++ autograd function
+    (the engine reads them in place, and after the backward it keeps the
+    autograd context's copies of x and q as its particles: set them again
+    before evaluating with the engine directly); CPU tensors through
+    FmmPotential (host round trips)."""
+    if x.is_cuda:
+        return _device_function()(x, q, engine.h.value)
     return FmmPotential.apply(x, q, engine)

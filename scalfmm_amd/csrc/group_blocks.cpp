@@ -17,6 +17,7 @@
 // FP2PGroupParticleContainer<double, 1, 1, NVALS> (FP2PGroupParticleContainer.hpp:8-57)
 // those are potential[v], forceX[v], forceY[v], forceZ[v] at v, NVALS + v,
 // 2 NVALS + v, 3 NVALS + v; the physical values are the NVALS symbolic ones.
+#include <algorithm>
 #include <cstdint>
 #include <cstring>
 #include <string>
@@ -186,6 +187,194 @@ int hfmm_evaluate_particle_blocks(hfmm_ctx* h, int32_t nblocks, const void* cons
                         std::memcpy(dst, &acc, sizeof(double));
                     }
             }
+    }
+    return HFMM_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Cell blocks of one level (GroupTree/Core/FGroupOfCells.hpp:18-242 with the
+// cells of GroupTree/Chebyshev/FChebCellPOD.hpp:11-112, alignment 8,
+// StarPUUtils/FStarPUDefaultAlign.hpp:4-6):
+//   raw buffer:  BlockHeader {startingIndex, endingIndex (int64),
+//                numberOfCellsInBlock (int32 + 4 bytes of padding)},
+//                the cells' Morton indexes (int64 each), then one
+//                FBasicCellPOD {mortonIndex (int64), coordinates[3], level
+//                (int32)} per cell (GroupTree/Core/FBasicCellPOD.hpp:9-13)
+//   multipoles:  per cell FChebCellPODPole: nvals x 2 p^3 doubles, RHS v at
+//                v 2 p^3 (FChebCellPOD.hpp:85-100); the symmetric kernel uses
+//                the first p^3 values of each RHS slot, FChebKernel keeps its
+//                compressed expansion behind them (FChebKernel.hpp:103, 148)
+//   locals:      the same with FChebCellPODLocal
+namespace {
+
+struct CellBlockHeader {
+    int64_t starting_index;
+    int64_t ending_index;
+    int32_t number_of_cells;
+    int32_t padding;
+};
+static_assert(sizeof(CellBlockHeader) == 24, "BlockHeader of FGroupOfCells");
+
+struct CellBlock {
+    CellBlockHeader h;
+    std::vector<int64_t> keys;
+    std::vector<int64_t> cell;  // the context's cell index (in its level) of each block cell
+};
+
+std::string cwhere(int32_t b) { return "cell block " + std::to_string(b) + ": "; }
+
+// Headers, index order and interval (the assertions of FGroupOfCells'
+// constructors and newCell, FGroupOfCells.hpp:88-112, 130-168, 347-354), and
+// every cell found among the context's cells of the level.
+int read_cell_blocks(hfmm::Context* c, int32_t level, int32_t nblocks, const void* const* blocks,
+                     std::vector<CellBlock>& out) {
+    const int64_t nc = c->ncell[level];
+    std::vector<int64_t> keys(static_cast<size_t>(nc));
+    HFMM_HIP(c, hipStreamSynchronize(c->stream));
+    HFMM_HIP(c, hipMemcpy(keys.data(), c->d_cellkey.as<uint64_t>() + c->lvl_off[level], sizeof(int64_t) * nc,
+                          hipMemcpyDeviceToHost));
+    out.resize(static_cast<size_t>(nblocks));
+    for (int32_t b = 0; b < nblocks; ++b) {
+        if (!blocks[b]) return hfmm::fail(c, HFMM_ERR_ARG, cwhere(b) + "null buffer");
+        const unsigned char* raw = static_cast<const unsigned char*>(blocks[b]);
+        CellBlock& k = out[b];
+        std::memcpy(&k.h, raw, sizeof(CellBlockHeader));
+        if (k.h.number_of_cells < 0 || k.h.ending_index < k.h.starting_index ||
+            k.h.ending_index - k.h.starting_index < k.h.number_of_cells)
+            return hfmm::fail(c, HFMM_ERR_ARG, cwhere(b) + "bad header");
+        k.keys.resize(static_cast<size_t>(k.h.number_of_cells));
+        if (k.h.number_of_cells)
+            std::memcpy(k.keys.data(), raw + sizeof(CellBlockHeader), sizeof(int64_t) * k.keys.size());
+        k.cell.resize(k.keys.size());
+        int64_t prev = -1;
+        for (size_t i = 0; i < k.keys.size(); ++i) {
+            const int64_t m = k.keys[i];
+            if (m <= prev || m < k.h.starting_index || m >= k.h.ending_index)
+                return hfmm::fail(c, HFMM_ERR_ARG, cwhere(b) + "cell " + std::to_string(i) +
+                                                       ": Morton index out of order or outside the block's interval");
+            prev = m;
+            const auto it = std::lower_bound(keys.begin(), keys.end(), m);
+            if (it == keys.end() || *it != m)
+                return hfmm::fail(c, HFMM_ERR_ARG, cwhere(b) + "cell " + std::to_string(m) + " of level " +
+                                                       std::to_string(level) + " is not a cell of the context's tree");
+            k.cell[i] = it - keys.begin();
+        }
+    }
+    return HFMM_OK;
+}
+
+// Device arrays of charge vector r: nodal M and L [cells][LD], and for the
+// global-SVD scheme the compressed M~ and L~ [cells][k] (one vector only).
+struct ExpArrays {
+    double* M;
+    double* L;
+    double* Mt;
+    double* Lt;
+};
+
+ExpArrays exp_arrays(hfmm::Context* c, int r) {
+    const int64_t words = c->total_cells * static_cast<int64_t>(c->LD);
+    ExpArrays a{c->d_M.as<double>(), c->d_L.as<double>(), nullptr, nullptr};
+    if (c->keep_cells && c->nrhs > 1) {
+        a.M = c->d_kept.as<double>() + 2 * words * r;
+        a.L = a.M + words;
+    }
+    if (c->m2l_scheme != HFMM_M2L_SYMMETRIC && c->glob.k > 0) {
+        a.Mt = c->d_Mt.as<double>();
+        a.Lt = c->d_Lt.as<double>();
+    }
+    return a;
+}
+
+int check_cell_call(hfmm::Context* c, int32_t level, int32_t nblocks, const void* const* blocks, int32_t nvals) {
+    if (!c->tree_built) return hfmm::fail(c, HFMM_ERR_STATE, "cell blocks before build_tree");
+    if (c->nsets != 1) return hfmm::fail(c, HFMM_ERR_ARG, "cell blocks: the context holds several particle sets");
+    if (level < 2 || level >= c->P.tree_height)
+        return hfmm::fail(c, HFMM_ERR_ARG, "cell blocks: level must lie in 2 .. height-1 (the kernel's levels)");
+    if (nblocks < 0 || (nblocks > 0 && !blocks)) return hfmm::fail(c, HFMM_ERR_ARG, "cell blocks: bad arguments");
+    if (nvals != c->nrhs) return hfmm::fail(c, HFMM_ERR_ARG, "cell blocks: nvals differs from the charge vectors");
+    if (nvals > 1 && !c->keep_cells)
+        return hfmm::fail(c, HFMM_ERR_STATE, "cell blocks of several charge vectors need kept cells");
+    return HFMM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int hfmm_get_cell_blocks(hfmm_ctx* h, int32_t level, int32_t nblocks, const void* const* blocks,
+                         double* const* multipoles, double* const* locals, int32_t nvals) {
+    if (!h) return HFMM_ERR_ARG;
+    hfmm::Context* c = hfmm::context_of(h);
+    HFMM_TRY(check_cell_call(c, level, nblocks, blocks, nvals));
+    if (!c->evaluated && !(c->keep_cells && !c->cells_dirty))
+        return hfmm::fail(c, HFMM_ERR_STATE, "get_cell_blocks before an evaluation");
+    std::vector<CellBlock> bl;
+    HFMM_TRY(read_cell_blocks(c, level, nblocks, blocks, bl));
+    const int nn = c->ops.nn, LD = c->LD;
+    const int64_t nc = c->ncell[level], first = c->lvl_off[level];
+    const int k = (c->m2l_scheme != HFMM_M2L_SYMMETRIC) ? c->glob.k : 0;
+    const bool compressed = k > 0 && k <= nn && c->m2l_scheme != HFMM_M2L_UNIFORM;
+    std::vector<double> M(static_cast<size_t>(nc) * LD), L(M.size()), Mt, Lt;
+    HFMM_HIP(c, hipStreamSynchronize(c->stream));
+    for (int r = 0; r < nvals; ++r) {
+        const ExpArrays a = exp_arrays(c, r);
+        HFMM_HIP(c, hipMemcpy(M.data(), a.M + first * LD, sizeof(double) * M.size(), hipMemcpyDeviceToHost));
+        HFMM_HIP(c, hipMemcpy(L.data(), a.L + first * LD, sizeof(double) * L.size(), hipMemcpyDeviceToHost));
+        if (compressed && a.Mt) {
+            Mt.resize(static_cast<size_t>(nc) * k);
+            Lt.resize(Mt.size());
+            HFMM_HIP(c, hipMemcpy(Mt.data(), a.Mt + first * k, sizeof(double) * Mt.size(), hipMemcpyDeviceToHost));
+            HFMM_HIP(c, hipMemcpy(Lt.data(), a.Lt + first * k, sizeof(double) * Lt.size(), hipMemcpyDeviceToHost));
+        }
+        for (int32_t b = 0; b < nblocks; ++b)
+            for (size_t i = 0; i < bl[b].cell.size(); ++i) {
+                const size_t src = static_cast<size_t>(bl[b].cell[i]);
+                const size_t dst = (i * nvals + r) * 2 * static_cast<size_t>(nn);
+                if (multipoles && multipoles[b]) {
+                    std::memcpy(multipoles[b] + dst, M.data() + src * LD, sizeof(double) * nn);
+                    if (!Mt.empty()) std::memcpy(multipoles[b] + dst + nn, Mt.data() + src * k, sizeof(double) * k);
+                }
+                if (locals && locals[b]) {
+                    std::memcpy(locals[b] + dst, L.data() + src * LD, sizeof(double) * nn);
+                    if (!Lt.empty()) std::memcpy(locals[b] + dst + nn, Lt.data() + src * k, sizeof(double) * k);
+                }
+            }
+    }
+    return HFMM_OK;
+}
+
+int hfmm_set_cell_blocks(hfmm_ctx* h, int32_t level, int32_t nblocks, const void* const* blocks,
+                         const double* const* multipoles, const double* const* locals, int32_t nvals) {
+    if (!h) return HFMM_ERR_ARG;
+    hfmm::Context* c = hfmm::context_of(h);
+    HFMM_TRY(check_cell_call(c, level, nblocks, blocks, nvals));
+    if (!c->keep_cells) return hfmm::fail(c, HFMM_ERR_STATE, "set_cell_blocks needs kept cells (HFMM_CELLS_KEPT)");
+    if (c->m2l_scheme != HFMM_M2L_SYMMETRIC)
+        return hfmm::fail(c, HFMM_ERR_ARG, "set_cell_blocks: the symmetric M2L scheme only");
+    std::vector<CellBlock> bl;
+    HFMM_TRY(read_cell_blocks(c, level, nblocks, blocks, bl));
+    HFMM_HIP(c, hipSetDevice(c->device));
+    HFMM_TRY(hfmm::prepare_kept_cells(c));  // expansion buffers, initial state of a new tree
+    const int nn = c->ops.nn, LD = c->LD;
+    const int64_t nc = c->ncell[level], first = c->lvl_off[level];
+    std::vector<double> M(static_cast<size_t>(nc) * LD), L(M.size());
+    HFMM_HIP(c, hipStreamSynchronize(c->stream));
+    for (int r = 0; r < nvals; ++r) {
+        const ExpArrays a = exp_arrays(c, r);
+        HFMM_HIP(c, hipMemcpy(M.data(), a.M + first * LD, sizeof(double) * M.size(), hipMemcpyDeviceToHost));
+        HFMM_HIP(c, hipMemcpy(L.data(), a.L + first * LD, sizeof(double) * L.size(), hipMemcpyDeviceToHost));
+        for (int32_t b = 0; b < nblocks; ++b)
+            for (size_t i = 0; i < bl[b].cell.size(); ++i) {
+                const size_t dst = static_cast<size_t>(bl[b].cell[i]) * LD;
+                const size_t src = (i * nvals + r) * 2 * static_cast<size_t>(nn);
+                if (multipoles && multipoles[b]) std::memcpy(M.data() + dst, multipoles[b] + src, sizeof(double) * nn);
+                if (locals && locals[b]) std::memcpy(L.data() + dst, locals[b] + src, sizeof(double) * nn);
+            }
+        HFMM_HIP(c, hipMemcpy(a.M + first * LD, M.data(), sizeof(double) * M.size(), hipMemcpyHostToDevice));
+        HFMM_HIP(c, hipMemcpy(a.L + first * LD, L.data(), sizeof(double) * L.size(), hipMemcpyHostToDevice));
     }
     return HFMM_OK;
 }

@@ -53,7 +53,9 @@ static hipEvent_t take_event(Context* c) {
 }
 
 void prof_begin(Context* c, const char* name) {
+    c->prof_open = false;
     if (!c->profiling) return;
+    if (!c->prof_filter.empty() && c->prof_filter.find("," + std::string(name) + ",") == std::string::npos) return;
     KernelTimer t;
     t.name = name;
     t.start = take_event(c);
@@ -61,11 +63,13 @@ void prof_begin(Context* c, const char* name) {
     if (!t.start || !t.stop) return;
     (void)hipEventRecord(t.start, c->stream);
     c->pending.push_back(t);
+    c->prof_open = true;
 }
 
 void prof_end(Context* c) {
-    if (!c->profiling || c->pending.empty()) return;
+    if (!c->profiling || !c->prof_open || c->pending.empty()) return;
     (void)hipEventRecord(c->pending.back().stop, c->stream);
+    c->prof_open = false;
 }
 
 int check_launch(Context* c, const char* name) {
@@ -208,7 +212,8 @@ static void free_all(Context* c) {
                       &c->d_scan_part, &c->d_qin, &c->d_flags, &c->d_scan, &c->d_sorted, &c->d_perm, &c->d_pleaf,
                       &c->d_count, &c->d_cellkey, &c->d_cellkey_tmp, &c->d_childstart, &c->d_leafstart,
                       &c->d_p2p, &c->d_p2pitems, &c->d_m2l, &c->d_order, &c->d_pcount, &c->d_dense, &c->d_M, &c->d_L, &c->d_Mb, &c->d_loc, &c->d_res_sorted,
-                      &c->d_res, &c->d_field, &c->d_setoff, &c->d_keep, &c->d_kept, &c->d_gBt, &c->d_gU, &c->d_gC, &c->d_Mt, &c->d_Lt, &c->d_uroots, &c->d_uK, &c->d_utw};
+                      &c->d_res, &c->d_field, &c->d_setoff, &c->d_keep, &c->d_kept, &c->d_gBt, &c->d_gU, &c->d_gC, &c->d_Mt, &c->d_Lt, &c->d_uroots, &c->d_uK, &c->d_utw,
+                      &c->d_p2pitems_m, &c->d_csoff, &c->d_csbase, &c->d_cpart};
     for (DevBuf* b : bufs) release(*b);
 }
 
@@ -224,6 +229,26 @@ struct hfmm_ctx {
     do {                                  \
         if (!(ctx)) return HFMM_ERR_ARG;  \
     } while (0)
+
+namespace hfmm {
+// Kept cells: the expansion buffers of the built tree, put in their initial
+// state (zero) when the tree is new or the cells were reset.
+int prepare_kept_cells(Context* c) {
+    const size_t exp_bytes = sizeof(double) * static_cast<size_t>(c->total_cells) * c->LD;
+    HFMM_TRY(ensure(c, c->d_M, exp_bytes + sizeof(double) * c->LD));
+    HFMM_TRY(ensure(c, c->d_L, exp_bytes));
+    HFMM_TRY(ensure(c, c->d_keep, exp_bytes));
+    const bool many = c->nrhs > 1;
+    if (many) HFMM_TRY(ensure(c, c->d_kept, 2 * exp_bytes * c->nrhs));
+    if (c->cells_dirty) {  // the cells of a new tree, or after hfmm_reset_cells
+        HFMM_TRY(zero_device(c, c->d_M.ptr, exp_bytes));
+        HFMM_TRY(zero_device(c, c->d_L.ptr, exp_bytes));
+        if (many) HFMM_TRY(zero_device(c, c->d_kept.ptr, 2 * exp_bytes * c->nrhs));
+        c->cells_dirty = false;
+    }
+    return HFMM_OK;
+}
+}  // namespace hfmm
 
 extern "C" {
 
@@ -557,16 +582,7 @@ int hfmm_evaluate_levels(hfmm_ctx* h, uint32_t ops, int32_t upper, int32_t lower
             HFMM_TRY(hfmm::zero_device(c, c->d_Lt.ptr, cexp));
         }
     }
-    if (c->keep_cells) {
-        HFMM_TRY(hfmm::ensure(c, c->d_keep, exp_bytes));
-        if (kept_many) HFMM_TRY(hfmm::ensure(c, c->d_kept, 2 * exp_bytes * c->nrhs));
-        if (c->cells_dirty) {  // the cells of a new tree, or after hfmm_reset_cells
-            HFMM_TRY(hfmm::zero_device(c, c->d_M.ptr, exp_bytes));
-            HFMM_TRY(hfmm::zero_device(c, c->d_L.ptr, exp_bytes));
-            if (kept_many) HFMM_TRY(hfmm::zero_device(c, c->d_kept.ptr, 2 * exp_bytes * c->nrhs));
-            c->cells_dirty = false;
-        }
-    }
+    if (c->keep_cells) HFMM_TRY(hfmm::prepare_kept_cells(c));
     // one pass sequence per charge vector (the tree and the lists are shared);
     // with one vector the sorted charges are already in place
     for (int r = 0; r < c->nrhs; ++r) {
@@ -962,6 +978,12 @@ int hfmm_set_stream_overlap(hfmm_ctx* h, int32_t enable) {
     return HFMM_OK;
 }
 
+int hfmm_set_p2p_mutual(hfmm_ctx* h, int32_t enable) {
+    CTX_CHECK(h);
+    h->c.p2p_mutual_on = enable != 0;
+    return HFMM_OK;
+}
+
 int hfmm_set_cell_mode(hfmm_ctx* h, int32_t mode) {
     CTX_CHECK(h);
     Context* c = &h->c;
@@ -999,6 +1021,14 @@ int hfmm_get_m2l_variant(hfmm_ctx* h, int32_t* variant, float* tune_ms) {
 int hfmm_set_profiling(hfmm_ctx* h, int32_t enable) {
     CTX_CHECK(h);
     h->c.profiling = enable != 0;
+    return HFMM_OK;
+}
+
+int hfmm_set_profiling_filter(hfmm_ctx* h, const char* names) {
+    CTX_CHECK(h);
+    Context* c = &h->c;
+    c->prof_filter.clear();
+    if (names && *names) c->prof_filter = "," + std::string(names) + ",";
     return HFMM_OK;
 }
 

@@ -20,6 +20,8 @@ constexpr int kFarSlots = 189;  // M2L slots per cell
 constexpr int kTileTargets = 16;  // targets per M2L tile (the MFMA N dimension)
 constexpr int kP2PBig = 64;    // leaves with more particles run P2P through LDS-staged sources
 constexpr int kP2PItem = 128;  // targets per P2P work item of a big leaf (two per lane of a wave)
+constexpr int kP2PMutualItem = 256;  // targets per work item of the mutual mixed P2P (four per lane)
+constexpr int kNbrForward = 13;  // neighbour slots 0..12 (positions 0..12): the mutual P2P's forward half
 constexpr int kSortedPad = 16; // padding doubles after the sorted particle SoA
 constexpr int kWaveM2LMaxOrder = 6;  // highest order of the wave-level mixed M2L kernel
 
@@ -134,6 +136,19 @@ struct Context {
     DevBuf d_p2p;                             // int32 [nleaf][26]
     DevBuf d_p2pitems;                        // int2 (leaf, first target) work items of the big leaves
     int64_t p2p_items_cap = 0;                // upper bound of the item count (the count itself is on the device)
+    // Mutual mixed P2P (each unordered pair of big leaves evaluated once): work
+    // items of kP2PMutualItem targets, and the source-side sums that a leaf
+    // receives from the items of its big backward neighbours (slots 13..25):
+    // per leaf 13 slot offsets (-1: none) and the first float4 of its slots
+    // (each slot holds one float4 per particle of the leaf)
+    bool p2p_mutual = false;                  // tables built and the partial buffer allocated
+    bool p2p_mutual_on = true;                // hfmm_set_p2p_mutual (the ordered-pair kernel when off)
+    DevBuf d_p2pitems_m;                      // int2 (leaf, first target)
+    int64_t p2p_items_m_cap = 0;
+    DevBuf d_csoff;                           // int32 [nleaf][13]
+    DevBuf d_csbase;                          // int64 [nleaf + 1]
+    DevBuf d_cpart;                           // float4 source-side partial sums
+    int64_t cpart_units = 0;                  // float4 entries in use
     DevBuf d_m2l;                             // int32 [cells of levels 2..h-1][189]
     int64_t m2l_off[kMaxLevels] = {0};        // offset (in cells) of a level in d_m2l
     DevBuf d_order;                           // int32 parity-sorted cell index per level (same offsets as d_m2l)
@@ -175,6 +190,8 @@ struct Context {
 
     // ---- profiling
     bool profiling = false;
+    std::string prof_filter;  // ",name,name," of the kernels to time; empty: every kernel
+    bool prof_open = false;   // prof_begin recorded a start event that prof_end closes
     std::vector<KernelTimer> pending;
     std::vector<hipEvent_t> event_pool;
     std::vector<hfmm_kernel_time> totals;
@@ -191,6 +208,10 @@ struct Context {
 inline bool is_mixed(const Context* c) {
     return c->P.precision == HFMM_PRECISION_MIXED || c->P.precision == HFMM_PRECISION_MIXED_HILO;
 }
+
+// Kept cells: allocates the expansion buffers of the built tree and zeroes them
+// when the tree is new or the cells were reset (hfmm_api.cpp).
+int prepare_kept_cells(Context* c);
 
 // Error helpers ---------------------------------------------------------------
 int fail(Context* c, int code, const std::string& msg);
@@ -233,6 +254,14 @@ int update_charges_device(Context* c, int rhs = 0);
 int m2l_slot_counts(Context* c, int level, unsigned long long* counts);
 // Device counter holding the number of P2P work items of the big leaves.
 inline int32_t* p2p_item_count(Context* c) { return reinterpret_cast<int32_t*>(c->d_count.as<int64_t>() + 12); }
+// Device counters of the mutual mixed P2P: its work item count and the total of
+// its source-side partial sums (float4 entries).
+inline int32_t* p2p_mutual_item_count(Context* c) { return reinterpret_cast<int32_t*>(c->d_count.as<int64_t>() + 16); }
+inline int64_t* p2p_mutual_units(Context* c) { return c->d_count.as<int64_t>() + 17; }
+// Builds the mutual P2P's work items and slot tables on a built tree and sizes
+// the partial buffer (mixed forms); leaves c->p2p_mutual false when the
+// partial sums would exceed a quarter of the free device memory.
+int build_p2p_mutual_tables(Context* c);
 
 // FP64 passes (fmm_device.hip)
 int run_p2m(Context* c);

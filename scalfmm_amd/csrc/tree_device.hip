@@ -237,7 +237,7 @@ int radix_sort(Context* c, uint64_t* keys, int32_t* vals, uint64_t* keys2, int32
     if (n <= 1) return HFMM_OK;
     const int64_t ntiles = (n + kTile - 1) / kTile;
     HFMM_TRY(ensure(c, c->d_hist, sizeof(int32_t) * static_cast<size_t>(256 * ntiles + 1)));
-    HFMM_TRY(ensure(c, c->d_count, sizeof(int64_t) * 16));
+    HFMM_TRY(ensure(c, c->d_count, sizeof(int64_t) * 24));
     int32_t* hist = c->d_hist.as<int32_t>();
     const int passes = (nbits + 7) / 8;
     uint64_t *ka = keys, *kb = keys2;
@@ -446,22 +446,53 @@ __global__ __launch_bounds__(kBlock) void k_m2l_slot_counts(int ncell, const uin
 // returns the count on the host.
 // P2P work items of the big leaves: ceil(count / kP2PItem) items of
 // consecutive targets per leaf with more than kP2PBig particles.
+template <int ITEM>
 __global__ __launch_bounds__(kBlock) void k_p2p_item_counts(int nleaf, const int32_t* __restrict__ lstart,
                                                            int32_t* __restrict__ cnt) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nleaf) return;
     const int m = lstart[i + 1] - lstart[i];
-    cnt[i] = m > kP2PBig ? (m + kP2PItem - 1) / kP2PItem : 0;
+    cnt[i] = m > kP2PBig ? (m + ITEM - 1) / ITEM : 0;
 }
 
+template <int ITEM>
 __global__ __launch_bounds__(kBlock) void k_p2p_items(int nleaf, const int32_t* __restrict__ lstart,
                                                      const int32_t* __restrict__ off, int2* __restrict__ items) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nleaf) return;
     const int m = lstart[i + 1] - lstart[i];
     if (m <= kP2PBig) return;
-    const int k = (m + kP2PItem - 1) / kP2PItem;
-    for (int q = 0; q < k; ++q) items[off[i] + q] = make_int2(i, lstart[i] + q * kP2PItem);
+    const int k = (m + ITEM - 1) / ITEM;
+    for (int q = 0; q < k; ++q) items[off[i] + q] = make_int2(i, lstart[i] + q * ITEM);
+}
+
+// Slot tables of the mutual mixed P2P.  A work item of big leaf A (tile t of
+// kP2PMutualItem targets) evaluates A against its big forward neighbours
+// (slots s < 13) once for both sides; the source-side sums for the particles
+// of neighbour B go to B's slot csoff[B][12 - s] + t, A being B's backward
+// neighbour in slot 25 - s.  Per big leaf B: the 13 offsets (one per backward
+// slot 13..25, -1 when that neighbour is absent or small) and units[B] =
+// (number of slots) x (particles of B) float4 entries.
+__global__ __launch_bounds__(kBlock) void k_colslot_counts(int nleaf, const int32_t* __restrict__ lstart,
+                                                          const int32_t* __restrict__ slots,
+                                                          int32_t* __restrict__ csoff, int64_t* __restrict__ units) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nleaf) return;
+    const int m = lstart[i + 1] - lstart[i];
+    int run = 0;
+    for (int b = 0; b < kNbrForward; ++b) {
+        int o = -1;
+        const int a = slots[static_cast<int64_t>(i) * kNbrSlots + kNbrForward + b];
+        if (m > kP2PBig && a >= 0) {
+            const int na = lstart[a + 1] - lstart[a];
+            if (na > kP2PBig) {
+                o = run;
+                run += (na + kP2PMutualItem - 1) / kP2PMutualItem;
+            }
+        }
+        csoff[static_cast<int64_t>(i) * kNbrForward + b] = o;
+    }
+    units[i] = static_cast<int64_t>(run) * m;
 }
 
 int make_groups(Context* c, const uint64_t* keys, int64_t n, int shift, uint64_t* gkey, int32_t* gstart,
@@ -521,7 +552,7 @@ int build_tree_device(Context* c) {
     HFMM_TRY(ensure(c, c->d_keys2, sizeof(uint64_t) * n));
     HFMM_TRY(ensure(c, c->d_vals, sizeof(int32_t) * n));
     HFMM_TRY(ensure(c, c->d_vals2, sizeof(int32_t) * n));
-    HFMM_TRY(ensure(c, c->d_count, sizeof(int64_t) * 16));
+    HFMM_TRY(ensure(c, c->d_count, sizeof(int64_t) * 24));
     uint64_t* keys = c->d_keys.as<uint64_t>();
     int32_t* vals = c->d_vals.as<int32_t>();
     int32_t* err = reinterpret_cast<int32_t*>(c->d_count.as<int64_t>() + 13);
@@ -636,14 +667,14 @@ int build_tree_device(Context* c) {
         HFMM_TRY(ensure(c, c->d_flags, sizeof(int32_t) * static_cast<size_t>(nl + 1)));
         HFMM_TRY(ensure(c, c->d_scan, sizeof(int32_t) * static_cast<size_t>(nl + 1)));
         prof_begin(c, "p2p_items");
-        hipLaunchKernelGGL(k_p2p_item_counts, dim3(blocks_for(nl)), dim3(kBlock), 0, c->stream, static_cast<int>(nl),
+        hipLaunchKernelGGL(k_p2p_item_counts<kP2PItem>, dim3(blocks_for(nl)), dim3(kBlock), 0, c->stream, static_cast<int>(nl),
                            c->d_leafstart.as<int32_t>(), c->d_flags.as<int32_t>());
         prof_end(c);
         HFMM_TRY(check_launch(c, "p2p_items"));
         HFMM_TRY(exclusive_scan<int32_t>(c, c->d_flags.as<int32_t>(), c->d_scan.as<int32_t>(), nl, p2p_item_count(c),
                                          "p2p_items"));
         prof_begin(c, "p2p_items");
-        hipLaunchKernelGGL(k_p2p_items, dim3(blocks_for(nl)), dim3(kBlock), 0, c->stream, static_cast<int>(nl),
+        hipLaunchKernelGGL(k_p2p_items<kP2PItem>, dim3(blocks_for(nl)), dim3(kBlock), 0, c->stream, static_cast<int>(nl),
                            c->d_leafstart.as<int32_t>(), c->d_scan.as<int32_t>(), c->d_p2pitems.as<int2>());
         prof_end(c);
         HFMM_TRY(check_launch(c, "p2p_items"));
@@ -676,6 +707,54 @@ int build_tree_device(Context* c) {
     }
     c->tree_built = true;
     c->evaluated = false;
+    c->p2p_mutual = false;
+    if (is_mixed(c)) HFMM_TRY(build_p2p_mutual_tables(c));
+    return HFMM_OK;
+}
+
+int build_p2p_mutual_tables(Context* c) {
+    c->p2p_mutual = false;
+    const int64_t nl = c->ncell[c->height() - 1];
+    const int32_t* lstart = c->d_leafstart.as<int32_t>();
+    c->p2p_items_m_cap = c->n / kP2PMutualItem + c->n / (kP2PBig + 1) + 1;
+    HFMM_TRY(ensure(c, c->d_p2pitems_m, sizeof(int2) * static_cast<size_t>(c->p2p_items_m_cap)));
+    HFMM_TRY(ensure(c, c->d_flags, sizeof(int32_t) * static_cast<size_t>(nl + 1)));
+    HFMM_TRY(ensure(c, c->d_scan, sizeof(int32_t) * static_cast<size_t>(nl + 1)));
+    prof_begin(c, "p2p_items");
+    hipLaunchKernelGGL(k_p2p_item_counts<kP2PMutualItem>, dim3(blocks_for(nl)), dim3(kBlock), 0, c->stream,
+                       static_cast<int>(nl), lstart, c->d_flags.as<int32_t>());
+    prof_end(c);
+    HFMM_TRY(check_launch(c, "p2p_items"));
+    HFMM_TRY(exclusive_scan<int32_t>(c, c->d_flags.as<int32_t>(), c->d_scan.as<int32_t>(), nl,
+                                     p2p_mutual_item_count(c), "p2p_items"));
+    prof_begin(c, "p2p_items");
+    hipLaunchKernelGGL(k_p2p_items<kP2PMutualItem>, dim3(blocks_for(nl)), dim3(kBlock), 0, c->stream,
+                       static_cast<int>(nl), lstart, c->d_scan.as<int32_t>(), c->d_p2pitems_m.as<int2>());
+    prof_end(c);
+    HFMM_TRY(check_launch(c, "p2p_items"));
+    HFMM_TRY(ensure(c, c->d_csoff, sizeof(int32_t) * static_cast<size_t>(nl) * kNbrForward));
+    HFMM_TRY(ensure(c, c->d_csbase, sizeof(int64_t) * static_cast<size_t>(nl + 1)));
+    int64_t* base = c->d_csbase.as<int64_t>();
+    prof_begin(c, "p2p_items");
+    hipLaunchKernelGGL(k_colslot_counts, dim3(blocks_for(nl)), dim3(kBlock), 0, c->stream, static_cast<int>(nl), lstart,
+                       c->d_p2p.as<int32_t>(), c->d_csoff.as<int32_t>(), base);
+    prof_end(c);
+    HFMM_TRY(check_launch(c, "p2p_items"));
+    HFMM_TRY(exclusive_scan<int64_t>(c, base, base, nl, p2p_mutual_units(c), "p2p_items"));
+    int64_t units = 0;
+    HFMM_HIP(c, hipMemcpyAsync(&units, p2p_mutual_units(c), sizeof(int64_t), hipMemcpyDeviceToHost, c->stream));
+    HFMM_HIP(c, hipStreamSynchronize(c->stream));
+    // base[nl] = the total, so a leaf's slot count is (base[i+1] - base[i]) / n_i
+    HFMM_HIP(c, hipMemcpyAsync(base + nl, p2p_mutual_units(c), sizeof(int64_t), hipMemcpyDeviceToDevice, c->stream));
+    const size_t bytes = sizeof(float) * 4 * static_cast<size_t>(units > 0 ? units : 1);
+    if (bytes > c->d_cpart.bytes) {
+        size_t free_b = 0, total_b = 0;
+        HFMM_HIP(c, hipMemGetInfo(&free_b, &total_b));
+        if (bytes > (free_b + c->d_cpart.bytes) / 4) return HFMM_OK;  // ordered-pair kernel instead
+    }
+    HFMM_TRY(ensure(c, c->d_cpart, bytes));
+    c->cpart_units = units;
+    c->p2p_mutual = true;
     return HFMM_OK;
 }
 

@@ -191,6 +191,10 @@ class Engine:
         """Run the P2P on a second stream beside P2M..L2L (default off)."""
         self._check(self._L.hfmm_set_stream_overlap(self.h, 1 if on else 0))
 
+    def set_p2p_mutual(self, on=True):
+        """Mixed forms: each unordered pair of two big neighbour leaves once (default on)."""
+        self._check(self._L.hfmm_set_p2p_mutual(self.h, 1 if on else 0))
+
     def reset_cells(self):
         """tree.forEachCell(resetToInitialState) for the kept cells."""
         self._check(self._L.hfmm_reset_cells(self.h))
@@ -214,6 +218,35 @@ class Engine:
         if n:
             self.n, self.nrhs, self.nsets = n, int(nvals), 1
             self.tree_stamp += 1
+
+    def _cell_args(self, blocks, multipoles, locals_):
+        nb = len(blocks)
+        for arrs in (multipoles, locals_):
+            if arrs is None:
+                continue
+            if len(arrs) != nb:
+                raise ValueError("one expansion buffer per cell block")
+            for a in arrs:
+                if a is not None and (a.dtype != np.float64 or not a.flags.c_contiguous):
+                    raise ValueError("expansion buffers must be contiguous float64 arrays")
+        bp = (C.c_void_p * max(nb, 1))(*[b.ctypes.data for b in blocks])
+
+        def ptrs(arrs):
+            if arrs is None:
+                return None
+            return (C.POINTER(C.c_double) * max(nb, 1))(*[_dp(a) if a is not None else None for a in arrs])
+        return nb, bp, ptrs(multipoles), ptrs(locals_)
+
+    def get_cell_blocks(self, level, blocks, multipoles, locals_, nvals=1):
+        """hfmm_get_cell_blocks: the expansions of `level` into the cell
+        groups' multipole / local buffers (FChebCellPOD layout)."""
+        nb, bp, mp, lp = self._cell_args(blocks, multipoles, locals_)
+        self._check(self._L.hfmm_get_cell_blocks(self.h, int(level), nb, bp, mp, lp, int(nvals)))
+
+    def set_cell_blocks(self, level, blocks, multipoles, locals_, nvals=1):
+        """hfmm_set_cell_blocks: the cell groups' expansions into the kept cells."""
+        nb, bp, mp, lp = self._cell_args(blocks, multipoles, locals_)
+        self._check(self._L.hfmm_set_cell_blocks(self.h, int(level), nb, bp, mp, lp, int(nvals)))
 
     def synchronize(self):
         self._check(self._L.hfmm_synchronize(self.h))
@@ -335,6 +368,10 @@ class Engine:
     # ---- timing ---------------------------------------------------------------
     def set_profiling(self, on=True):
         self._check(self._L.hfmm_set_profiling(self.h, 1 if on else 0))
+
+    def set_profiling_filter(self, names=()):
+        """Time only the named kernels (an empty list: every kernel)."""
+        self._check(self._L.hfmm_set_profiling_filter(self.h, ",".join(names).encode()))
 
     def kernel_times(self):
         cnt = C.c_int32()

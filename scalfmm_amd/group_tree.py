@@ -10,16 +10,28 @@ blocks in the reference's byte layout (paths relative to its source tree):
   FGroupSeqAlgorithm(&tree, &kernel).execute(ops[, upper, lower])
       GroupTree/Core/FGroupSeqAlgorithm.hpp:61-83, Core/FCoreCommon.hpp:65-94
 
+  FGroupOfCells with FChebCellPOD cells  GroupTree/Core/FGroupOfCells.hpp:18-370,
+                                          GroupTree/Chebyshev/FChebCellPOD.hpp:11-112,
+                                          GroupTree/Core/FGroupTree.hpp:288-356 (upper levels)
+
 Each particle block is a byte buffer laid out as the reference allocates it
 (a 64-byte header, 24-byte leaf headers, then x, y, z and the NVALS physical
 values as arrays padded to 64-byte packs) plus a separate attributes buffer
 (potentials and forces).  execute() hands the raw buffers to the engine's
 block entry (hfmm_evaluate_particle_blocks), which reads the particles out of
-the blocks and adds the results into the attributes buffers.  The cell blocks
-of the reference's tree are not built: multipoles and locals live in the
-engine, which keeps them between executions on the same particles as the
-reference's tree keeps its cells (every pass adds into them), until
-resetCells() (the reference's forEachCell(resetToInitialState)).
+the blocks and adds the results into the attributes buffers.
+
+With an order, the tree also holds the reference's cell blocks of levels
+height-1 .. 1: per block a raw buffer (24-byte header, Morton indexes, one
+24-byte FBasicCellPOD per cell) and the multipole and local buffers
+(nvals x 2 p^3 doubles per cell, FChebCellPOD's layout).  execute() copies the
+blocks' expansions of levels 2 .. height-1 into the engine's kept cells
+(hfmm_set_cell_blocks), evaluates, and copies them back
+(hfmm_get_cell_blocks), so the blocks hold what the reference's cells hold
+after its passes; a caller may read or seed them between executions.
+Without an order the engine alone keeps the cells between executions on the
+same particles, as the reference's tree keeps its cells (every pass adds into
+them), until resetCells() (the reference's forEachCell(resetToInitialState)).
 """
 from __future__ import annotations
 
@@ -170,18 +182,92 @@ class FGroupOfParticles:
             fn(self.getLeaf(i))
 
 
+_CELL_HEADER = np.dtype([("startingIndex", "<i8"), ("endingIndex", "<i8"), ("numberOfCellsInBlock", "<i4"),
+                         ("pad", "<i4")])
+_CELL_POD = np.dtype([("mortonIndex", "<i8"), ("coordinates", "<i4", (3,)), ("level", "<i4")])
+assert _CELL_HEADER.itemsize == 24 and _CELL_POD.itemsize == 24
+
+
+def morton_to_coordinate(key, level):
+    """FTreeCoordinate::setPositionFromMorton (FTreeCoordinate.hpp:126-142)."""
+    c = [0, 0, 0]
+    for b in range(level):
+        for d, shift in enumerate((2, 1, 0)):
+            c[d] |= ((int(key) >> (3 * b + shift)) & 1) << b
+    return c
+
+
+class FGroupOfCells:
+    """One cell block (FGroupOfCells.hpp:125-168 allocates it) with the
+    Chebyshev cells' multipole and local buffers."""
+
+    def __init__(self, starting_index, ending_index, keys, level, order, nvals=1):
+        n = len(keys)
+        self.raw = np.zeros(_CELL_HEADER.itemsize + 8 * n + _CELL_POD.itemsize * n, dtype=np.uint8)
+        self.header = self.raw[:24].view(_CELL_HEADER)
+        self.header["startingIndex"] = starting_index
+        self.header["endingIndex"] = ending_index
+        self.header["numberOfCellsInBlock"] = n
+        self.cell_indexes = self.raw[24:24 + 8 * n].view(np.int64)
+        self.cell_indexes[:] = keys
+        self.cells = self.raw[24 + 8 * n:].view(_CELL_POD)
+        self.cells["mortonIndex"] = keys
+        # the particle constructor sets the coordinates, not the level (FGroupTree.hpp:257-263, 341-347)
+        self.cells["coordinates"] = [morton_to_coordinate(k, level) for k in keys]
+        self.vector_size = 2 * order ** 3  # FChebCellPOD::VectorSize
+        self.nvals = int(nvals)
+        self.multipoles = np.zeros(n * self.nvals * self.vector_size)
+        self.locals = np.zeros(n * self.nvals * self.vector_size)
+
+    def getRawBuffer(self):
+        return self.raw
+
+    def getRawMultipoleBuffer(self):
+        return self.multipoles
+
+    def getRawLocalBuffer(self):
+        return self.locals
+
+    def getNumberOfCellsInBlock(self):
+        return int(self.header["numberOfCellsInBlock"][0])
+
+    def getStartingIndex(self):
+        return int(self.header["startingIndex"][0])
+
+    def getEndingIndex(self):
+        return int(self.header["endingIndex"][0])
+
+    def getCellMortonIndex(self, i):
+        return int(self.cell_indexes[i])
+
+    def getMultipole(self, i, rhs=0):
+        """FChebCellPOD::getMultipole(rhs) of cell i (a view)."""
+        o = (i * self.nvals + rhs) * self.vector_size
+        return self.multipoles[o:o + self.vector_size]
+
+    def getLocal(self, i, rhs=0):
+        o = (i * self.nvals + rhs) * self.vector_size
+        return self.locals[o:o + self.vector_size]
+
+
+def _cell_blocks(keys, level, nb_elements_per_block, order, nvals):
+    return [FGroupOfCells(int(k[0]), int(k[-1]) + 1, k, level, order, nvals)
+            for k in (keys[b:b + nb_elements_per_block] for b in range(0, len(keys), nb_elements_per_block))]
+
+
 class FGroupTree:
     """The leaf level of the reference's group tree over a particle set:
     particles sorted by leaf Morton index (stable: insertion order inside a
     leaf), then nbElementsPerBlock leaves per particle block."""
 
-    def __init__(self, tree_height, box_width, box_center, nb_elements_per_block, x, y, z, q):
+    def __init__(self, tree_height, box_width, box_center, nb_elements_per_block, x, y, z, q, order=None):
         if nb_elements_per_block < 1:
             raise ValueError("nb_elements_per_block must be >= 1")
         self.tree_height = int(tree_height)
         self.box_width = float(box_width)
         self.box_center = tuple(float(c) for c in box_center)
         self.nb_elements_per_block = int(nb_elements_per_block)
+        self.order = None if order is None else int(order)  # of the Chebyshev cells
         self._reset_pending = False
         x, y, z = (np.ascontiguousarray(a, dtype=np.float64).reshape(-1) for a in (x, y, z))
         q = np.asarray(q, dtype=np.float64)
@@ -209,6 +295,16 @@ class FGroupTree:
                     g._pos[3 + v][off:off + c] = q[v, src]
                 off = g.newLeaf(int(k), i, int(c), off)
             self.groups.append(g)
+        # cell blocks: the leaves as the particle blocks group them, then per
+        # level the distinct parent keys, nbElementsPerBlock per block
+        # (FGroupTree.hpp:288-356), levels height-1 down to 1
+        self.cell_groups = {}
+        if self.order is not None:
+            level_keys = leaf_keys.astype(np.int64)
+            for level in range(self.tree_height - 1, 0, -1):
+                self.cell_groups[level] = _cell_blocks(level_keys, level, self.nb_elements_per_block,
+                                                       self.order, self.nvals)
+                level_keys = np.unique(level_keys >> 3)
 
     def getHeight(self):
         return self.tree_height
@@ -223,6 +319,16 @@ class FGroupTree:
         """tree.forEachCell([](cell){ cell->resetToInitialState(); }): the next
         execution starts from zero multipoles and locals."""
         self._reset_pending = True
+        for blocks in self.cell_groups.values():
+            for g in blocks:
+                g.multipoles[:] = 0.0
+                g.locals[:] = 0.0
+
+    def getNbCellGroupAtLevel(self, level):
+        return len(self.cell_groups.get(level, ()))
+
+    def getCellGroup(self, level, i):
+        return self.cell_groups[level][i]
 
     def getNbParticleGroup(self):
         return len(self.groups)
@@ -263,11 +369,28 @@ class FGroupSeqAlgorithm:
 
     def execute(self, operations=NEAR_AND_FAR, upper_level=2, lower_level=None):
         t = self.tree
+        e = self.kernel.engine
         if t._reset_pending:
-            self.kernel.engine.reset_cells()
+            e.reset_cells()
             t._reset_pending = False
-        self.kernel.engine.evaluate_particle_blocks([g.raw for g in t.groups], [g.attributes_raw for g in t.groups],
-                                                    t.nvals, operations, upper_level, lower_level)
+        blocks = [g.raw for g in t.groups]
+        attrs = [g.attributes_raw for g in t.groups]
+        levels = [lv for lv in sorted(t.cell_groups) if 2 <= lv < t.tree_height]
+        if levels:
+            if t.order != self.kernel.order:
+                raise ValueError("tree and kernel disagree on the order")
+            # the engine's tree over the blocks' particles (no pass), then the
+            # blocks' cells into its kept cells (the symmetric scheme; the
+            # others keep their cells in the engine)
+            e.evaluate_particle_blocks(blocks, attrs, t.nvals, 0)
+            for lv in levels if e.params.m2l_scheme == _abi.M2L_SYMMETRIC else ():
+                cg = t.cell_groups[lv]
+                e.set_cell_blocks(lv, [g.raw for g in cg], [g.multipoles for g in cg], [g.locals for g in cg],
+                                  t.nvals)
+        e.evaluate_particle_blocks(blocks, attrs, t.nvals, operations, upper_level, lower_level)
+        for lv in levels:
+            cg = t.cell_groups[lv]
+            e.get_cell_blocks(lv, [g.raw for g in cg], [g.multipoles for g in cg], [g.locals for g in cg], t.nvals)
         return t
 
 

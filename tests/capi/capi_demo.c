@@ -1,7 +1,10 @@
 /* A C program written against the reference's public C API for its Chebyshev
  * kernel (include/scalfmm_capi.h): uniform particles, one evaluation, the
- * energy and the first potential printed.  Compiled and linked by
- * tests/test_capi_build.py; the GPU tests drive the same calls through ctypes. */
+ * energy and the first potential printed.  Usage: capi_demo [n [output]];
+ * with an output path it writes n (int64), the positions (3 n, xyz per
+ * particle), the charges, the potentials and the forces (3 n) as doubles.
+ * Compiled and linked by tests/test_capi_build.py, run by
+ * tests/test_gpu_integration.py. */
 #define _XOPEN_SOURCE 600
 #include <stdio.h>
 #include <stdlib.h>
@@ -35,6 +38,17 @@ int main(int argc, char** argv) {
     double timers[9] = {0};
     scalfmm_get_timers(h, timers);
     printf("n %d energy %.12e pot[0] %.12e fx[0] %.12e m2l %.3e s\n", n, energy, pot[0], f[0], timers[2]);
+    if (argc > 2) {
+        FILE* out = fopen(argv[2], "wb");
+        if (!out) return 1;
+        const long long nn = n;
+        fwrite(&nn, sizeof(nn), 1, out);
+        fwrite(xyz, sizeof(double), 3 * (size_t)n, out);
+        fwrite(q, sizeof(double), (size_t)n, out);
+        fwrite(pot, sizeof(double), (size_t)n, out);
+        fwrite(f, sizeof(double), 3 * (size_t)n, out);
+        fclose(out);
+    }
     scalfmm_dealloc_handle(h, NULL);
     free(xyz);
     free(q);

@@ -26,3 +26,17 @@ def test_capi_exports_and_links(tmp_path):
                         "-L", os.path.dirname(lib), "-lscalfmm_hip", "-Wl,-rpath," + os.path.dirname(lib)],
                        capture_output=True, text=True)
     assert r.returncode == 0, r.stderr
+
+
+def test_adapter_header_compiles(tmp_path):
+    """include/scalfmm_hip_algorithm.hpp with the demo's own tree types: a C++
+    program compiles and links against it (no GPU; run by
+    tests/test_gpu_integration.py)."""
+    from scalfmm_amd import _abi
+    lib = _abi.LIB_PATH
+    exe = tmp_path / "adapter_demo"
+    r = subprocess.run(["g++", "-std=c++17", "-O1", "-Wall", "-Wextra", "-Werror", "-I", os.path.join(ROOT, "include"),
+                        os.path.join(ROOT, "tests", "adapter", "adapter_demo.cpp"), "-o", str(exe),
+                        "-L", os.path.dirname(lib), "-lscalfmm_hip", "-Wl,-rpath," + os.path.dirname(lib)],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr

@@ -192,3 +192,31 @@ def test_gradient_fd_full_size():
     eq = float(np.max(np.abs(np.array(gq) - np.array(fq))) / np.max(np.abs(fq)))
     _say(f"  finite differences at n={n}: dL/dx {ex:.2e}, dL/dq {eq:.2e} (max error / max entry)")
     assert ex < 1e-4 and eq < 1e-5
+
+
+def test_cpp_function_device_tensors():
+    """The libtorch C++ autograd function (csrc/torch_potential.cpp) on device
+    tensors, in a process that imports torch before the engine
+    (tests/autograd_device_run.py): at 4,000 particles the same bars against
+    the FP64 direct-sum gradient as above, and the host-tensor Function on the
+    same engine within 1e-13 (FP64) / 1e-6 (mixed, both FP32 near fields);
+    float32 inputs get float32 gradients; at the differentiable configuration
+    (2M, height 7, order 6, mixed) the sampled bars of test_gradient_sampled."""
+    import json
+    import os
+    import subprocess
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    r = subprocess.run([sys.executable, "-u", os.path.join(root, "tests", "autograd_device_run.py")],
+                       capture_output=True, text=True, timeout=600)
+    _say(r.stdout[-2000:] + r.stderr[-2000:])
+    assert r.returncode == 0, r.stderr[-2000:]
+    out = json.loads(r.stdout.strip().splitlines()[-1])
+    for mode in ("fp64", "mixed"):
+        m = out[mode]
+        assert m["dq_vs_direct"] < 2e-6 and m["dx_vs_direct"] < 2e-5, m
+        host_tol = 1e-13 if mode == "fp64" else 1e-6
+        assert m["phi_vs_host"] < host_tol and m["dq_vs_host"] < host_tol and m["dx_vs_host"] < host_tol, m
+        assert m["grad_device"].startswith("cuda") and m["grad_dtype"] == "torch.float64"
+    assert out["float32_grad_dtypes"] == ["torch.float32", "torch.float32"]
+    f = out["full"]
+    assert f["dq_vs_direct"] < 1e-5 and f["dx_vs_direct"] < 1e-4, f

@@ -114,3 +114,37 @@ def test_tree_several_physical_values():
             a[:] = k
     pot, fx, fy, fz = t.results(idx_vals=1)
     assert np.all(pot == 1) and np.all(fx == 3) and np.all(fy == 5) and np.all(fz == 7)
+
+
+def test_cell_blocks_layout(oracle, fixture20k):
+    """Cell blocks of every level (FGroupTree.hpp:224-356, FGroupOfCells.hpp:
+    125-168): 24-byte header, Morton indexes, 24-byte FBasicCellPOD per cell
+    (FBasicCellPOD.hpp:9-13) with the coordinates of the Morton index
+    (checked against the oracle's decode); nbElementsPerBlock cells per block;
+    the cells of a level are the oracle's cell keys; FChebCellPOD buffers of
+    nvals x 2 p^3 doubles per cell (FChebCellPOD.hpp:11-21)."""
+    d = fixture20k["data"]
+    h, per, order, nvals = 5, 100, 4, 2
+    q = np.stack([d[:, 3], -d[:, 3]])
+    t = gt.FGroupTree(h, fixture20k["width"], fixture20k["center"], per, d[:, 0], d[:, 1], d[:, 2], q, order=order)
+    ot = oracle.Tree(d[:, 0], d[:, 1], d[:, 2], fixture20k["center"], fixture20k["width"], h)
+    assert sorted(t.cell_groups) == list(range(1, h))
+    for level in range(1, h):
+        blocks = [t.getCellGroup(level, i) for i in range(t.getNbCellGroupAtLevel(level))]
+        keys = np.concatenate([g.cell_indexes for g in blocks])
+        if level >= 2:
+            np.testing.assert_array_equal(keys, ot.cell_keys(level))
+        for i, g in enumerate(blocks):
+            n = g.getNumberOfCellsInBlock()
+            assert n == per or i == len(blocks) - 1
+            assert g.raw.nbytes == 24 + 8 * n + 24 * n
+            assert g.getStartingIndex() == g.cell_indexes[0] and g.getEndingIndex() == g.cell_indexes[-1] + 1
+            np.testing.assert_array_equal(g.cells["mortonIndex"], g.cell_indexes)
+            assert g.multipoles.size == n * nvals * 2 * order ** 3 == g.locals.size
+            for c in (0, n - 1):
+                coord = oracle.demorton(int(g.cell_indexes[c]))
+                assert list(g.cells["coordinates"][c]) == list(coord)
+            assert np.all(g.cells["level"] == 0)
+    t.getCellGroup(2, 0).getLocal(0, 1)[:] = 3.0
+    t.resetCells()
+    assert not t.getCellGroup(2, 0).locals.any()
