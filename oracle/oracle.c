@@ -906,6 +906,26 @@ void orc_chebg_destroy(orc_chebg* g) {
 int orc_chebg_rank(const orc_chebg* g) { return g->k; }
 const orc_cheb* orc_chebg_cheb(const orc_chebg* g) { return g->c; }
 
+/* Replaces the rank and the operators U, B (nn x k, column-major, un-weighted)
+ * and C ([343][k x k] column-major) of a handle: the tests apply operators
+ * built from another basis of the same singular subspaces (the choice inside
+ * a degenerate singular value that a truncation splits is the SVD
+ * implementation's, FChebM2LHandler.hpp:459-484). */
+int orc_chebg_set(orc_chebg* g, int k, const double* U, const double* B, const double* C) {
+    if (!g || k <= 0 || k > g->c->nn || !U || !B || !C) return ORC_ERR_ARG;
+    const int nn = g->c->nn;
+    double* nU = (double*)malloc(sizeof(double) * (size_t)nn * k);
+    double* nB = (double*)malloc(sizeof(double) * (size_t)nn * k);
+    double* nC = (double*)malloc(sizeof(double) * (size_t)343 * k * k);
+    if (!nU || !nB || !nC) { free(nU); free(nB); free(nC); return ORC_ERR_NOMEM; }
+    memcpy(nU, U, sizeof(double) * (size_t)nn * k);
+    memcpy(nB, B, sizeof(double) * (size_t)nn * k);
+    memcpy(nC, C, sizeof(double) * (size_t)343 * k * k);
+    free(g->U); free(g->B); free(g->C);
+    g->U = nU; g->B = nB; g->C = nC; g->k = k;
+    return ORC_OK;
+}
+
 void orc_chebg_get(const orc_chebg* g, double* U, double* B, double* C, double* S1, double* S2) {
     const int nn = g->c->nn, k = g->k;
     if (U) memcpy(U, g->U, sizeof(double) * (size_t)nn * k);

@@ -179,6 +179,7 @@ void orc_chebg_destroy(orc_chebg* g);
 int orc_chebg_rank(const orc_chebg* g);
 const orc_cheb* orc_chebg_cheb(const orc_chebg* g);
 void orc_chebg_get(const orc_chebg* g, double* U, double* B, double* C, double* S1, double* S2);
+int orc_chebg_set(orc_chebg* g, int k, const double* U, const double* B, const double* C);
 /* The FMM with FChebKernel (Kernels/Chebyshev/FChebKernel.hpp:93-200): P2M and
  * M2M followed by the compression M~ = B^T M, M2L on the compressed
  * expansions (L~ += 2/w C_t M~), L2L and L2P preceded by L += U L~; the working

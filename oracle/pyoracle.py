@@ -96,6 +96,7 @@ def lib():
     L.orc_unif_create.argtypes = [C.c_int, C.POINTER(vp)]
     L.orc_chebg_rank.argtypes = [vp]
     L.orc_chebg_get.argtypes = [vp, _f64p, _f64p, _f64p, _f64p, _f64p]
+    L.orc_chebg_set.argtypes = [vp, C.c_int, _f64p, _f64p, _f64p]
     L.orc_fmm_global.argtypes = [vp, C.c_int64, _f64p, _f64p, _f64p, _f64p, _f64p, C.c_double, C.c_int,
                                  C.c_uint, C.c_int, C.c_int, C.c_int, _f64p, _f64p, _f64p, _f64p, C.POINTER(Stats)]
     L.orc_run_execute.argtypes = [vp, C.c_uint, C.c_int, C.c_int, _f64p, _f64p, _f64p, _f64p]
@@ -379,6 +380,17 @@ class ChebGlobal:
         if getattr(self, "h", None) and _lib is not None:
             _lib.orc_chebg_destroy(self.h)
             self.h = None
+
+    def set_operators(self, U, B, Cm):
+        """Replaces U, B (nnodes x k) and C (343 x k x k, C[pos][i, j])."""
+        k = U.shape[1]
+        Uf = np.ascontiguousarray(U.T).reshape(-1)
+        Bf = np.ascontiguousarray(B.T).reshape(-1)
+        Cf = np.ascontiguousarray(np.asarray(Cm).transpose(0, 2, 1)).reshape(-1)
+        st = lib().orc_chebg_set(self.h, int(k), Uf, Bf, Cf)
+        if st:
+            raise RuntimeError(f"orc_chebg_set failed: {st}")
+        self.rank = int(k)
 
     def operators(self):
         """U, B (nnodes x k), C (343 x k x k with C[pos][i, j]), S1, S2."""

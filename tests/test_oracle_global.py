@@ -141,3 +141,74 @@ def test_uniform_kernel_reproduces_published_errors(oracle, fixture20k):
     for k in range(4):
         err = oracle.accuracy(d[:, 4 + k], got[k])["rel_l2"]
         assert abs(err - published[k]) <= 5e-6 * published[k], (k, err, published[k])
+
+
+# Data/ResultCompareKernels.txt:18-23 (FChebKernel, order 7, height 5, the 20k
+# fixture): potential and force errors against the direct sums
+PUBLISHED_CHEB = {"pot": 1.12952e-08, "fx": 1.66854e-06, "fy": 1.75929e-06, "fz": 1.80712e-06}
+
+
+def test_global_kernel_published_digits_within_the_degenerate_choice(oracle, fixture20k):
+    """Why FChebKernel's published errors are not reproduced digit for digit.
+
+    At order 7 the singular values 150, 151 and 152 of K_col and K_row are
+    one triply degenerate value (the cube's axis permutations), well apart
+    from 149 and 153, and the rank rule keeps 150 of them: the rank the
+    reference printed, its M2L operator size 343 k^2 8 = 61,740,000 bytes
+    (FChebM2LHandler.hpp:96-103).  Which vector of the degenerate plane is kept
+    is the SVD routine's choice (LAPACK's dgesvd on the reference's machine).
+    Operators from other choices inside the triplet (independently for U and
+    B) span errors that contain every published value, so the published
+    digits are one of the reference-conforming results, and the oracle's
+    (Jacobi) choice another."""
+    p, nn, eps = 7, 343, 1e-7
+    X = _nodes(oracle, p)
+    w = _weights(oracle, p)
+    Ks, pos = [], []
+    for i in range(-3, 4):
+        for j in range(-3, 4):
+            for k in range(-3, 4):
+                if max(abs(i), abs(j), abs(k)) <= 1:
+                    continue
+                Y = X + 2.0 * np.array([i, j, k])
+                Ks.append(w[:, None] * w[None, :] / np.linalg.norm(X[:, None, :] - Y[None, :, :], axis=2))
+                pos.append((i + 3) * 49 + (j + 3) * 7 + (k + 3))
+    _, S1, V1t = np.linalg.svd(np.concatenate(Ks, 0), full_matrices=False)  # K_col -> B
+    U2, S2, _ = np.linalg.svd(np.concatenate(Ks, 1), full_matrices=False)   # K_row -> U
+    for S in (S1, S2):
+        tail = np.sqrt(np.cumsum((S ** 2)[::-1])[::-1] / np.sum(S ** 2))
+        k = int(np.argmax(tail <= eps))  # FSvd::getRank: smallest k with the tail below eps
+        assert k == 150 and 343 * k * k * 8 == 61_740_000
+        assert abs(S[149] / S[150] - 1) < 1e-8 and abs(S[150] / S[151] - 1) < 1e-8  # sigma 150..152
+        assert S[148] / S[149] > 1.02 and S[151] / S[152] > 1.1                      # isolated
+    V1 = V1t.T
+    P = [U2[:, :152].T @ K @ V1[:, :152] for K in Ks]  # weighted operators in the leading bases
+    g = oracle.ChebGlobal(p, dense=True)  # an order-7 handle whose operators are replaced below
+    d = fixture20k["data"]
+    x, y, z, q = (d[:, c].copy() for c in range(4))
+
+    def errors(a, b):
+        Ua = np.concatenate([U2[:, :149], U2[:, 149:152] @ a[:, None]], 1)
+        Bb = np.concatenate([V1[:, :149], V1[:, 149:152] @ b[:, None]], 1)
+        Ea = np.zeros((152, 150))
+        Ea[:149, :149] = np.eye(149)
+        Eb = Ea.copy()
+        Ea[149:152, 149] = a
+        Eb[149:152, 149] = b
+        C = np.zeros((343, 150, 150))
+        for t, Pt in zip(pos, P):
+            C[t] = Ea.T @ Pt @ Eb
+        g.set_operators(Ua / w[:, None], Bb / w[:, None], C)
+        got = oracle.fmm_global(g, x, y, z, q, fixture20k["center"], fixture20k["width"], 5)
+        return [float(np.linalg.norm(got[c] - d[:, 4 + c]) / np.linalg.norm(d[:, 4 + c])) for c in range(4)]
+
+    rng = np.random.default_rng(1)
+    runs = [errors(np.eye(3)[c], np.eye(3)[c]) for c in range(3)]
+    for _ in range(12):
+        a, b = rng.normal(size=3), rng.normal(size=3)
+        runs.append(errors(a / np.linalg.norm(a), b / np.linalg.norm(b)))
+    runs = np.array(runs)
+    for c, name in enumerate(("pot", "fx", "fy", "fz")):
+        lo, hi = runs[:, c].min(), runs[:, c].max()
+        assert lo <= PUBLISHED_CHEB[name] <= hi, (name, lo, PUBLISHED_CHEB[name], hi)
+        assert hi - lo > 1e-3 * PUBLISHED_CHEB[name]  # the choice moves the fourth digit
