@@ -75,8 +75,7 @@ def test_cpp_adapter_program(oracle, tmp_path):
         assert _rel(full[k], ref6[k]) <= 1e-12, k
         assert _rel(classic[k], ref7[k]) <= 1e-12, k
         assert np.array_equal(split[k], full[k]), k
-    run = oracle.Run(oracle.Cheb(6), x, y, z, q, f["center"], f["width"], 5)
-    run.execute()
+    run = oracle.Run(oracle.Cheb(6), x, y, z, q, f["center"], f["width"], 5)  # one full execution
     assert np.array_equal(keys, run.tree.cell_keys(3))
     OM, OL = run.expansions(3)
     assert _rel(M, OM) <= 1e-13 and _rel(L, OL) <= 1e-12
