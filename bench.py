@@ -137,8 +137,8 @@ def cpu_baseline(dist, n, h, p):
 # source file of each profiled kernel: a committed summary's traffic is used
 # only while that file is unchanged (its sha256 is recorded in the summary)
 KERNEL_SOURCE = {
-    "k_p2p_mixed_big": "fmm_mixed.hip", "k_p2p_mixed_mutual": "fmm_mixed.hip", "k_m2l_mixed": "fmm_mixed.hip",
-    "k_p2p_mixed": "fmm_mixed.hip", "k_m2l_f64": "fmm_device.hip", "k_p2p_f64_big": "fmm_device.hip",
+    "k_p2p_mixed_big": "p2p_mixed.hip", "k_p2p_mixed_mutual": "p2p_mixed.hip", "k_m2l_mixed": "fmm_mixed.hip",
+    "k_p2p_mixed": "p2p_mixed.hip", "k_m2l_f64": "fmm_device.hip", "k_p2p_f64_big": "fmm_device.hip",
     "k_p2p_f64": "fmm_device.hip", "k_unif_m2l": "fmm_global.hip",
 }
 
