@@ -291,6 +291,19 @@ int hfmm_get_results_device(hfmm_ctx* ctx, const double** pot, const double** fx
 int hfmm_direct(hfmm_ctx* ctx, int64_t ntargets, const int64_t* targets, double* pot, double* fx, double* fy,
                 double* fz);
 
+/* hfmm_direct without device allocation: the same kernel and the same results,
+ * bit for bit, with the target indices and results staged in one fixed
+ * device array per GPU (HFMM_DIRECT_NOALLOC_MAX_TARGETS x 40 bytes, 2.5 MiB,
+ * held until the process ends) instead of buffers allocated per call.  No
+ * counterpart in the reference: it is for callers that cannot allocate device
+ * memory (code under stream capture, serving loops, harnesses that forbid
+ * hipMalloc).  More targets than the maximum fail with HFMM_ERR_ARG.  Every
+ * call on one GPU shares that GPU's array: calls that overlap on one GPU (two
+ * threads, two contexts) race and must be serialised by the caller. */
+#define HFMM_DIRECT_NOALLOC_MAX_TARGETS 65536
+int hfmm_direct_noalloc(hfmm_ctx* ctx, int64_t ntargets, const int64_t* targets, double* pot, double* fx, double* fy,
+                        double* fz);
+
 /* The field grad(phi) at every particle (input order): the force without the
  * target charge, F = q grad(phi) (FChebSymKernel.hpp:432-486 accumulate q times
  * it).  Needed by the adjoint of the potential: for L(phi), dL/dx_j =

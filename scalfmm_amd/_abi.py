@@ -38,7 +38,7 @@ EXPORTS = (
     "hfmm_get_tree_info", "hfmm_get_sorted_keys", "hfmm_get_cell_keys", "hfmm_get_leaf_ranges",
     "hfmm_get_p2p_list", "hfmm_get_m2l_list", "hfmm_get_operator", "hfmm_get_expansions",
     "hfmm_set_profiling", "hfmm_get_kernel_times", "hfmm_reset_kernel_times", "hfmm_selftest_mfma_f64",
-    "hfmm_selftest_mfma_bf16", "hfmm_direct",
+    "hfmm_selftest_mfma_bf16", "hfmm_direct", "hfmm_direct_noalloc",
     "hfmm_get_field", "hfmm_get_field_device", "hfmm_set_charges", "hfmm_set_charges_device",
     "hfmm_version", "hfmm_set_particle_sets",
     "hfmm_batch_default_height", "hfmm_batch_create", "hfmm_batch_destroy", "hfmm_batch_last_error",
@@ -51,6 +51,7 @@ EXPORTS = (
     "hfmm_get_cell_blocks", "hfmm_set_cell_blocks",
 )
 MAX_RHS = 64
+DIRECT_NOALLOC_MAX_TARGETS = 65536  # HFMM_DIRECT_NOALLOC_MAX_TARGETS
 _NON_STATUS = ("hfmm_last_error", "hfmm_version", "hfmm_batch_last_error", "hfmm_batch_default_height")
 
 
@@ -113,6 +114,7 @@ def lib(path: str | None = None):
     L.hfmm_selftest_mfma_f64.argtypes = [vp, dp]
     L.hfmm_selftest_mfma_bf16.argtypes = [vp, C.POINTER(C.c_float)]
     L.hfmm_direct.argtypes = [vp, C.c_int64, C.POINTER(C.c_int64), dp, dp, dp, dp]
+    L.hfmm_direct_noalloc.argtypes = [vp, C.c_int64, C.POINTER(C.c_int64), dp, dp, dp, dp]
     L.hfmm_get_field.argtypes = [vp, dp, dp, dp]
     L.hfmm_set_charges.argtypes = [vp, dp]
     L.hfmm_set_charges_device.argtypes = [vp, vp]

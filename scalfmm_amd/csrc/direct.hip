@@ -4,6 +4,9 @@
 // (Tests/Kernels/DirectAlgorithm.cpp:203-215, Src/Kernels/P2P/FP2P.hpp:80-101).
 #include <hip/hip_runtime.h>
 
+#include <mutex>
+#include <string>
+
 #include "hfmm_internal.hpp"
 
 namespace hfmm {
@@ -66,7 +69,75 @@ __global__ __launch_bounds__(kDirectBlock) void k_direct(int64_t n, const double
     }
 }
 
+// Staging of hfmm_direct_noalloc: the target indices (kDirectStaticTargets
+// int64) followed by the results (4 kDirectStaticTargets doubles), 40 bytes per
+// target, 2.5 MiB in all.  Every GPU the library runs on holds its own copy
+// until the process ends; host code uses the per-device address that
+// init_direct_scratch reads, never the array's name.
+constexpr int64_t kDirectStaticTargets = HFMM_DIRECT_NOALLOC_MAX_TARGETS;
+__device__ __align__(256) unsigned char scratch[kDirectStaticTargets * 5 * sizeof(double)];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};  // device address of `scratch` per device
+
+// Launch of k_direct over c's particles for nt targets whose indices are at
+// device address dt, results (four arrays of nt) at dout.
+int launch_direct(Context* c, int64_t nt, const int64_t* dt, double* dout) {
+    const int64_t blocks = (nt + kDirectTargets - 1) / kDirectTargets;
+    hipLaunchKernelGGL(k_direct, dim3(static_cast<unsigned>(blocks)), dim3(kDirectBlock), 0, c->stream, c->n, c->x,
+                       c->y, c->z, c->q, dt, nt, dout);
+    return check_launch(c, "direct");
+}
+
+// Copies the four result arrays at dout (nt each) into the caller's host
+// arrays (NULL skips one) and waits for them.
+int copy_direct_results(Context* c, int64_t nt, const double* dout, double* pot, double* fx, double* fy, double* fz) {
+    double* outs[4] = {pot, fx, fy, fz};
+    for (int k = 0; k < 4; ++k)
+        if (outs[k] && hipMemcpyAsync(outs[k], dout + k * nt, sizeof(double) * nt, hipMemcpyDeviceToHost, c->stream) !=
+                           hipSuccess)
+            return fail(c, HFMM_ERR_DEVICE, "direct: result copy failed");
+    if (hipStreamSynchronize(c->stream) != hipSuccess) return fail(c, HFMM_ERR_DEVICE, "direct: synchronize failed");
+    return HFMM_OK;
+}
+
 }  // namespace
+
+int init_direct_scratch(Context* c) {
+    if (c->device < 0 || c->device >= kMaxDevices) return fail(c, HFMM_ERR_ARG, "direct staging: device index too large");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch_ptr[c->device]) return HFMM_OK;
+    if (hipSetDevice(c->device) != hipSuccess) return fail(c, HFMM_ERR_DEVICE, "direct staging: hipSetDevice failed");
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr)
+        return fail(c, HFMM_ERR_DEVICE, std::string("direct staging: hipGetSymbolAddress failed: ") + hipGetErrorString(e));
+    g_scratch_ptr[c->device] = ptr;
+    return HFMM_OK;
+}
+
+int direct_sum_noalloc(Context* c, int64_t nt, const int64_t* targets, double* pot, double* fx, double* fy,
+                       double* fz) {
+    if (nt > kDirectStaticTargets)
+        return fail(c, HFMM_ERR_ARG,
+                    "direct_noalloc: " + std::to_string(nt) + " targets exceed the static staging's " +
+                        std::to_string(kDirectStaticTargets));
+    void* base = nullptr;
+    {
+        std::lock_guard<std::mutex> lock(g_scratch_mu);
+        if (c->device >= 0 && c->device < kMaxDevices) base = g_scratch_ptr[c->device];
+    }
+    if (!base) return fail(c, HFMM_ERR_STATE, "direct_noalloc: the static staging of this device is not set up");
+    int64_t* dt = static_cast<int64_t*>(base);
+    double* dout = reinterpret_cast<double*>(dt + kDirectStaticTargets);
+    // every call rewrites the staging it reads: the target indices here, each
+    // result entry by the kernel (one per target), so nothing needs clearing
+    if (hipMemcpyAsync(dt, targets, sizeof(int64_t) * nt, hipMemcpyHostToDevice, c->stream) != hipSuccess)
+        return fail(c, HFMM_ERR_DEVICE, "direct_noalloc: target copy failed");
+    HFMM_TRY(launch_direct(c, nt, dt, dout));
+    return copy_direct_results(c, nt, dout, pot, fx, fy, fz);
+}
 
 int direct_sum(Context* c, int64_t nt, const int64_t* targets, double* pot, double* fx, double* fy, double* fz) {
     DevBuf dt, dout;
@@ -75,21 +146,8 @@ int direct_sum(Context* c, int64_t nt, const int64_t* targets, double* pot, doub
     if (st == HFMM_OK && hipMemcpyAsync(dt.ptr, targets, sizeof(int64_t) * nt, hipMemcpyHostToDevice, c->stream) !=
                              hipSuccess)
         st = fail(c, HFMM_ERR_DEVICE, "direct: target copy failed");
-    if (st == HFMM_OK) {
-        const int64_t blocks = (nt + kDirectTargets - 1) / kDirectTargets;
-        hipLaunchKernelGGL(k_direct, dim3(static_cast<unsigned>(blocks)), dim3(kDirectBlock), 0, c->stream, c->n, c->x,
-                           c->y, c->z, c->q, dt.as<int64_t>(), nt, dout.as<double>());
-        st = check_launch(c, "direct");
-    }
-    if (st == HFMM_OK) {
-        double* outs[4] = {pot, fx, fy, fz};
-        for (int k = 0; k < 4 && st == HFMM_OK; ++k)
-            if (outs[k] && hipMemcpyAsync(outs[k], dout.as<double>() + k * nt, sizeof(double) * nt,
-                                          hipMemcpyDeviceToHost, c->stream) != hipSuccess)
-                st = fail(c, HFMM_ERR_DEVICE, "direct: result copy failed");
-        if (st == HFMM_OK && hipStreamSynchronize(c->stream) != hipSuccess)
-            st = fail(c, HFMM_ERR_DEVICE, "direct: synchronize failed");
-    }
+    if (st == HFMM_OK) st = launch_direct(c, nt, dt.as<int64_t>(), dout.as<double>());
+    if (st == HFMM_OK) st = copy_direct_results(c, nt, dout.as<double>(), pot, fx, fy, fz);
     release(dt);
     release(dout);
     return st;

@@ -298,7 +298,8 @@ int create_context(const hfmm_params* params, hipStream_t shared_stream, hfmm_ct
     }
     std::string err;
     const int scheme = params->m2l_scheme;
-    int st = hfmm::build_cheb_operators(params->order, params->epsilon, c->ops, err);
+    int st = hfmm::init_direct_scratch(c);
+    if (st == HFMM_OK) st = hfmm::build_cheb_operators(params->order, params->epsilon, c->ops, err);
     if (st == HFMM_OK && scheme != HFMM_M2L_SYMMETRIC &&
         (scheme < HFMM_M2L_GLOBAL_SVD || scheme > HFMM_M2L_UNIFORM || params->precision != HFMM_PRECISION_FP64))
         st = HFMM_ERR_ARG;  // unknown scheme, or a non-symmetric one outside FP64
@@ -1068,6 +1069,19 @@ int hfmm_direct(hfmm_ctx* h, int64_t ntargets, const int64_t* targets, double* p
         if (targets[i] < 0 || targets[i] >= c->n) return hfmm::fail(c, HFMM_ERR_ARG, "direct: target out of range");
     HFMM_HIP(c, hipSetDevice(c->device));
     return hfmm::direct_sum(c, ntargets, targets, pot, fx, fy, fz);
+}
+
+int hfmm_direct_noalloc(hfmm_ctx* h, int64_t ntargets, const int64_t* targets, double* pot, double* fx, double* fy,
+                        double* fz) {
+    CTX_CHECK(h);
+    Context* c = &h->c;
+    if (c->n <= 0) return hfmm::fail(c, HFMM_ERR_STATE, "direct before set_particles");
+    if (c->nsets > 1) return hfmm::fail(c, HFMM_ERR_ARG, "direct: the context holds several particle sets");
+    if (ntargets <= 0 || !targets) return hfmm::fail(c, HFMM_ERR_ARG, "direct: bad targets");
+    for (int64_t i = 0; i < ntargets; ++i)
+        if (targets[i] < 0 || targets[i] >= c->n) return hfmm::fail(c, HFMM_ERR_ARG, "direct: target out of range");
+    HFMM_HIP(c, hipSetDevice(c->device));
+    return hfmm::direct_sum_noalloc(c, ntargets, targets, pot, fx, fy, fz);
 }
 
 int hfmm_selftest_mfma_bf16(hfmm_ctx* h, float* out) {

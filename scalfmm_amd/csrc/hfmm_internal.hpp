@@ -296,5 +296,10 @@ int selftest_mfma_bf16(Context* c, float* out);
 
 // FP64 direct sum for sampled targets (direct.hip)
 int direct_sum(Context* c, int64_t nt, const int64_t* targets, double* pot, double* fx, double* fy, double* fz);
+// hfmm_direct's kernel with its staging in the static per-device array;
+// init_direct_scratch reads that array's address on c's device (once per
+// device, at context creation).
+int direct_sum_noalloc(Context* c, int64_t nt, const int64_t* targets, double* pot, double* fx, double* fy, double* fz);
+int init_direct_scratch(Context* c);
 
 }  // namespace hfmm
