@@ -100,6 +100,112 @@ __global__ __launch_bounds__(64) void k_global_m2l(int64_t ncell, const int32_t*
 }
 
 // ---------------------------------------------------------------------------
+// The same M2L on FP64 MFMA (v_mfma_f64_16x16x4f64), for ranks up to
+// 16 kGlobalMfmaMaxTiles.  The operators are used as a padded row-major copy
+// Cr[pos][i][j] (KP = 16 NR rows and columns, zeros outside k x k).  A
+// workgroup of four waves owns one column tile of 16 targets of one parity at
+// one level (the parity-sorted cell order of the symmetric kernel, one parity
+// per XCD); wave w takes the tile's slots w, w + 4, ... and keeps all NR row
+// tiles of its partial L~ in registers; the four partial sums are added in
+// wave order through LDS (the same bits on every run) and added, scaled, to
+// L~.  Per slot and 16-wide K block, lane (i, g) = (lane % 16, lane / 16)
+// reads four consecutive entries 16 kb + 4 g + s (s < 4) of its target's M~
+// (B operand) and of row i of every row tile of C (A operand): a permutation
+// of the K order shared by both, so row-major loads of 32 bytes.  Register r
+// of lane (i, g) holds D[4 r + g][i] (the FP64 layout the symmetric kernel's
+// hfmm_selftest_mfma_f64 checks).
+constexpr int kGlobalMfmaMaxTiles = 16;
+
+typedef double d4 __attribute__((ext_vector_type(4)));
+
+__global__ __launch_bounds__(256) void k_global_pad_ops(const double* __restrict__ C, int k, int KP,
+                                                        double* __restrict__ Cr) {
+    const int64_t pos = blockIdx.x;
+    const double* src = C + pos * k * k;
+    double* dst = Cr + pos * KP * KP;
+    for (int e = threadIdx.x; e < KP * KP; e += blockDim.x) {
+        const int i = e / KP, j = e % KP;
+        dst[e] = (i < k && j < k) ? src[static_cast<int64_t>(j) * k + i] : 0.;
+    }
+}
+
+template <int NR>
+__global__ __launch_bounds__(256) void k_global_m2l_mfma(M2LLevels lv, const int32_t* __restrict__ slots,
+                                                        const int32_t* __restrict__ order,
+                                                        const int32_t* __restrict__ pcount,
+                                                        const int16_t* __restrict__ slotpos,
+                                                        const double* __restrict__ Cr, const double* __restrict__ Mt,
+                                                        int k, double* __restrict__ Lt) {
+    constexpr int KP = 16 * NR;
+    constexpr int W = 4;
+    __shared__ double red[16 * KP];  // [target column][row]
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int col = lane & 15, g = lane >> 4;
+    const int bid = blockIdx.x;
+    if (bid >= lv.tile_begin[lv.nlev]) return;
+    int li = 0;
+    while (bid >= lv.tile_begin[li + 1]) ++li;
+    const int lb = bid - lv.tile_begin[li];
+    const int e = lb % 8;  // parity: consecutive workgroups land on different XCDs
+    const int ptile = lb / 8;
+    const int32_t* pc = pcount + lv.pcount_off[li];
+    int first = 0;
+    for (int q = 0; q < e; ++q) first += pc[q];
+    const int cnt = min(16, pc[e] - ptile * 16);
+    if (cnt <= 0) return;  // the same for the whole workgroup
+    first += ptile * 16;
+    const int64_t coff = lv.cell_off[li];
+    const int32_t* lslots = slots + coff * kFarSlots;
+    const double* Ml = Mt + coff * k;
+    const int tgt = col < cnt ? order[coff + first + col] : -1;
+    d4 acc[NR];
+#pragma unroll
+    for (int rt = 0; rt < NR; ++rt) acc[rt] = d4{0., 0., 0., 0.};
+    for (int s = wave; s < kFarSlots; s += W) {
+        const int src = tgt >= 0 ? lslots[static_cast<int64_t>(tgt) * kFarSlots + s] : -1;
+        if (__ballot(src >= 0) == 0) continue;  // uniform over the wave
+        const double* C = Cr + static_cast<int64_t>(slotpos[e * kFarSlots + s]) * KP * KP + col * KP + 4 * g;
+        const double* m = Ml + static_cast<int64_t>(src >= 0 ? src : 0) * k;
+#pragma unroll 2
+        for (int kb = 0; kb < NR; ++kb) {
+            double b[4];
+#pragma unroll
+            for (int s2 = 0; s2 < 4; ++s2) {
+                const int j = 16 * kb + 4 * g + s2;
+                b[s2] = (src >= 0 && j < k) ? m[j] : 0.;
+            }
+#pragma unroll
+            for (int rt = 0; rt < NR; ++rt) {
+                const d4 a = *reinterpret_cast<const d4*>(C + 16 * rt * KP + 16 * kb);
+#pragma unroll
+                for (int s2 = 0; s2 < 4; ++s2) acc[rt] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[s2], b[s2], acc[rt], 0, 0, 0);
+            }
+        }
+    }
+    // partial sums in wave order: wave 0 writes, waves 1..3 add
+    for (int w = 0; w < W; ++w) {
+        if (wave == w) {
+#pragma unroll
+            for (int rt = 0; rt < NR; ++rt)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int idx = col * KP + 16 * rt + 4 * r + g;
+                    red[idx] = (w == 0 ? 0. : red[idx]) + acc[rt][r];
+                }
+        }
+        __syncthreads();
+    }
+    const double scale = lv.scale[li];
+    double* Ll = Lt + coff * k;
+    for (int idx = tid; idx < 16 * KP; idx += 64 * W) {
+        const int cc = idx / KP, i = idx % KP;
+        if (cc < cnt && i < k) Ll[static_cast<int64_t>(order[coff + first + cc]) * k + i] += scale * red[idx];
+    }
+}
+
+// ---------------------------------------------------------------------------
 // The uniform scheme (FUnifKernel): expansions in Fourier space.  A cell's p^3
 // node values, zero-padded to N^3 (N = 2p-1), are 3D-DFT'd; real input, so
 // only kx < p is kept: NF = p N^2 complex entries, index kx + p (ky + N kz).
@@ -365,10 +471,46 @@ int run_global_decompress(Context* c, int l) {
     return check_launch(c, "global_decompress");
 }
 
+// The padded row-major operators of the MFMA M2L, built once per operator set.
+static int prepare_global_mfma(Context* c, int KP) {
+    if (c->gcr_kp == KP) return HFMM_OK;
+    HFMM_TRY(ensure(c, c->d_gCr, sizeof(double) * 343 * static_cast<size_t>(KP) * KP));
+    hipLaunchKernelGGL(k_global_pad_ops, dim3(343), dim3(256), 0, c->stream, c->d_gC.as<double>(), c->glob.k, KP,
+                       c->d_gCr.as<double>());
+    HFMM_TRY(check_launch(c, "global_pad_ops"));
+    c->gcr_kp = KP;
+    return HFMM_OK;
+}
+
 int run_global_m2l(Context* c) {
     if (c->m2l_scheme == HFMM_M2L_UNIFORM) return run_unif_m2l(c);
     const int k = c->glob.k, R = (k + 63) / 64;
     if (R > kMaxRankRows) return fail(c, HFMM_ERR_ARG, "global M2L rank above 512");
+    const int NR = (k + 15) / 16;
+    if (NR <= kGlobalMfmaMaxTiles) {
+        HFMM_TRY(prepare_global_mfma(c, 16 * NR));
+        int tiles = 0;
+        const M2LLevels lv = make_levels(c, 16, tiles);
+        if (tiles == 0) return HFMM_OK;
+        prof_begin(c, "global_m2l");
+        const dim3 grid(static_cast<unsigned>(tiles));
+        const int32_t* slots = c->d_m2l.as<int32_t>();
+        const int32_t* ord = c->d_order.as<int32_t>();
+        const int32_t* pc = c->d_pcount.as<int32_t>();
+        const int16_t* sp = c->d_slotpos.as<int16_t>();
+        const double* Cr = c->d_gCr.as<double>();
+        const double* Mt = c->d_Mt.as<double>();
+        double* Lt = c->d_Lt.as<double>();
+        switch (NR) {
+#define CASE(N) case N: hipLaunchKernelGGL(k_global_m2l_mfma<N>, grid, dim3(256), 0, c->stream, lv, slots, ord, pc, sp, Cr, Mt, k, Lt); break;
+            CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7) CASE(8)
+            CASE(9) CASE(10) CASE(11) CASE(12) CASE(13) CASE(14) CASE(15) CASE(16)
+#undef CASE
+            default: break;
+        }
+        prof_end(c);
+        return check_launch(c, "global_m2l");
+    }
     const int L = c->height() - 1;
     // the M2L levels of the working range, from the top
     const int lo = c->work_upper > c->top() ? c->work_upper : c->top();

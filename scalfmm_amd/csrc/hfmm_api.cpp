@@ -212,7 +212,7 @@ static void free_all(Context* c) {
                       &c->d_scan_part, &c->d_qin, &c->d_flags, &c->d_scan, &c->d_sorted, &c->d_perm, &c->d_pleaf,
                       &c->d_count, &c->d_cellkey, &c->d_cellkey_tmp, &c->d_childstart, &c->d_leafstart,
                       &c->d_p2p, &c->d_p2pitems, &c->d_m2l, &c->d_order, &c->d_pcount, &c->d_dense, &c->d_M, &c->d_L, &c->d_Mb, &c->d_loc, &c->d_res_sorted,
-                      &c->d_res, &c->d_field, &c->d_setoff, &c->d_keep, &c->d_kept, &c->d_gBt, &c->d_gU, &c->d_gC, &c->d_Mt, &c->d_Lt, &c->d_uroots, &c->d_uK, &c->d_utw,
+                      &c->d_res, &c->d_field, &c->d_setoff, &c->d_keep, &c->d_kept, &c->d_gBt, &c->d_gU, &c->d_gC, &c->d_gCr, &c->d_Mt, &c->d_Lt, &c->d_uroots, &c->d_uK, &c->d_utw,
                       &c->d_p2pitems_m, &c->d_csoff, &c->d_csbase, &c->d_cpart};
     for (DevBuf* b : bufs) release(*b);
 }
@@ -931,6 +931,7 @@ int hfmm_set_m2l_scheme(hfmm_ctx* h, int32_t scheme) {
                 HFMM_HIP(c, hipMemcpy(c->d_gC.ptr, g.C.data(), sizeof(double) * g.C.size(), hipMemcpyHostToDevice));
             }
             c->glob = std::move(g);
+            c->gcr_kp = 0;  // the MFMA M2L's padded operators belong to the old set
             // the interpolation tables of the basis the scheme works in
             const hfmm::ChebOperators& tab = scheme == HFMM_M2L_UNIFORM ? u : c->ops;
             HFMM_HIP(c, hipMemcpy(c->d_child.ptr, tab.child.data(), sizeof(double) * tab.child.size(),

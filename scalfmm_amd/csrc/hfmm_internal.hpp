@@ -183,6 +183,8 @@ struct Context {
     DevBuf d_gBt;             // B^T as [nn][k] (the compression reads a row per node)
     DevBuf d_gU;              // U as [k][nn]
     DevBuf d_gC;              // C [343][k x k]
+    DevBuf d_gCr;             // C as [343][KP][KP] row-major, zero-padded (the MFMA M2L's operands)
+    int gcr_kp = 0;           // KP of d_gCr (0: not built for the current operators)
     DevBuf d_Mt, d_Lt;        // [total_cells][k]
     DevBuf d_keep;            // kept mode: snapshot of the expansions a pass overwrites
     DevBuf d_kept;            // kept mode, nrhs > 1: M and L of every charge vector
