@@ -360,6 +360,17 @@ __device__ __forceinline__ void mix_segment(MixState<P, COLS>& st, int i0, int i
                 for (int ct = 0; ct < COLS; ++ct)
                     *reinterpret_cast<f4*>(Cw + (16 * ct + col) * CR + 16 * rt + 4 * kq) = c1[rt][ct];
         }
+        // this wave's first U rows of GEMM2 (row tile 0, K block 0), requested
+        // before the barriers instead of after them
+        bf8 u0[3] = {};
+        {
+            const uint16_t* a = U + static_cast<int64_t>(16 * wave + col < LD ? 16 * wave + col : 0) * KP32 + 8 * kq;
+            if (8 * kq < KP16) {
+                u0[0] = ld8(a);
+                u0[1] = ld8(a + static_cast<int64_t>(LD) * KP32);
+                if constexpr (NPL == 3) u0[2] = ld8(a + 2 * static_cast<int64_t>(LD) * KP32);
+            }
+        }
         __syncthreads();
         // Rows 16 KT .. KP32 of C are zero: the segments run with growing KT,
         // so no earlier slot wrote them, and the LDS was cleared at the start.
@@ -416,7 +427,11 @@ __device__ __forceinline__ void mix_segment(MixState<P, COLS>& st, int i0, int i
                 // last block)
                 const bool live = 32 * kb + 8 * kq < KP16;
                 bf8 ah = {}, am = {}, al = {};
-                if (live) {
+                if (kb == 0 && t == 0) {  // requested before the barriers
+                    ah = u0[0];
+                    am = u0[1];
+                    al = u0[2];
+                } else if (live) {
                     ah = ld8(a);
                     am = ld8(a + static_cast<int64_t>(LD) * KP32);
                     if constexpr (NPL == 3) al = ld8(a + 2 * static_cast<int64_t>(LD) * KP32);
