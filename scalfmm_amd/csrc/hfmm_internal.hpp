@@ -260,6 +260,9 @@ inline int32_t* p2p_item_count(Context* c) { return reinterpret_cast<int32_t*>(c
 // its source-side partial sums (float4 entries).
 inline int32_t* p2p_mutual_item_count(Context* c) { return reinterpret_cast<int32_t*>(c->d_count.as<int64_t>() + 16); }
 inline int64_t* p2p_mutual_units(Context* c) { return c->d_count.as<int64_t>() + 17; }
+// The next work item of the mutual mixed P2P (claimed by the waves one at a
+// time; cleared before each launch).
+inline int32_t* p2p_mutual_next(Context* c) { return reinterpret_cast<int32_t*>(c->d_count.as<int64_t>() + 18); }
 // Builds the mutual P2P's work items and slot tables on a built tree and sizes
 // the partial buffer (mixed forms); leaves c->p2p_mutual false when the
 // partial sums would exceed a quarter of the free device memory.

@@ -311,7 +311,8 @@ __global__ __launch_bounds__(256, 4) void k_p2p_mixed_mutual(const int2* __restr
                                                          const int32_t* __restrict__ slots, const int4* __restrict__ loc,
                                                          const int32_t* __restrict__ csoff,
                                                          const int64_t* __restrict__ csbase, f4* __restrict__ cpart,
-                                                         int64_t n, double inv_u, double* __restrict__ res) {
+                                                         int32_t* __restrict__ next, int64_t n, double inv_u,
+                                                         double* __restrict__ res) {
     static_assert(64 * T == kP2PMutualItem, "work item = one wave x T targets");
     __shared__ int4 s_src[4][128];
     __shared__ f4 s_x[4][kPhases * 64];
@@ -320,7 +321,15 @@ __global__ __launch_bounds__(256, 4) void k_p2p_mixed_mutual(const int2* __restr
     int4* src = s_src[wave];
     f4* xch = s_x[wave];
     const int total = *nitems;
-    for (int it = blockIdx.x * 4 + wave; it < total; it += gridDim.x * 4) {
+    // work items claimed one at a time (an atomic counter: the items differ in
+    // cost by the sizes of their leaves' neighbours, so a fixed share per wave
+    // leaves the grid waiting for the slowest); every wave leaves once the
+    // counter passes the last item
+    for (;;) {
+        int claim = 0;
+        if (lane == 0) claim = atomicAdd(next, 1);
+        const int it = __builtin_amdgcn_readfirstlane(__shfl(claim, 0));
+        if (it >= total) break;
         const int2 item = items[it];
         const int leaf = __builtin_amdgcn_readfirstlane(item.x);
         const int first = __builtin_amdgcn_readfirstlane(item.y);
@@ -482,11 +491,12 @@ int run_p2p_mixed(Context* c) {
     }
     const int64_t want = (c->p2p_items_m_cap + 3) / 4;
     const unsigned grid = static_cast<unsigned>(std::max<int64_t>(1, std::min<int64_t>(want, resident[c->device])));
+    HFMM_TRY(zero_device(c, p2p_mutual_next(c), sizeof(int32_t)));  // a fill kernel: capturable
     prof_begin(c, "p2p_mixed_mutual");
     hipLaunchKernelGGL(k_p2p_mixed_mutual<kMutualT>, dim3(grid), dim3(256), 0, c->stream, c->d_p2pitems_m.as<int2>(),
                        p2p_mutual_item_count(c), c->d_leafstart.as<int32_t>(), c->d_p2p.as<int32_t>(), loc,
-                       c->d_csoff.as<int32_t>(), c->d_csbase.as<int64_t>(), c->d_cpart.as<f4>(), c->n, inv_u,
-                       c->d_res_sorted.as<double>());
+                       c->d_csoff.as<int32_t>(), c->d_csbase.as<int64_t>(), c->d_cpart.as<f4>(), p2p_mutual_next(c),
+                       c->n, inv_u, c->d_res_sorted.as<double>());
     prof_end(c);
     HFMM_TRY(check_launch(c, "p2p_mixed_mutual"));
     prof_begin(c, "p2p_combine");
