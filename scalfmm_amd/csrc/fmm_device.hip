@@ -90,59 +90,102 @@ __device__ __forceinline__ void lagrange_weights(double xi, const double* __rest
 }
 
 // ---------------------------------------------------------------------------
-// P2M: one wave per leaf, 64 particles per round staged in LDS.
+// P2M: one wave per window of 64 sorted particles.  Each lane forms the
+// interpolation weights of its particle in the LDS; the wave then walks the
+// leaves that start in the window, lanes over the coefficients.  A leaf that
+// ends inside the window sums its particles from the LDS in one round (most
+// leaves hold a few particles); the last leaf of a window may run past it and
+// is summed in rounds of 64 particles from its first one.  A leaf's rounds
+// always begin at its first particle, so its sums do not depend on where the
+// windows fall.
 template <int P, int BASIS>
-__global__ __launch_bounds__(64) void k_p2m(int nleaf, const uint64_t* __restrict__ lkeys,
-                                           const int32_t* __restrict__ lstart, const double* __restrict__ sp,
-                                           int64_t n, Box bx, const double* __restrict__ Troots,
+__device__ __forceinline__ void p2m_weights(const double* __restrict__ sp, int64_t n, int64_t e, Box bx,
+                                            uint64_t lkey, const double* __restrict__ sT,
+                                            const double* __restrict__ uroots, double* S, double* Q) {
+    double a[3], b[3];
+    leaf_bounds(bx, lkey, a, b);
+    const double pos[3] = {sp[e], sp[n + e], sp[2 * n + e]};
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        const double xi = (2. * pos[d] - b[d] - a[d]) / (b[d] - a[d]);
+        if constexpr (BASIS == 1)
+            lagrange_weights<P, false>(xi, uroots, S + d * P, nullptr);
+        else
+            interp_weights<P, false>(xi, sT, S + d * P, nullptr);
+    }
+    *Q = sp[3 * n + e];
+}
+
+template <int P, int BASIS>
+__global__ __launch_bounds__(64) void k_p2m(int64_t n, const int32_t* __restrict__ pleaf,
+                                           const uint64_t* __restrict__ lkeys, const int32_t* __restrict__ lstart,
+                                           const double* __restrict__ sp, Box bx, const double* __restrict__ Troots,
                                            const double* __restrict__ uroots, double* __restrict__ M, int LD) {
     constexpr int NN = P * P * P;
     constexpr int PER = (NN + 63) / 64;
     __shared__ double sS[64][3 * P];
     __shared__ double sQ[64];
     __shared__ double sT[P * P];
-    const int leaf = blockIdx.x;
+    __shared__ int sStart[65];
     const int lane = threadIdx.x;
+    const int64_t w0 = static_cast<int64_t>(blockIdx.x) * 64;
+    const int64_t wend = min(w0 + 64, n);
     for (int i = lane; i < P * P; i += 64) sT[i] = Troots[i];
-    double a[3], b[3];
-    leaf_bounds(bx, lkeys[leaf], a, b);
-    double acc[PER];
-#pragma unroll
-    for (int t = 0; t < PER; ++t) acc[t] = 0.;
-    const int s0 = lstart[leaf], s1 = lstart[leaf + 1];
+    // the leaves that start in the window: first .. last
+    int first = pleaf[w0];
+    if (lstart[first] < w0) ++first;
+    const int last = pleaf[wend - 1];
+    if (first > last) return;  // inside one long leaf: its first window sums it
     __syncthreads();
-    for (int base = s0; base < s1; base += 64) {
-        const int cnt = min(64, s1 - base);
-        if (lane < cnt) {
-            const int64_t e = base + lane;
-            const double pos[3] = {sp[e], sp[n + e], sp[2 * n + e]};
+    if (w0 + lane < wend) p2m_weights<P, BASIS>(sp, n, w0 + lane, bx, lkeys[pleaf[w0 + lane]], sT, uroots,
+                                                &sS[lane][0], &sQ[lane]);
+    // the bounds of the window's leaves (at most 64) through the LDS: the leaf
+    // loop then holds no vector load, whose wait would also cover the stores
+    // of the previous leaves
+    if (lane <= last - first) sStart[lane] = lstart[first + lane];
+    if (lane == 0) sStart[last - first + 1] = lstart[last + 1];
+    __syncthreads();
+    for (int leaf = first; leaf <= last; ++leaf) {
+        const int s0 = sStart[leaf - first], s1 = sStart[leaf - first + 1];
+        double acc[PER];
 #pragma unroll
-            for (int d = 0; d < 3; ++d) {
-                const double xi = (2. * pos[d] - b[d] - a[d]) / (b[d] - a[d]);
-                if constexpr (BASIS == 1)
-                    lagrange_weights<P, false>(xi, uroots, &sS[lane][d * P], nullptr);
-                else
-                    interp_weights<P, false>(xi, sT, &sS[lane][d * P], nullptr);
+        for (int t = 0; t < PER; ++t) acc[t] = 0.;
+        // one round over particles m0 .. m1-1 of the LDS rows; per coefficient the
+        // terms are added in particle order
+        auto round = [&](int m0, int m1) {
+            double sum[PER];
+#pragma unroll
+            for (int t = 0; t < PER; ++t) sum[t] = 0.;
+            for (int m = m0; m < m1; ++m) {
+                const double q = sQ[m];
+#pragma unroll
+                for (int t = 0; t < PER; ++t) {
+                    const int idx = lane + 64 * t;
+                    const int i = idx % P, j = (idx / P) % P, k = idx / (P * P);
+                    if (NN % 64 == 0 || idx < NN) sum[t] += q * sS[m][P + j] * sS[m][2 * P + k] * sS[m][i];
+                }
             }
-            sQ[lane] = sp[3 * n + e];
+#pragma unroll
+            for (int t = 0; t < PER; ++t) acc[t] += sum[t];
+        };
+        if (s1 <= wend) {
+            round(static_cast<int>(s0 - w0), static_cast<int>(s1 - w0));
+        } else {
+            // the window's last leaf runs past it: rounds of 64 from s0
+            for (int base = s0; base < s1; base += 64) {
+                const int cnt = min(64, s1 - base);
+                __syncthreads();
+                if (lane < cnt)
+                    p2m_weights<P, BASIS>(sp, n, base + lane, bx, lkeys[leaf], sT, uroots, &sS[lane][0], &sQ[lane]);
+                __syncthreads();
+                round(0, cnt);
+            }
         }
-        __syncthreads();
 #pragma unroll
         for (int t = 0; t < PER; ++t) {
             const int idx = lane + 64 * t;
-            if (idx < NN) {
-                const int i = idx % P, j = (idx / P) % P, k = idx / (P * P);
-                double s = 0.;
-                for (int m = 0; m < cnt; ++m) s += sQ[m] * sS[m][P + j] * sS[m][2 * P + k] * sS[m][i];
-                acc[t] += s;
-            }
+            if (idx < LD) M[static_cast<int64_t>(leaf) * LD + idx] = idx < NN ? acc[t] : 0.;  // zero padding
         }
-        __syncthreads();
-    }
-#pragma unroll
-    for (int t = 0; t < PER; ++t) {
-        const int idx = lane + 64 * t;
-        if (idx < LD) M[static_cast<int64_t>(leaf) * LD + idx] = idx < NN ? acc[t] : 0.;  // zero padding
     }
 }
 
@@ -251,25 +294,28 @@ __global__ __launch_bounds__(64) void k_l2l(const int32_t* __restrict__ cstart, 
 }
 
 // ---------------------------------------------------------------------------
-// L2P: one wave per leaf, one lane per particle; adds into the sorted results.
+// L2P: one lane per sorted particle; adds into the sorted results.
 //   phi = sum_k Sz_k sum_j Sy_j sum_i Sx_i L[k][j][i], gradients with dS on one
 // axis; Sx, dSx, Sy, dSy stay in registers, Sz and dSz are formed per k.
+// Leaves hold from one to thousands of particles (a Plummer tree of 10M
+// particles at height 8: 1.13M leaves, 641k of them with one particle), so a
+// wave per leaf would leave most lanes idle.  Each lane reads its leaf's local
+// expansion straight from memory: the lanes of one leaf read the same
+// addresses (one fetch for the wave), the lanes of small leaves their own.
 template <int P>
-__global__ __launch_bounds__(64) void k_l2p(int nleaf, const uint64_t* __restrict__ lkeys,
-                                           const int32_t* __restrict__ lstart, const double* __restrict__ sp,
-                                           int64_t n, Box bx, const double* __restrict__ Troots,
-                                           const double* __restrict__ L, int LD, double* __restrict__ res) {
-    constexpr int NN = P * P * P;
-    __shared__ double sL[NN];
+__global__ __launch_bounds__(256) void k_l2p(int64_t n, const int32_t* __restrict__ pleaf,
+                                            const uint64_t* __restrict__ lkeys, const double* __restrict__ sp, Box bx,
+                                            const double* __restrict__ Troots, const double* __restrict__ L, int LD,
+                                            double* __restrict__ res) {
     __shared__ double sT[P * P];
-    const int leaf = blockIdx.x, lane = threadIdx.x;
-    for (int i = lane; i < P * P; i += 64) sT[i] = Troots[i];
-    for (int i = lane; i < NN; i += 64) sL[i] = L[static_cast<int64_t>(leaf) * LD + i];
-    double a[3], b[3];
-    leaf_bounds(bx, lkeys[leaf], a, b);
+    for (int i = threadIdx.x; i < P * P; i += 256) sT[i] = Troots[i];
     __syncthreads();
-    const int s0 = lstart[leaf], s1 = lstart[leaf + 1];
-    for (int e = s0 + lane; e < s1; e += 64) {
+    const int64_t e = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x;
+    if (e < n) {
+        const int leaf = pleaf[e];
+        double a[3], b[3];
+        leaf_bounds(bx, lkeys[leaf], a, b);
+        const double* __restrict__ sL = L + static_cast<int64_t>(leaf) * LD;
         const double pos[3] = {sp[e], sp[n + e], sp[2 * n + e]};
         double Sx[P], dSx[P], Sy[P], dSy[P];
         const double xi0 = (2. * pos[0] - b[0] - a[0]) / (b[0] - a[0]);
@@ -867,9 +913,10 @@ int p2m_impl(Context* c) {
     auto kern = k_p2m<P, 0>;
     if constexpr (P <= kUniformMaxOrder)
         if (c->basis == 1) kern = k_p2m<P, 1>;
-    hipLaunchKernelGGL(kern, dim3(static_cast<unsigned>(nl)), dim3(64), 0, c->stream, static_cast<int>(nl),
-                       c->d_cellkey.as<uint64_t>() + c->lvl_off[L], c->d_leafstart.as<int32_t>(),
-                       c->d_sorted.as<double>(), c->n, make_box(c), c->d_Troots.as<double>(),
+    (void)nl;
+    hipLaunchKernelGGL(kern, dim3(static_cast<unsigned>((c->n + 63) / 64)), dim3(64), 0, c->stream, c->n,
+                       c->d_pleaf.as<int32_t>(), c->d_cellkey.as<uint64_t>() + c->lvl_off[L],
+                       c->d_leafstart.as<int32_t>(), c->d_sorted.as<double>(), make_box(c), c->d_Troots.as<double>(),
                        c->d_uroots.as<double>(), c->d_M.as<double>() + static_cast<size_t>(c->lvl_off[L]) * c->LD,
                        c->LD);
     prof_end(c);
@@ -917,11 +964,10 @@ int l2p_impl(Context* c) {
             return check_launch(c, "l2p");
         }
     }
-    hipLaunchKernelGGL(k_l2p<P>, dim3(static_cast<unsigned>(nl)), dim3(64), 0, c->stream, static_cast<int>(nl),
-                       c->d_cellkey.as<uint64_t>() + c->lvl_off[L], c->d_leafstart.as<int32_t>(),
-                       c->d_sorted.as<double>(), c->n, make_box(c), c->d_Troots.as<double>(),
-                       c->d_L.as<double>() + static_cast<size_t>(c->lvl_off[L]) * c->LD, c->LD,
-                       c->d_res_sorted.as<double>());
+    hipLaunchKernelGGL(k_l2p<P>, dim3(blocks256(c->n)), dim3(256), 0, c->stream, c->n, c->d_pleaf.as<int32_t>(),
+                       c->d_cellkey.as<uint64_t>() + c->lvl_off[L], c->d_sorted.as<double>(), make_box(c),
+                       c->d_Troots.as<double>(), c->d_L.as<double>() + static_cast<size_t>(c->lvl_off[L]) * c->LD,
+                       c->LD, c->d_res_sorted.as<double>());
     prof_end(c);
     return check_launch(c, "l2p");
 }
