@@ -459,6 +459,53 @@ __global__ __launch_bounds__(256) void k_p2p_f64(int64_t n, const int32_t* __res
     res[3 * n + i] = fz;
 }
 
+// k_p2p_f64 for R charge vectors at once: the separation, 1/r and 1/r^3 of a
+// pair once, then the terms of each vector, each summed as in k_p2p_f64 (the
+// same bits per vector).  qc: the chunk's sorted charges as [n][R]; res: the
+// results of its first vector, the next vectors 4n further each.
+template <int R>
+__global__ __launch_bounds__(256) void k_p2p_f64_rhs(int64_t n, const int32_t* __restrict__ pleaf,
+                                                    const int32_t* __restrict__ lstart,
+                                                    const int32_t* __restrict__ slots, const double* __restrict__ sp,
+                                                    const double* __restrict__ qc, double* __restrict__ res) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int li = pleaf[i];
+    if (lstart[li + 1] - lstart[li] > kP2PBig) return;  // k_p2p_f64_big, per vector
+    const double tx = sp[i], ty = sp[n + i], tz = sp[2 * n + i];
+    double pot[R], fx[R], fy[R], fz[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) pot[r] = fx[r] = fy[r] = fz[r] = 0.;
+    for (int s = -1; s < kNbrSlots; ++s) {
+        const int nb = s < 0 ? li : slots[static_cast<int64_t>(li) * kNbrSlots + s];
+        if (nb < 0) continue;
+        const int j0 = lstart[nb], j1 = lstart[nb + 1];
+        for (int j = j0; j < j1; ++j) {
+            if (j == i) continue;
+            const double dx = tx - sp[j], dy = ty - sp[n + j], dz = tz - sp[2 * n + j];
+            const double inv = 1.0 / sqrt(dx * dx + dy * dy + dz * dz);
+            const double inv3 = inv * inv * inv;
+            const double* q = qc + static_cast<int64_t>(j) * R;
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                const double qs = q[r];
+                pot[r] += inv * qs;
+                fx[r] += -inv3 * dx * qs;
+                fy[r] += -inv3 * dy * qs;
+                fz[r] += -inv3 * dz * qs;
+            }
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        double* o = res + static_cast<int64_t>(r) * 4 * n;
+        o[i] = pot[r];
+        o[n + i] = fx[r];
+        o[2 * n + i] = fy[r];
+        o[3 * n + i] = fz[r];
+    }
+}
+
 // P2P of the big leaves (more than kP2PBig particles).  Each wave takes work
 // items of kP2PItem consecutive targets of one leaf (two per lane) and walks
 // the 27 neighbour leaves in blocks of 8 sources.  Every source address is
@@ -959,7 +1006,7 @@ int l2p_impl(Context* c) {
                                c->d_leafstart.as<int32_t>(), c->d_sorted.as<double>(), c->n, make_box(c),
                                c->d_uroots.as<double>(),
                                c->d_L.as<double>() + static_cast<size_t>(c->lvl_off[L]) * c->LD, c->LD,
-                               c->d_res_sorted.as<double>());
+                               sorted_results(c));
             prof_end(c);
             return check_launch(c, "l2p");
         }
@@ -967,7 +1014,7 @@ int l2p_impl(Context* c) {
     hipLaunchKernelGGL(k_l2p<P>, dim3(blocks256(c->n)), dim3(256), 0, c->stream, c->n, c->d_pleaf.as<int32_t>(),
                        c->d_cellkey.as<uint64_t>() + c->lvl_off[L], c->d_sorted.as<double>(), make_box(c),
                        c->d_Troots.as<double>(), c->d_L.as<double>() + static_cast<size_t>(c->lvl_off[L]) * c->LD,
-                       c->LD, c->d_res_sorted.as<double>());
+                       c->LD, sorted_results(c));
     prof_end(c);
     return check_launch(c, "l2p");
 }
@@ -1032,16 +1079,18 @@ int run_m2l(Context* c) {
 }
 
 int run_p2p(Context* c) {
-    prof_begin(c, "p2p");
-    hipLaunchKernelGGL(k_p2p_f64, dim3(blocks256(c->n)), dim3(256), 0, c->stream, c->n, c->d_pleaf.as<int32_t>(),
-                       c->d_leafstart.as<int32_t>(), c->d_p2p.as<int32_t>(), c->d_sorted.as<double>(),
-                       c->d_res_sorted.as<double>());
-    prof_end(c);
-    HFMM_TRY(check_launch(c, "p2p"));
+    if (!c->p2p_small_done) {
+        prof_begin(c, "p2p");
+        hipLaunchKernelGGL(k_p2p_f64, dim3(blocks256(c->n)), dim3(256), 0, c->stream, c->n, c->d_pleaf.as<int32_t>(),
+                           c->d_leafstart.as<int32_t>(), c->d_p2p.as<int32_t>(), c->d_sorted.as<double>(),
+                           sorted_results(c));
+        prof_end(c);
+        HFMM_TRY(check_launch(c, "p2p"));
+    }
     prof_begin(c, "p2p_big");
     hipLaunchKernelGGL(k_p2p_f64_big, dim3((p2p_tiled_grid(c) + 3) / 4), dim3(256), 0, c->stream,
                        c->d_p2pitems.as<int2>(), p2p_item_count(c), c->d_leafstart.as<int32_t>(),
-                       c->d_p2p.as<int32_t>(), c->d_sorted.as<double>(), c->n, c->d_res_sorted.as<double>());
+                       c->d_p2p.as<int32_t>(), c->d_sorted.as<double>(), c->n, sorted_results(c));
     prof_end(c);
     return check_launch(c, "p2p_big");
 }
@@ -1068,12 +1117,37 @@ int combine_device(Context* c, double* dst, const double* src, int64_t count, bo
     return check_launch(c, "combine");
 }
 
-int zero_results(Context* c) { return zero_device(c, c->d_res_sorted.ptr, sizeof(double) * 4 * c->n); }
+int run_p2p_small_all(Context* c) {
+    HFMM_TRY(gather_all_charges(c, false));
+    RhsChunk ch[HFMM_MAX_RHS];
+    const int nch = rhs_chunks(c->nrhs, ch);
+    const int64_t npad = (c->n + 7) & ~int64_t(7);
+    prof_begin(c, "p2p_rhs");
+    for (int k = 0; k < nch; ++k) {
+        const double* qc = c->d_qall.as<double>() + npad * ch[k].r0;
+        double* res = c->d_res_sorted.as<double>() + static_cast<size_t>(ch[k].r0) * 4 * c->n;
+        const dim3 grid(blocks256(c->n)), block(256);
+        const int32_t* pl = c->d_pleaf.as<int32_t>();
+        const int32_t* ls = c->d_leafstart.as<int32_t>();
+        const int32_t* sl = c->d_p2p.as<int32_t>();
+        const double* sp = c->d_sorted.as<double>();
+        switch (ch[k].R) {
+            case 8: hipLaunchKernelGGL(k_p2p_f64_rhs<8>, grid, block, 0, c->stream, c->n, pl, ls, sl, sp, qc, res); break;
+            case 4: hipLaunchKernelGGL(k_p2p_f64_rhs<4>, grid, block, 0, c->stream, c->n, pl, ls, sl, sp, qc, res); break;
+            case 2: hipLaunchKernelGGL(k_p2p_f64_rhs<2>, grid, block, 0, c->stream, c->n, pl, ls, sl, sp, qc, res); break;
+            default: hipLaunchKernelGGL(k_p2p_f64_rhs<1>, grid, block, 0, c->stream, c->n, pl, ls, sl, sp, qc, res); break;
+        }
+    }
+    prof_end(c);
+    return check_launch(c, "p2p_rhs");
+}
+
+int zero_results(Context* c) { return zero_device(c, sorted_results(c), sizeof(double) * 4 * c->n); }
 
 int scatter_results(Context* c, int rhs) {
     prof_begin(c, "scatter");
     hipLaunchKernelGGL(k_scatter, dim3(blocks256(c->n)), dim3(256), 0, c->stream, c->n, c->d_perm.as<int32_t>(),
-                       c->d_sorted.as<double>(), c->d_res_sorted.as<double>(),
+                       c->d_sorted.as<double>(), sorted_results(c),
                        c->d_res.as<double>() + static_cast<size_t>(rhs) * 4 * c->n,
                        c->d_field.as<double>() + static_cast<size_t>(rhs) * 3 * c->n);
     prof_end(c);

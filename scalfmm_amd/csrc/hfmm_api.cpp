@@ -211,7 +211,7 @@ static void free_all(Context* c) {
                       &c->d_slotpos, &c->d_slotord, &c->d_slotseg, &c->d_posoff, &c->d_opm, &c->d_opm2, &c->d_in, &c->d_keys, &c->d_keys2, &c->d_vals, &c->d_vals2, &c->d_hist,
                       &c->d_scan_part, &c->d_qin, &c->d_flags, &c->d_scan, &c->d_sorted, &c->d_perm, &c->d_pleaf,
                       &c->d_count, &c->d_cellkey, &c->d_cellkey_tmp, &c->d_childstart, &c->d_leafstart,
-                      &c->d_p2p, &c->d_p2pitems, &c->d_m2l, &c->d_order, &c->d_pcount, &c->d_dense, &c->d_M, &c->d_L, &c->d_Mb, &c->d_loc, &c->d_res_sorted,
+                      &c->d_p2p, &c->d_p2pitems, &c->d_m2l, &c->d_order, &c->d_pcount, &c->d_dense, &c->d_M, &c->d_L, &c->d_Mb, &c->d_loc, &c->d_res_sorted, &c->d_qall,
                       &c->d_res, &c->d_field, &c->d_setoff, &c->d_keep, &c->d_kept, &c->d_gBt, &c->d_gU, &c->d_gC, &c->d_gCr, &c->d_Mt, &c->d_Lt, &c->d_uroots, &c->d_uK, &c->d_utw,
                       &c->d_p2pitems_m, &c->d_csoff, &c->d_csbase, &c->d_cpart};
     for (DevBuf* b : bufs) release(*b);
@@ -562,7 +562,11 @@ int hfmm_evaluate_levels(hfmm_ctx* h, uint32_t ops, int32_t upper, int32_t lower
     // one extra zero multipole after the last cell: the M2L kernel reads it for missing sources
     HFMM_TRY(hfmm::ensure(c, c->d_M, exp_bytes + sizeof(double) * c->LD));
     HFMM_TRY(hfmm::ensure(c, c->d_L, exp_bytes));
-    HFMM_TRY(hfmm::ensure(c, c->d_res_sorted, sizeof(double) * 4 * c->n));
+    // several charge vectors: the near field of the leaves of at most kP2PBig
+    // particles runs once for all of them (the pair geometry once per pair),
+    // and the sorted results hold one block per vector
+    const bool near_all = c->nrhs > 1 && (ops & HFMM_P2P);
+    HFMM_TRY(hfmm::ensure(c, c->d_res_sorted, sizeof(double) * 4 * c->n * (near_all ? c->nrhs : 1)));
     HFMM_TRY(hfmm::ensure(c, c->d_res, sizeof(double) * 4 * c->n * c->nrhs));
     HFMM_TRY(hfmm::ensure(c, c->d_field, sizeof(double) * 3 * c->n * c->nrhs));
     HFMM_TRY(hfmm::zero_device(c, c->d_M.as<double>() + static_cast<size_t>(c->total_cells) * c->LD,
@@ -584,9 +588,20 @@ int hfmm_evaluate_levels(hfmm_ctx* h, uint32_t ops, int32_t upper, int32_t lower
         }
     }
     if (c->keep_cells) HFMM_TRY(hfmm::prepare_kept_cells(c));
+    struct PerVector {  // the single-vector state again on every way out
+        Context* c;
+        ~PerVector() {
+            c->res_rhs = 0;
+            c->p2p_small_done = false;
+        }
+    } per_vector{c};
+    if (near_all)
+        HFMM_TRY(hfmm::is_mixed(c) ? hfmm::run_p2p_mixed_small_all(c) : hfmm::run_p2p_small_all(c));
     // one pass sequence per charge vector (the tree and the lists are shared);
     // with one vector the sorted charges are already in place
     for (int r = 0; r < c->nrhs; ++r) {
+        c->res_rhs = near_all ? r : 0;
+        c->p2p_small_done = near_all;
         double* kept = kept_many ? c->d_kept.as<double>() + 2 * exp_words * r : nullptr;
         if (c->nrhs > 1) HFMM_TRY(hfmm::update_charges_device(c, r));
         if (kept) {

@@ -159,7 +159,11 @@ struct Context {
     DevBuf d_M, d_L;      // [total_cells][LD]
     DevBuf d_Mb;          // mixed precision: multipoles in FP32 [total_cells + 1][LD] (the M2L splits them)
     DevBuf d_loc;         // mixed precision: leaf-local fixed-point int4 (x, y, z, q bits) of the sorted particles
-    DevBuf d_res_sorted;  // pot, grad(phi) x, y, z (sorted order; the charge is applied by the scatter)
+    DevBuf d_res_sorted;  // pot, grad(phi) x, y, z (sorted order; the charge is applied by the scatter),
+                          // one such block per charge vector when the near field runs for all vectors at once
+    DevBuf d_qall;        // the sorted charges of every vector, in chunks of R vectors laid out [n][R]
+    int res_rhs = 0;      // the vector whose block of d_res_sorted the passes read and write
+    bool p2p_small_done = false;  // the leaves of at most kP2PBig particles already hold their near field
     DevBuf d_res;         // pot, fx, fy, fz (input order)
     DevBuf d_field;       // grad(phi) x, y, z (input order): the force without the target charge
     // Cell state between evaluations: fresh (every evaluation starts from zero
@@ -276,6 +280,33 @@ int run_l2l(Context* c, int level);  // parents at `level`
 int run_l2p(Context* c);
 int run_p2p(Context* c);
 int zero_results(Context* c);
+// The sorted results of vector c->res_rhs.
+inline double* sorted_results(Context* c) {
+    return c->d_res_sorted.as<double>() + static_cast<size_t>(c->res_rhs) * 4 * static_cast<size_t>(c->n);
+}
+// Charge vectors of the all-vector near field in chunks of R = 8, 4, 2, 1
+// (largest first): the chunk of vectors r0 .. r0 + R - 1 holds its sorted
+// charges at d_qall + n r0 (elements of the form's type) as [n][R].
+struct RhsChunk {
+    int r0, R;
+};
+inline int rhs_chunks(int nrhs, RhsChunk* out) {
+    int k = 0;
+    for (int r0 = 0; r0 < nrhs;) {
+        const int left = nrhs - r0;
+        const int R = left >= 8 ? 8 : left >= 4 ? 4 : left >= 2 ? 2 : 1;
+        out[k++] = RhsChunk{r0, R};
+        r0 += R;
+    }
+    return k;
+}
+// The sorted charges of every vector into d_qall (FP32 for the mixed forms).
+int gather_all_charges(Context* c, bool f32);
+// Near field of the leaves of at most kP2PBig particles for every charge vector
+// at once (the pair geometry once per pair), into the per-vector blocks of
+// d_res_sorted; the big leaves then run per vector.
+int run_p2p_small_all(Context* c);
+int run_p2p_mixed_small_all(Context* c);
 // results of charge vector `rhs` into its slot of the input-order outputs
 int scatter_results(Context* c, int rhs = 0);
 // clears device memory with a fill kernel (capturable, unlike hipMemsetAsync)

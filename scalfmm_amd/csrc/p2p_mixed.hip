@@ -133,6 +133,63 @@ __global__ __launch_bounds__(256) void k_p2p_mixed(int64_t n, const int32_t* __r
     write_p2p(n, i, sum, comp, inv_u, res);
 }
 
+// k_p2p_mixed for R charge vectors at once: the separation, 1/r and 1/r^2 of
+// a pair once, then the terms of each vector, each summed and compensated as
+// in k_p2p_mixed (the same bits per vector).  qc: the chunk's sorted charges
+// (FP32) as [n][R]; res: the results of its first vector, the next vectors 4n
+// further each.
+template <int R>
+__global__ __launch_bounds__(256) void k_p2p_mixed_rhs(int64_t n, const int32_t* __restrict__ pleaf,
+                                                      const int32_t* __restrict__ lstart,
+                                                      const int32_t* __restrict__ slots, const int4* __restrict__ loc,
+                                                      const float* __restrict__ qc, double inv_u,
+                                                      double* __restrict__ res) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int li = pleaf[i];
+    if (lstart[li + 1] - lstart[li] > kP2PBig) return;  // the big-leaf kernels, per vector
+    const int4 t = loc[i];
+    float sum[R][4] = {}, comp[R][4] = {};
+    for (int s = -1; s < kNbrSlots; ++s) {
+        const int nb = s < 0 ? li : slots[static_cast<int64_t>(li) * kNbrSlots + s];
+        if (nb < 0) continue;
+        int c[3];
+        slot_offset(s, c);
+        const int tc[3] = {t.x - c[0], t.y - c[1], t.z - c[2]};
+        const int j0 = lstart[nb], j1 = lstart[nb + 1];
+        for (int jb = j0; jb < j1; jb += kKahanBlock) {
+            float part[R][4] = {};
+            const int je = min(j1, jb + kKahanBlock);
+            for (int j = jb; j < je; ++j) {
+                const int4 sj = loc[j];
+                const float dx = static_cast<float>(tc[0] - sj.x);
+                const float dy = static_cast<float>(tc[1] - sj.y);
+                const float dz = static_cast<float>(tc[2] - sj.z);
+                const float r2 = dx * dx + dy * dy + dz * dz;
+                float inv = __builtin_amdgcn_rsqf(r2);
+                inv = r2 == 0.f ? 0.f : inv;  // the target itself (or an unresolvable pair)
+                const float inv2 = inv * inv;
+                const float* q = qc + static_cast<int64_t>(j) * R;
+#pragma unroll
+                for (int r = 0; r < R; ++r) {
+                    const float qi = q[r] * inv;
+                    const float qi3 = qi * inv2;
+                    part[r][0] += qi;
+                    part[r][1] -= qi3 * dx;
+                    part[r][2] -= qi3 * dy;
+                    part[r][3] -= qi3 * dz;
+                }
+            }
+#pragma unroll
+            for (int r = 0; r < R; ++r)
+#pragma unroll
+                for (int k = 0; k < 4; ++k) kahan_add(sum[r][k], comp[r][k], part[r][k]);
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < R; ++r) write_p2p(n, i, sum[r], comp[r], inv_u, res + static_cast<int64_t>(r) * 4 * n);
+}
+
 // Big-leaf sources against TPL targets per lane, in blocks of 16; every
 // source address is wave-uniform, so the sources arrive through scalar loads
 // into SGPRs and feed the pair arithmetic as scalar operands (no LDS, no
@@ -465,17 +522,20 @@ int run_p2p_mixed(Context* c) {
     prof_end(c);
     HFMM_TRY(check_launch(c, "local_coords"));
     const double inv_u = kUnitsPerLeaf / bx.leafw;
-    prof_begin(c, "p2p_mixed");
-    hipLaunchKernelGGL(k_p2p_mixed, dim3(blocks_of(c->n)), dim3(256), 0, c->stream, c->n, c->d_pleaf.as<int32_t>(),
-                       c->d_leafstart.as<int32_t>(), c->d_p2p.as<int32_t>(), loc, inv_u, c->d_res_sorted.as<double>());
-    prof_end(c);
-    HFMM_TRY(check_launch(c, "p2p_mixed"));
+    if (!c->p2p_small_done) {
+        prof_begin(c, "p2p_mixed");
+        hipLaunchKernelGGL(k_p2p_mixed, dim3(blocks_of(c->n)), dim3(256), 0, c->stream, c->n,
+                           c->d_pleaf.as<int32_t>(), c->d_leafstart.as<int32_t>(), c->d_p2p.as<int32_t>(), loc, inv_u,
+                           sorted_results(c));
+        prof_end(c);
+        HFMM_TRY(check_launch(c, "p2p_mixed"));
+    }
     if (!(c->p2p_mutual && c->p2p_mutual_on)) {
         const unsigned grid = (p2p_tiled_grid(c) + 3) / 4;
         prof_begin(c, "p2p_mixed_big");
         hipLaunchKernelGGL(k_p2p_mixed_big<kP2PItem / 64>, dim3(grid), dim3(256), 0, c->stream,
                            c->d_p2pitems.as<int2>(), p2p_item_count(c), c->d_leafstart.as<int32_t>(),
-                           c->d_p2p.as<int32_t>(), loc, c->n, inv_u, c->d_res_sorted.as<double>());
+                           c->d_p2p.as<int32_t>(), loc, c->n, inv_u, sorted_results(c));
         prof_end(c);
         return check_launch(c, "p2p_mixed_big");
     }
@@ -496,15 +556,50 @@ int run_p2p_mixed(Context* c) {
     hipLaunchKernelGGL(k_p2p_mixed_mutual<kMutualT>, dim3(grid), dim3(256), 0, c->stream, c->d_p2pitems_m.as<int2>(),
                        p2p_mutual_item_count(c), c->d_leafstart.as<int32_t>(), c->d_p2p.as<int32_t>(), loc,
                        c->d_csoff.as<int32_t>(), c->d_csbase.as<int64_t>(), c->d_cpart.as<f4>(), p2p_mutual_next(c),
-                       c->n, inv_u, c->d_res_sorted.as<double>());
+                       c->n, inv_u, sorted_results(c));
     prof_end(c);
     HFMM_TRY(check_launch(c, "p2p_mixed_mutual"));
     prof_begin(c, "p2p_combine");
     hipLaunchKernelGGL(k_p2p_combine, dim3(blocks_of(c->n)), dim3(256), 0, c->stream, c->n, c->d_pleaf.as<int32_t>(),
                        c->d_leafstart.as<int32_t>(), c->d_csbase.as<int64_t>(), c->d_cpart.as<f4>(), inv_u,
-                       c->d_res_sorted.as<double>());
+                       sorted_results(c));
     prof_end(c);
     return check_launch(c, "p2p_combine");
+}
+
+int run_p2p_mixed_small_all(Context* c) {
+    const int L = c->height() - 1;
+    HFMM_TRY(gather_all_charges(c, true));
+    HFMM_TRY(ensure(c, c->d_loc, sizeof(int4) * static_cast<size_t>(c->n + kKahanBlock)));
+    const Box bx = make_box(c);
+    int4* loc = c->d_loc.as<int4>();
+    HFMM_TRY(zero_device(c, loc + c->n, sizeof(int4) * kKahanBlock));
+    prof_begin(c, "local_coords");
+    hipLaunchKernelGGL(k_local_coords, dim3(blocks_of(c->n)), dim3(256), 0, c->stream, c->n, c->d_pleaf.as<int32_t>(),
+                       c->d_cellkey.as<uint64_t>() + c->lvl_off[L], bx, c->d_sorted.as<double>(), loc);
+    prof_end(c);
+    HFMM_TRY(check_launch(c, "local_coords"));
+    const double inv_u = kUnitsPerLeaf / bx.leafw;
+    RhsChunk ch[HFMM_MAX_RHS];
+    const int nch = rhs_chunks(c->nrhs, ch);
+    const int64_t npad = (c->n + 7) & ~int64_t(7);
+    const dim3 grid(blocks_of(c->n)), block(256);
+    const int32_t* pl = c->d_pleaf.as<int32_t>();
+    const int32_t* ls = c->d_leafstart.as<int32_t>();
+    const int32_t* sl = c->d_p2p.as<int32_t>();
+    prof_begin(c, "p2p_mixed_rhs");
+    for (int k = 0; k < nch; ++k) {
+        const float* qc = c->d_qall.as<float>() + npad * ch[k].r0;
+        double* res = c->d_res_sorted.as<double>() + static_cast<size_t>(ch[k].r0) * 4 * c->n;
+        switch (ch[k].R) {
+            case 8: hipLaunchKernelGGL(k_p2p_mixed_rhs<8>, grid, block, 0, c->stream, c->n, pl, ls, sl, loc, qc, inv_u, res); break;
+            case 4: hipLaunchKernelGGL(k_p2p_mixed_rhs<4>, grid, block, 0, c->stream, c->n, pl, ls, sl, loc, qc, inv_u, res); break;
+            case 2: hipLaunchKernelGGL(k_p2p_mixed_rhs<2>, grid, block, 0, c->stream, c->n, pl, ls, sl, loc, qc, inv_u, res); break;
+            default: hipLaunchKernelGGL(k_p2p_mixed_rhs<1>, grid, block, 0, c->stream, c->n, pl, ls, sl, loc, qc, inv_u, res); break;
+        }
+    }
+    prof_end(c);
+    return check_launch(c, "p2p_mixed_rhs");
 }
 
 }  // namespace hfmm

@@ -322,6 +322,18 @@ __global__ __launch_bounds__(kBlock) void k_gather_charges(int64_t n, const int3
     sq[i] = q[perm[i]];
 }
 
+// The sorted charges of vectors r0 .. r0 + R - 1 as [n][R] (T = float: the
+// mixed forms' P2P charges, rounded as k_local_coords rounds them).
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_gather_charge_chunk(int64_t n, const int32_t* __restrict__ perm,
+                                                               const double* __restrict__ q, int r0, int R,
+                                                               T* __restrict__ out) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int64_t o = perm[i];
+    for (int r = 0; r < R; ++r) out[i * R + r] = static_cast<T>(q[static_cast<int64_t>(r0 + r) * n + o]);
+}
+
 // flags[i] = 1 where (keys[i] >> shift) starts a new group.
 __global__ __launch_bounds__(kBlock) void k_head_flags(int64_t n, const uint64_t* __restrict__ keys, int shift,
                                                       int32_t* __restrict__ flags) {
@@ -536,6 +548,26 @@ int update_charges_device(Context* c, int rhs) {
     hipLaunchKernelGGL(k_gather_charges, dim3(blocks_for(c->n)), dim3(kBlock), 0, c->stream, c->n,
                        c->d_perm.as<int32_t>(), c->q + static_cast<size_t>(rhs) * c->n,
                        c->d_sorted.as<double>() + 3 * c->n);
+    prof_end(c);
+    return check_launch(c, "gather_charges");
+}
+
+int gather_all_charges(Context* c, bool f32) {
+    const int64_t npad = (c->n + 7) & ~int64_t(7);  // chunk starts 64-byte aligned
+    HFMM_TRY(ensure(c, c->d_qall, (f32 ? sizeof(float) : sizeof(double)) * static_cast<size_t>(npad) * c->nrhs));
+    RhsChunk ch[HFMM_MAX_RHS];
+    const int nch = rhs_chunks(c->nrhs, ch);
+    prof_begin(c, "gather_charges");
+    for (int k = 0; k < nch; ++k) {
+        if (f32)
+            hipLaunchKernelGGL(k_gather_charge_chunk<float>, dim3(blocks_for(c->n)), dim3(kBlock), 0, c->stream, c->n,
+                               c->d_perm.as<int32_t>(), c->q, ch[k].r0, ch[k].R,
+                               c->d_qall.as<float>() + npad * ch[k].r0);
+        else
+            hipLaunchKernelGGL(k_gather_charge_chunk<double>, dim3(blocks_for(c->n)), dim3(kBlock), 0, c->stream,
+                               c->n, c->d_perm.as<int32_t>(), c->q, ch[k].r0, ch[k].R,
+                               c->d_qall.as<double>() + npad * ch[k].r0);
+    }
     prof_end(c);
     return check_launch(c, "gather_charges");
 }
