@@ -339,22 +339,21 @@ __device__ __forceinline__ void mutual_pair(const int (&t)[3], float qt, int4 s,
     const float dz = static_cast<float>(t[2] - s.z);
     const float r2 = dx * dx + dy * dy + dz * dz;
     const float inv = __builtin_amdgcn_rsqf(r2);
-    const float inv2 = inv * inv;
-    const float a = qs * inv;
-    const float a3 = a * inv2;
-    tp[0] += a;
+    // 1/r^3 once for both sides: the potentials fold into FMAs, 4 products per pair
+    const float inv3 = inv * (inv * inv);
+    const float a3 = qs * inv3;
+    tp[0] = fmaf(qs, inv, tp[0]);
     tp[1] -= a3 * dx;
     tp[2] -= a3 * dy;
     tp[3] -= a3 * dz;
-    const float b = qt * inv;
-    const float b3 = b * inv2;
+    const float b3 = qt * inv3;
     if (FIRST) {
-        sp[0] = b;
+        sp[0] = qt * inv;
         sp[1] = b3 * dx;
         sp[2] = b3 * dy;
         sp[3] = b3 * dz;
     } else {
-        sp[0] += b;
+        sp[0] = fmaf(qt, inv, sp[0]);
         sp[1] += b3 * dx;
         sp[2] += b3 * dy;
         sp[3] += b3 * dz;
