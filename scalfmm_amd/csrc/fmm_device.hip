@@ -190,17 +190,30 @@ __global__ __launch_bounds__(64) void k_p2m(int64_t n, const int32_t* __restrict
 }
 
 // ---------------------------------------------------------------------------
-// M2M: one wave per parent; child[m] -> parent[n] += Sx[nx][mx] Sy[ny][my] Sz[nz][mz] child[m].
+// M2M and L2L: a workgroup of 4 waves shares the 8 children's interpolation
+// matrices in the LDS, one parent per wave.  The waves of a workgroup walk
+// different numbers of children, so the separable passes through each wave's
+// own LDS buffers are ordered by wave-local fences: the wave's LDS accesses run
+// in order, and the fence keeps the compiler from moving other lanes' reads
+// above the writes they depend on.
+__device__ __forceinline__ void wave_lds_order() { __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront"); }
+
+// M2M: child[m] -> parent[n] += Sx[nx][mx] Sy[ny][my] Sz[nz][mz] child[m].
 template <int P>
-__global__ __launch_bounds__(64) void k_m2m(const int32_t* __restrict__ cstart, const uint64_t* __restrict__ ckeys,
-                                           const double* __restrict__ interp, const double* __restrict__ Mc,
-                                           double* __restrict__ Mp, int LD) {
+__global__ __launch_bounds__(256) void k_m2m(int np, const int32_t* __restrict__ cstart,
+                                            const uint64_t* __restrict__ ckeys, const double* __restrict__ interp,
+                                            const double* __restrict__ Mc, double* __restrict__ Mp, int LD) {
     constexpr int NN = P * P * P;
     constexpr int PER = (NN + 63) / 64;
     __shared__ double sI[8 * 3 * P * P];
-    __shared__ double b0[NN], b1[NN];
-    const int parent = blockIdx.x, lane = threadIdx.x;
-    for (int i = lane; i < 8 * 3 * P * P; i += 64) sI[i] = interp[i];
+    __shared__ double sb[4][2][NN];
+    const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    for (int i = threadIdx.x; i < 8 * 3 * P * P; i += 256) sI[i] = interp[i];
+    __syncthreads();
+    const int parent = blockIdx.x * 4 + wave;
+    if (parent >= np) return;  // after the workgroup's only barrier
+    double* b0 = sb[wave][0];
+    double* b1 = sb[wave][1];
     double acc[PER];
 #pragma unroll
     for (int t = 0; t < PER; ++t) acc[t] = 0.;
@@ -211,7 +224,7 @@ __global__ __launch_bounds__(64) void k_m2m(const int32_t* __restrict__ cstart, 
         const double* Sy = sI + (cidx * 3 + 1) * P * P;
         const double* Sz = sI + (cidx * 3 + 2) * P * P;
         for (int i = lane; i < NN; i += 64) b0[i] = Mc[static_cast<int64_t>(ch) * LD + i];
-        __syncthreads();
+        wave_lds_order();
         for (int idx = lane; idx < NN; idx += 64) {  // x
             const int nx = idx % P, rest = idx / P;
             double s = 0.;
@@ -219,7 +232,7 @@ __global__ __launch_bounds__(64) void k_m2m(const int32_t* __restrict__ cstart, 
             for (int m = 0; m < P; ++m) s += Sx[nx * P + m] * b0[rest * P + m];
             b1[idx] = s;
         }
-        __syncthreads();
+        wave_lds_order();
         for (int idx = lane; idx < NN; idx += 64) {  // y
             const int i = idx % P, ny = (idx / P) % P, k = idx / (P * P);
             double s = 0.;
@@ -227,7 +240,7 @@ __global__ __launch_bounds__(64) void k_m2m(const int32_t* __restrict__ cstart, 
             for (int m = 0; m < P; ++m) s += Sy[ny * P + m] * b1[k * P * P + m * P + i];
             b0[idx] = s;
         }
-        __syncthreads();
+        wave_lds_order();
 #pragma unroll
         for (int t = 0; t < PER; ++t) {  // z
             const int idx = lane + 64 * t;
@@ -239,7 +252,7 @@ __global__ __launch_bounds__(64) void k_m2m(const int32_t* __restrict__ cstart, 
                 acc[t] += s;
             }
         }
-        __syncthreads();
+        wave_lds_order();
     }
 #pragma unroll
     for (int t = 0; t < PER; ++t) {
@@ -248,18 +261,24 @@ __global__ __launch_bounds__(64) void k_m2m(const int32_t* __restrict__ cstart, 
     }
 }
 
-// L2L: one wave per parent; child[m] += sum_n Sx[nx][mx] Sy[ny][my] Sz[nz][mz] parent[n].
+// L2L: child[m] += sum_n Sx[nx][mx] Sy[ny][my] Sz[nz][mz] parent[n].
 template <int P>
-__global__ __launch_bounds__(64) void k_l2l(const int32_t* __restrict__ cstart, const uint64_t* __restrict__ ckeys,
-                                           const double* __restrict__ interp, const double* __restrict__ Lp,
-                                           double* __restrict__ Lc, int LD) {
+__global__ __launch_bounds__(256) void k_l2l(int np, const int32_t* __restrict__ cstart,
+                                            const uint64_t* __restrict__ ckeys, const double* __restrict__ interp,
+                                            const double* __restrict__ Lp, double* __restrict__ Lc, int LD) {
     constexpr int NN = P * P * P;
     __shared__ double sI[8 * 3 * P * P];
-    __shared__ double par[NN], b1[NN], b2[NN];
-    const int parent = blockIdx.x, lane = threadIdx.x;
-    for (int i = lane; i < 8 * 3 * P * P; i += 64) sI[i] = interp[i];
-    for (int i = lane; i < NN; i += 64) par[i] = Lp[static_cast<int64_t>(parent) * LD + i];
+    __shared__ double sb[4][3][NN];
+    const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    for (int i = threadIdx.x; i < 8 * 3 * P * P; i += 256) sI[i] = interp[i];
     __syncthreads();
+    const int parent = blockIdx.x * 4 + wave;
+    if (parent >= np) return;  // after the workgroup's only barrier
+    double* par = sb[wave][0];
+    double* b1 = sb[wave][1];
+    double* b2 = sb[wave][2];
+    for (int i = lane; i < NN; i += 64) par[i] = Lp[static_cast<int64_t>(parent) * LD + i];
+    wave_lds_order();
     const int c0 = cstart[parent], c1 = cstart[parent + 1];
     for (int ch = c0; ch < c1; ++ch) {
         const int cidx = static_cast<int>(ckeys[ch] & 7u);
@@ -273,7 +292,7 @@ __global__ __launch_bounds__(64) void k_l2l(const int32_t* __restrict__ cstart, 
             for (int nn = 0; nn < P; ++nn) s += Sx[nn * P + mx] * par[rest * P + nn];
             b1[idx] = s;
         }
-        __syncthreads();
+        wave_lds_order();
         for (int idx = lane; idx < NN; idx += 64) {  // y
             const int i = idx % P, my = (idx / P) % P, k = idx / (P * P);
             double s = 0.;
@@ -281,7 +300,7 @@ __global__ __launch_bounds__(64) void k_l2l(const int32_t* __restrict__ cstart, 
             for (int nn = 0; nn < P; ++nn) s += Sy[nn * P + my] * b1[k * P * P + nn * P + i];
             b2[idx] = s;
         }
-        __syncthreads();
+        wave_lds_order();
         for (int idx = lane; idx < NN; idx += 64) {  // z
             const int i = idx % P, j = (idx / P) % P, mz = idx / (P * P);
             double s = 0.;
@@ -289,7 +308,7 @@ __global__ __launch_bounds__(64) void k_l2l(const int32_t* __restrict__ cstart, 
             for (int nn = 0; nn < P; ++nn) s += Sz[nn * P + mz] * b2[nn * P * P + j * P + i];
             Lc[static_cast<int64_t>(ch) * LD + idx] += s;
         }
-        __syncthreads();
+        wave_lds_order();
     }
 }
 
@@ -974,8 +993,10 @@ template <int P>
 int m2m_impl(Context* c, int l) {
     const int64_t np = c->ncell[l];
     prof_begin(c, "m2m");
-    hipLaunchKernelGGL(k_m2m<P>, dim3(static_cast<unsigned>(np)), dim3(64), 0, c->stream,
-                       c->d_childstart.as<int32_t>() + c->cs_off[l], c->d_cellkey.as<uint64_t>() + c->lvl_off[l + 1],
+    if (np == 0) return HFMM_OK;
+    hipLaunchKernelGGL(k_m2m<P>, dim3(static_cast<unsigned>((np + 3) / 4)), dim3(256), 0, c->stream,
+                       static_cast<int>(np), c->d_childstart.as<int32_t>() + c->cs_off[l],
+                       c->d_cellkey.as<uint64_t>() + c->lvl_off[l + 1],
                        c->d_child.as<double>(), c->d_M.as<double>() + static_cast<size_t>(c->lvl_off[l + 1]) * c->LD,
                        c->d_M.as<double>() + static_cast<size_t>(c->lvl_off[l]) * c->LD, c->LD);
     prof_end(c);
@@ -986,8 +1007,10 @@ template <int P>
 int l2l_impl(Context* c, int l) {
     const int64_t np = c->ncell[l];
     prof_begin(c, "l2l");
-    hipLaunchKernelGGL(k_l2l<P>, dim3(static_cast<unsigned>(np)), dim3(64), 0, c->stream,
-                       c->d_childstart.as<int32_t>() + c->cs_off[l], c->d_cellkey.as<uint64_t>() + c->lvl_off[l + 1],
+    if (np == 0) return HFMM_OK;
+    hipLaunchKernelGGL(k_l2l<P>, dim3(static_cast<unsigned>((np + 3) / 4)), dim3(256), 0, c->stream,
+                       static_cast<int>(np), c->d_childstart.as<int32_t>() + c->cs_off[l],
+                       c->d_cellkey.as<uint64_t>() + c->lvl_off[l + 1],
                        c->d_child.as<double>(), c->d_L.as<double>() + static_cast<size_t>(c->lvl_off[l]) * c->LD,
                        c->d_L.as<double>() + static_cast<size_t>(c->lvl_off[l + 1]) * c->LD, c->LD);
     prof_end(c);
