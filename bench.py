@@ -43,9 +43,9 @@ PEAK = {  # MI355X published peaks
 WORKLOADS = {
     # name: (distribution, n, height, order, cpu sample (n, height)); the CPU
     # sample keeps the particles per leaf: n / 8^d particles d levels lower
-    "plummer10M-h8-p8": ("plummer", 10_000_000, 8, 8, (156_250, 6)),
-    "plummer10M-h8-p5": ("plummer", 10_000_000, 8, 5, (156_250, 6)),
-    "uniform1M-h7-p7": ("uniform", 1_000_000, 7, 7, (125_000, 6)),
+    "plummer10M-h8-p8": ("plummer", 10_000_000, 8, 8, (1_250_000, 7)),
+    "plummer10M-h8-p5": ("plummer", 10_000_000, 8, 5, (1_250_000, 7)),
+    "uniform1M-h7-p7": ("uniform", 1_000_000, 7, 7, (1_000_000, 7)),
     # 4,096 sets of log-uniform sizes in [1e3, 6.4e4], heights 3..5 by size,
     # 8 charge vectors; the CPU sample is the first 512 sets with one vector
     "batch4096-p5-r8": ("batch", 4096, None, 5, (512, None)),
