@@ -93,3 +93,34 @@ def test_near_field_only(mode):
         ref = a.results()
         for k in range(4):
             assert np.array_equal(got[k][r], ref[k]), (r, k)
+
+
+@pytest.mark.parametrize("scheme", ["global", "uniform"])
+def test_other_m2l_schemes(scheme):
+    """FChebKernel (one global SVD) and FUnifKernel (Lagrange basis) with three vectors: the near field
+    of all vectors at once beside the scheme's far field, every vector bit-equal to a context holding it
+    alone (order 5, FP64)."""
+    from scalfmm_amd import Engine, _abi
+    m2l = {"global": _abi.M2L_GLOBAL_SVD, "uniform": _abi.M2L_UNIFORM}[scheme]
+    n, nrhs = 20000, 3
+    x, y, z, q = _particles(n, nrhs, 13)
+
+    def engine():
+        e = Engine(H, 1.0, (0.5, 0.5, 0.5), P, 0.0, 0, _precision("fp64"))
+        e.set_m2l_scheme(m2l)
+        return e
+
+    e = engine()
+    e.set_particle_sets(np.array([0, n]), x, y, z, q)
+    e.build_tree()
+    e.evaluate()
+    got = e.results()
+    _say(f"{scheme}: {n} Plummer particles, {nrhs} vectors")
+    for r in range(nrhs):
+        a = engine()
+        a.set_particles(x, y, z, q[r])
+        a.build_tree()
+        a.evaluate()
+        ref = a.results()
+        for k in range(4):
+            assert np.array_equal(got[k][r], ref[k]), (r, k)
