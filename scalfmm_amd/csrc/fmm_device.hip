@@ -992,8 +992,8 @@ int p2m_impl(Context* c) {
 template <int P>
 int m2m_impl(Context* c, int l) {
     const int64_t np = c->ncell[l];
-    prof_begin(c, "m2m");
     if (np == 0) return HFMM_OK;
+    prof_begin(c, "m2m");
     hipLaunchKernelGGL(k_m2m<P>, dim3(static_cast<unsigned>((np + 3) / 4)), dim3(256), 0, c->stream,
                        static_cast<int>(np), c->d_childstart.as<int32_t>() + c->cs_off[l],
                        c->d_cellkey.as<uint64_t>() + c->lvl_off[l + 1],
@@ -1006,8 +1006,8 @@ int m2m_impl(Context* c, int l) {
 template <int P>
 int l2l_impl(Context* c, int l) {
     const int64_t np = c->ncell[l];
-    prof_begin(c, "l2l");
     if (np == 0) return HFMM_OK;
+    prof_begin(c, "l2l");
     hipLaunchKernelGGL(k_l2l<P>, dim3(static_cast<unsigned>((np + 3) / 4)), dim3(256), 0, c->stream,
                        static_cast<int>(np), c->d_childstart.as<int32_t>() + c->cs_off[l],
                        c->d_cellkey.as<uint64_t>() + c->lvl_off[l + 1],
