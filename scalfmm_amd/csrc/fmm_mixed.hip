@@ -696,8 +696,12 @@ __device__ __forceinline__ void wave_segment(WaveMixState<P, CT>& st, int i0, in
     }
 }
 
+// Two waves per SIMD at most 256 registers each: the three-part split with two
+// tiles per wave needs 314 and would run one wave per SIMD; held to 256 it
+// spills 36 bytes per lane and runs its order-5 trees 1.36x faster (the second
+// wave hides the gathers of the first).
 template <int P, int CT, int NPL>
-__global__ __launch_bounds__(256) void k_m2l_mixed_wave(M2LLevels lv, const int32_t* __restrict__ slots,
+__global__ __launch_bounds__(256, 2) void k_m2l_mixed_wave(M2LLevels lv, const int32_t* __restrict__ slots,
                                                        const int32_t* __restrict__ order,
                                                        const int32_t* __restrict__ pcount,
                                                        const int16_t* __restrict__ slotord,
