@@ -396,24 +396,23 @@ __global__ __launch_bounds__(256) void k_l2p(int64_t n, const int32_t* __restric
 }
 
 // L2P on the uniform (Lagrange) basis (FUnifInterpolator.hpp:476-640,
-// applyL2P + applyL2PGradient): one wave per leaf, one lane per particle.
+// applyL2P + applyL2PGradient): one lane per sorted particle, the leaf's local
+// expansion read directly as in k_l2p.
 template <int P>
-__global__ __launch_bounds__(64) void k_l2p_lagrange(int nleaf, const uint64_t* __restrict__ lkeys,
-                                                    const int32_t* __restrict__ lstart,
-                                                    const double* __restrict__ sp, int64_t n, Box bx,
-                                                    const double* __restrict__ uroots, const double* __restrict__ L,
-                                                    int LD, double* __restrict__ res) {
-    constexpr int NN = P * P * P;
-    __shared__ double sL[NN];
+__global__ __launch_bounds__(256) void k_l2p_lagrange(int64_t n, const int32_t* __restrict__ pleaf,
+                                                     const uint64_t* __restrict__ lkeys,
+                                                     const double* __restrict__ sp, Box bx,
+                                                     const double* __restrict__ uroots, const double* __restrict__ L,
+                                                     int LD, double* __restrict__ res) {
     __shared__ double sR[P];
-    const int leaf = blockIdx.x, lane = threadIdx.x;
-    if (lane < P) sR[lane] = uroots[lane];
-    for (int i = lane; i < NN; i += 64) sL[i] = L[static_cast<int64_t>(leaf) * LD + i];
-    double a[3], b[3];
-    leaf_bounds(bx, lkeys[leaf], a, b);
+    if (threadIdx.x < P) sR[threadIdx.x] = uroots[threadIdx.x];
     __syncthreads();
-    const int s0 = lstart[leaf], s1 = lstart[leaf + 1];
-    for (int e = s0 + lane; e < s1; e += 64) {
+    const int64_t e = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x;
+    if (e < n) {
+        const int leaf = pleaf[e];
+        double a[3], b[3];
+        leaf_bounds(bx, lkeys[leaf], a, b);
+        const double* __restrict__ sL = L + static_cast<int64_t>(leaf) * LD;
         const double pos[3] = {sp[e], sp[n + e], sp[2 * n + e]};
         double S[3][P], dS[3][P];
 #pragma unroll
@@ -1020,14 +1019,12 @@ int l2l_impl(Context* c, int l) {
 template <int P>
 int l2p_impl(Context* c) {
     const int L = c->height() - 1;
-    const int64_t nl = c->ncell[L];
     prof_begin(c, "l2p");
     if constexpr (P <= kUniformMaxOrder) {
         if (c->basis == 1) {
-            hipLaunchKernelGGL(k_l2p_lagrange<P>, dim3(static_cast<unsigned>(nl)), dim3(64), 0, c->stream,
-                               static_cast<int>(nl), c->d_cellkey.as<uint64_t>() + c->lvl_off[L],
-                               c->d_leafstart.as<int32_t>(), c->d_sorted.as<double>(), c->n, make_box(c),
-                               c->d_uroots.as<double>(),
+            hipLaunchKernelGGL(k_l2p_lagrange<P>, dim3(blocks256(c->n)), dim3(256), 0, c->stream, c->n,
+                               c->d_pleaf.as<int32_t>(), c->d_cellkey.as<uint64_t>() + c->lvl_off[L],
+                               c->d_sorted.as<double>(), make_box(c), c->d_uroots.as<double>(),
                                c->d_L.as<double>() + static_cast<size_t>(c->lvl_off[L]) * c->LD, c->LD,
                                sorted_results(c));
             prof_end(c);
