@@ -325,19 +325,36 @@ __global__ __launch_bounds__(64) void k_unif_inverse(int64_t ncell, const double
     }
 }
 
+// The M2L levels of one uniform-scheme launch: level i owns workgroups
+// first[i] .. first[i + 1] - 1, one per target cell.
+struct UnifLevels {
+    int nlev;
+    int64_t first[kMaxLevels + 1];
+    const int32_t* slots[kMaxLevels];
+    const uint64_t* keys[kMaxLevels];
+    const double2* Mh[kMaxLevels];
+    double2* Lh[kMaxLevels];
+    double scale[kMaxLevels];
+};
+
 // L^[cell] += scale K^_pos . M^[src] entrywise (FUnifM2LHandler::applyFC)
-// over the cell's 189 slots in slot order.
+// over the cell's 189 slots in slot order, one wave per target cell; every
+// M2L level in one launch (per-level launches left the top levels' 64 and
+// 512 cells to a nearly idle GPU).
 template <int P>
-__global__ __launch_bounds__(64) void k_unif_m2l(int64_t ncell, const int32_t* __restrict__ slots,
-                                                 const uint64_t* __restrict__ keys,
-                                                 const int16_t* __restrict__ slotpos,
-                                                 const double2* __restrict__ K, const double2* __restrict__ Mh,
-                                                 double scale, double2* __restrict__ Lh) {
+__global__ __launch_bounds__(64) void k_unif_m2l(UnifLevels lv, const int16_t* __restrict__ slotpos,
+                                                 const double2* __restrict__ K) {
     constexpr int N = 2 * P - 1, NF = P * N * N, R = (NF + 63) / 64;
-    const int64_t cell = blockIdx.x;
-    if (cell >= ncell) return;
+    const int64_t bid = blockIdx.x;
+    if (bid >= lv.first[lv.nlev]) return;
+    int li = 0;
+    while (bid >= lv.first[li + 1]) ++li;
+    const int64_t cell = bid - lv.first[li];
+    const int32_t* __restrict__ slots = lv.slots[li];
+    const double2* __restrict__ Mh = lv.Mh[li];
+    double2* __restrict__ Lh = lv.Lh[li];
     const int lane = threadIdx.x;
-    const int par = static_cast<int>(keys[cell] & 7u);
+    const int par = static_cast<int>(lv.keys[li][cell] & 7u);
     double2 acc[R];
 #pragma unroll
     for (int r = 0; r < R; ++r) acc[r] = make_double2(0., 0.);
@@ -356,6 +373,7 @@ __global__ __launch_bounds__(64) void k_unif_m2l(int64_t ncell, const int32_t* _
             }
         }
     }
+    const double scale = lv.scale[li];
 #pragma unroll
     for (int r = 0; r < R; ++r) {
         const int f = lane + 64 * r;
@@ -411,26 +429,31 @@ int run_unif_m2l(Context* c) {
     const int lo = c->work_upper > c->top() ? c->work_upper : c->top();
     const int hi = c->work_lower - 1 < L ? c->work_lower - 1 : L;
     const int k = c->glob.k;
-    prof_begin(c, "unif_m2l");
+    UnifLevels lv{};
+    lv.first[0] = 0;
     for (int l = lo; l <= hi; ++l) {
         const int64_t nc = c->ncell[l];
         if (nc == 0) continue;
-        const double scale = 2. / (c->P.box_width / static_cast<double>(int64_t(1) << (l - c->set_levels)));
-        const int32_t* slots = c->d_m2l.as<int32_t>() + static_cast<size_t>(c->m2l_off[l]) * kFarSlots;
-        const uint64_t* keys = c->d_cellkey.as<uint64_t>() + c->lvl_off[l];
-        const double2* Mh = reinterpret_cast<const double2*>(c->d_Mt.as<double>() + c->lvl_off[l] * k);
-        double2* Lh = reinterpret_cast<double2*>(c->d_Lt.as<double>() + c->lvl_off[l] * k);
-        const int16_t* sp = c->d_slotpos.as<int16_t>();
-        const double2* K = reinterpret_cast<const double2*>(c->d_uK.as<double>());
-        const dim3 grid(static_cast<unsigned>(nc));
-        switch (c->ops.p) {
-#define CASE(PP) case PP: hipLaunchKernelGGL(k_unif_m2l<PP>, grid, dim3(64), 0, c->stream, nc, slots, keys, sp, K, Mh, scale, Lh); break;
-            HFMM_UNIF_ORDERS(CASE)
-#undef CASE
-            default: return fail(c, HFMM_ERR_ARG, "uniform M2L: unsupported order");
-        }
-        HFMM_TRY(check_launch(c, "unif_m2l"));
+        const int i = lv.nlev++;
+        lv.scale[i] = 2. / (c->P.box_width / static_cast<double>(int64_t(1) << (l - c->set_levels)));
+        lv.slots[i] = c->d_m2l.as<int32_t>() + static_cast<size_t>(c->m2l_off[l]) * kFarSlots;
+        lv.keys[i] = c->d_cellkey.as<uint64_t>() + c->lvl_off[l];
+        lv.Mh[i] = reinterpret_cast<const double2*>(c->d_Mt.as<double>() + c->lvl_off[l] * k);
+        lv.Lh[i] = reinterpret_cast<double2*>(c->d_Lt.as<double>() + c->lvl_off[l] * k);
+        lv.first[i + 1] = lv.first[i] + nc;
     }
+    if (lv.nlev == 0) return HFMM_OK;
+    const int16_t* sp = c->d_slotpos.as<int16_t>();
+    const double2* K = reinterpret_cast<const double2*>(c->d_uK.as<double>());
+    const dim3 grid(static_cast<unsigned>(lv.first[lv.nlev]));
+    prof_begin(c, "unif_m2l");
+    switch (c->ops.p) {
+#define CASE(PP) case PP: hipLaunchKernelGGL(k_unif_m2l<PP>, grid, dim3(64), 0, c->stream, lv, sp, K); break;
+        HFMM_UNIF_ORDERS(CASE)
+#undef CASE
+        default: prof_end(c); return fail(c, HFMM_ERR_ARG, "uniform M2L: unsupported order");
+    }
+    HFMM_TRY(check_launch(c, "unif_m2l"));
     prof_end(c);
     return HFMM_OK;
 }
