@@ -66,6 +66,22 @@ __device__ __forceinline__ f4 mmaN(bf8 ah, bf8 am, bf8 al, bf8 bh, bf8 bm, bf8 b
     return mfma_bf16(ah, bh, c);
 }
 
+// mmaN for COLS accumulators sharing the A operand, the partial products
+// interleaved over the accumulators (each accumulator receives its products
+// in mmaN's order), so that no MFMA waits for the result of the one before.
+template <int NPL, int COLS>
+__device__ __forceinline__ void mmaN_cols(const bf8 (&a)[3], const bf8 (&b)[3][COLS], f4 (&c)[COLS]) {
+    // (A part, B part) of each partial product, smallest first as in mma6
+    constexpr int kA6[6] = {0, 2, 1, 0, 1, 0}, kB6[6] = {2, 0, 1, 1, 0, 0};
+    constexpr int kA3[3] = {0, 1, 0}, kB3[3] = {1, 0, 0};
+#pragma unroll
+    for (int q = 0; q < (NPL == 3 ? 6 : 3); ++q) {
+        const int ia = NPL == 3 ? kA6[q] : kA3[q], ib = NPL == 3 ? kB6[q] : kB3[q];
+#pragma unroll
+        for (int ct = 0; ct < COLS; ++ct) c[ct] = mfma_bf16(a[ia], b[ib][ct], c[ct]);
+    }
+}
+
 __device__ __forceinline__ uint16_t bf16_bits(__bf16 v) { return __builtin_bit_cast(uint16_t, v); }
 
 // x = hi + mid + lo, each part rounded to nearest bfloat16.
@@ -203,6 +219,7 @@ struct MixState {
     static constexpr int NN = P * P * P;
     static constexpr int NT = (NN + 15) / 16;
     static constexpr int NTW = (NT + kM2LWaves - 1) / kM2LWaves;
+    static constexpr int LDP = (NN + 15) / 16 * 16;  // the padded node count (the context's LD)
     f4 acc[NTW][COLS];
     f4 comp[COLS];  // Kahan compensation of the first row tile (the lowest modes)
 };
@@ -211,25 +228,42 @@ struct MixState {
 // per lane (two 16-byte loads, 16-byte aligned: LD and 8 kq are multiples of
 // 8), split into the three bfloat16 parts in registers.
 template <int COLS>
-__device__ __forceinline__ void load_mfrag(bf8 (&b)[3][COLS], const float* const (&Mrow)[COLS], int kb) {
+__device__ __forceinline__ void fetch_mfrag(f4 (&x)[COLS][2], const float* const (&Mrow)[COLS], int kb) {
 #pragma unroll
     for (int ct = 0; ct < COLS; ++ct) {
         const float* p = Mrow[ct] + 32 * kb;
-        const f4 x0 = *reinterpret_cast<const f4*>(p);
-        const f4 x1 = *reinterpret_cast<const f4*>(p + 4);
-#pragma unroll
-        for (int e = 0; e < 4; ++e) {
-            __bf16 h, m, l;
-            split3(x0[e], h, m, l);
-            b[0][ct][e] = h;
-            b[1][ct][e] = m;
-            b[2][ct][e] = l;
-            split3(x1[e], h, m, l);
-            b[0][ct][4 + e] = h;
-            b[1][ct][4 + e] = m;
-            b[2][ct][4 + e] = l;
-        }
+        x[ct][0] = *reinterpret_cast<const f4*>(p);
+        x[ct][1] = *reinterpret_cast<const f4*>(p + 4);
     }
+}
+
+template <int COLS>
+__device__ __forceinline__ void split_mcol(bf8 (&b)[3][COLS], const f4 (&x)[COLS][2], int ct) {
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+        __bf16 h, m, l;
+        split3(x[ct][0][e], h, m, l);
+        b[0][ct][e] = h;
+        b[1][ct][e] = m;
+        b[2][ct][e] = l;
+        split3(x[ct][1][e], h, m, l);
+        b[0][ct][4 + e] = h;
+        b[1][ct][4 + e] = m;
+        b[2][ct][4 + e] = l;
+    }
+}
+
+template <int COLS>
+__device__ __forceinline__ void split_mfrag(bf8 (&b)[3][COLS], const f4 (&x)[COLS][2]) {
+#pragma unroll
+    for (int ct = 0; ct < COLS; ++ct) split_mcol<COLS>(b, x, ct);
+}
+
+template <int COLS>
+__device__ __forceinline__ void load_mfrag(bf8 (&b)[3][COLS], const float* const (&Mrow)[COLS], int kb) {
+    f4 x[COLS][2];
+    fetch_mfrag<COLS>(x, Mrow, kb);
+    split_mfrag<COLS>(b, x);
 }
 
 // VT rows of rank tile rt, node block kb, three planes.
@@ -243,10 +277,13 @@ __device__ __forceinline__ void load_vfrag(bf8 (&a)[3], const uint16_t* VT, int 
 
 // GEMM1 on node block kb with the VT rows rolling one rank tile ahead: va[0]
 // holds rank tile 0 of this block on entry and rank tile 0 of block kb + 1 on
-// exit (when `more`).
+// exit (when `more`).  With `more`, the FP32 multipole fragments of block
+// kb + 1 (xn) are split into bn alongside the MFMAs, column tile ct in the
+// region of rank tile ct % KT, so the conversions overlap the matrix work.
 template <int COLS, int KT, int KP16, int NPL>
 __device__ __forceinline__ void g1_step(f4 (&c1)[KT][COLS], const bf8 (&b)[3][COLS], bf8 (&va)[2][3],
-                                        const uint16_t* VT, int LD, int col, int kq, int kb, bool more) {
+                                        const uint16_t* VT, int LD, int col, int kq, int kb, bool more,
+                                        bf8 (&bn)[3][COLS], const f4 (&xn)[COLS][2]) {
 #pragma unroll
     for (int rt = 0; rt < KT; ++rt) {
         const int cur = rt & 1;
@@ -254,9 +291,12 @@ __device__ __forceinline__ void g1_step(f4 (&c1)[KT][COLS], const bf8 (&b)[3][CO
             load_vfrag<KP16>(va[cur ^ 1], VT, LD, rt + 1, col, kq, kb);
         else if (more)
             load_vfrag<KP16>(va[cur ^ 1], VT, LD, 0, col, kq, kb + 1);
+        mmaN_cols<NPL, COLS>(va[cur], b, c1[rt]);
+        if (more) {
 #pragma unroll
-        for (int ct = 0; ct < COLS; ++ct)
-            c1[rt][ct] = mmaN<NPL>(va[cur][0], va[cur][1], va[cur][2], b[0][ct], b[1][ct], b[2][ct], c1[rt][ct]);
+            for (int ct = 0; ct < COLS; ++ct)
+                if (ct % KT == rt) split_mcol<COLS>(bn, xn, ct);
+        }
         __builtin_amdgcn_sched_barrier(0);
     }
     if (KT & 1) {  // the next block's rank tile 0 landed in va[1]
@@ -266,16 +306,36 @@ __device__ __forceinline__ void g1_step(f4 (&c1)[KT][COLS], const bf8 (&b)[3][CO
     }
 }
 
-// GEMM1 on node block kb: the KT rank rows of VT against the multipole fragments.
-template <int COLS, int KT, int KP16, int NPL>
-__device__ __forceinline__ void g1_block(f4 (&c1)[KT][COLS], const bf8 (&b)[3][COLS], const uint16_t* VT, int LD,
-                                         int col, int kq, int kb) {
+// U rows of GEMM2 step s = kb NTW + t: row tile wave + W t (a wave with fewer
+// real row tiles computes throw-away ones on row 0), K block kb, the planes
+// of the split.  U's rank columns 16 KT .. KP32 are zero padding: the lanes
+// that hold only padding get zeros without a load (half the bytes of an odd
+// KT's last block).
+template <int NTW, int KP16, int KP32, int NPL>
+__device__ __forceinline__ void load_urows(bf8 (&a)[3], const uint16_t* U, int LD, int wave, int col, int kq,
+                                           int s) {
+    const int kb = s / NTW, t = s % NTW;
+    const int row = 16 * (wave + kM2LWaves * t) + col;
+    const uint16_t* p = U + static_cast<int64_t>(row < LD ? row : 0) * KP32 + 32 * kb + 8 * kq;
+    a[0] = bf8{};
+    a[1] = bf8{};
+    a[2] = bf8{};
+    if (32 * kb + 8 * kq < KP16) {
+        a[0] = ld8(p);
+        a[1] = ld8(p + static_cast<int64_t>(LD) * KP32);
+        if constexpr (NPL == 3) a[2] = ld8(p + 2 * static_cast<int64_t>(LD) * KP32);
+    }
+}
+
+// GEMM2's B fragments of K block kb (the split sums of C), from the LDS.
+template <int COLS, int NPL>
+__device__ __forceinline__ void read_bfrags(bf8 (&b)[3][COLS], const bf8* Bs, int lane, int kb) {
 #pragma unroll
-    for (int rt = 0; rt < KT; ++rt) {
-        const uint16_t* a = VT + static_cast<int64_t>(16 * rt + col) * LD + 32 * kb + 8 * kq;
-        const bf8 ah = ld8(a), am = ld8(a + KP16 * LD), al = ld8(a + 2 * KP16 * LD);
-#pragma unroll
-        for (int ct = 0; ct < COLS; ++ct) c1[rt][ct] = mmaN<NPL>(ah, am, al, b[0][ct], b[1][ct], b[2][ct], c1[rt][ct]);
+    for (int ct = 0; ct < COLS; ++ct) {
+        const bf8* q = Bs + ((ct * 2 + kb) * 3) * 64 + lane;
+        b[0][ct] = q[0];
+        b[1][ct] = q[64];
+        b[2][ct] = NPL == 3 ? q[128] : bf8{};
     }
 }
 
@@ -308,6 +368,12 @@ __device__ __forceinline__ void mix_segment(MixState<P, COLS>& st, int i0, int i
     constexpr int KP16 = 16 * KT;       // GEMM1 rows: rank padded to 16
     constexpr int KB32 = (KT + 1) / 2;  // GEMM2 K blocks: rank padded to 32
     constexpr int KP32 = 32 * KB32;
+    // node blocks per wave: kb1 - kb0 is NBW for every wave when the block
+    // count divides over the waves, else NBW or NBW - 1
+    constexpr int KBP = MixState<P, COLS>::LDP / 32;
+    constexpr int NBW = (KBP + W - 1) / W;
+    constexpr bool EVEN = KBP % W == 0;
+    const int nbw = EVEN ? NBW : kb1 - kb0;
     for (int i = i0; i < i1; ++i) {
         const float* Mrow[COLS];
         bool any = false;
@@ -336,19 +402,27 @@ __device__ __forceinline__ void mix_segment(MixState<P, COLS>& st, int i0, int i
             // fragments of block kb + 1 and the VT rows of the next rank tile
             // are requested before the MFMAs that precede their use, so the
             // L2 latency overlaps the matrix work.
+            // The FP32 multipole fragments of block j are requested two blocks
+            // ahead (xr, a ring of two) and split during block j - 1's MFMAs
+            // (b3, a ring of two).  The block loop is unrolled: with a
+            // loop-carried c1 the compiler moves the accumulators between
+            // AGPRs and VGPRs every iteration.
+            f4 xr[2][COLS][2];
             bf8 b3[2][3][COLS];
             bf8 va[2][3];
             if (kb0 < kb1) {
                 load_mfrag<COLS>(b3[0], Mrow, kb0);
+                if (nbw > 1) fetch_mfrag<COLS>(xr[1], Mrow, kb0 + 1);
+                if (nbw > 2) fetch_mfrag<COLS>(xr[0], Mrow, kb0 + 2);
                 load_vfrag<KP16>(va[0], VT, LD, 0, col, kq, kb0);
             }
-            for (int kb = kb0; kb < kb1; kb += 2) {
-                const bool more = kb + 1 < kb1;
-                if (more) load_mfrag<COLS>(b3[1], Mrow, kb + 1);
-                g1_step<COLS, KT, KP16, NPL>(c1, b3[0], va, VT, LD, col, kq, kb, more);
-                if (more) {
-                    if (kb + 2 < kb1) load_mfrag<COLS>(b3[0], Mrow, kb + 2);
-                    g1_step<COLS, KT, KP16, NPL>(c1, b3[1], va, VT, LD, col, kq, kb + 1, kb + 2 < kb1);
+#pragma unroll
+            for (int j = 0; j < NBW; ++j) {
+                if (EVEN || j < nbw) {
+                    const bool more = j + 1 < nbw;
+                    g1_step<COLS, KT, KP16, NPL>(c1, b3[j & 1], va, VT, LD, col, kq, kb0 + j, more,
+                                                 b3[(j + 1) & 1], xr[(j + 1) & 1]);
+                    if (j + 3 < nbw) fetch_mfrag<COLS>(xr[(j + 1) & 1], Mrow, kb0 + j + 3);
                 }
             }
             // partial C as [wave][target][row]: lane (col, kq) holds rows
@@ -360,17 +434,14 @@ __device__ __forceinline__ void mix_segment(MixState<P, COLS>& st, int i0, int i
                 for (int ct = 0; ct < COLS; ++ct)
                     *reinterpret_cast<f4*>(Cw + (16 * ct + col) * CR + 16 * rt + 4 * kq) = c1[rt][ct];
         }
-        // this wave's first U rows of GEMM2 (row tile 0, K block 0), requested
-        // before the barriers instead of after them
-        bf8 u0[3] = {};
-        {
-            const uint16_t* a = U + static_cast<int64_t>(16 * wave + col < LD ? 16 * wave + col : 0) * KP32 + 8 * kq;
-            if (8 * kq < KP16) {
-                u0[0] = ld8(a);
-                u0[1] = ld8(a + static_cast<int64_t>(LD) * KP32);
-                if constexpr (NPL == 3) u0[2] = ld8(a + 2 * static_cast<int64_t>(LD) * KP32);
-            }
-        }
+        // GEMM2 runs in steps s = kb NTW + t (K block kb, row tile t).  The U
+        // rows of a step are requested two steps ahead into a ring of three,
+        // those of steps 0 and 1 before the barriers: with one wave per SIMD
+        // only this wave's own MFMAs can cover the L2 latency.
+        constexpr int NS = KB32 * NTW;
+        bf8 ub[3][3];
+        load_urows<NTW, KP16, KP32, NPL>(ub[0], U, LD, wave, col, kq, 0);
+        if constexpr (NS > 1) load_urows<NTW, KP16, KP32, NPL>(ub[1], U, LD, wave, col, kq, 1);
         __syncthreads();
         // Rows 16 KT .. KP32 of C are zero: the segments run with growing KT,
         // so no earlier slot wrote them, and the LDS was cleared at the start.
@@ -407,43 +478,17 @@ __device__ __forceinline__ void mix_segment(MixState<P, COLS>& st, int i0, int i
         f4 fresh[COLS];
 #pragma unroll
         for (int ct = 0; ct < COLS; ++ct) fresh[ct] = f4{0.f, 0.f, 0.f, 0.f};
+        bf8 b[3][COLS];  // B fragments of the current K block
 #pragma unroll
-        for (int kb = 0; kb < KB32; ++kb) {
-            bf8 bh[COLS], bm[COLS], bl[COLS];
-#pragma unroll
-            for (int ct = 0; ct < COLS; ++ct) {
-                const bf8* b = Bs + ((ct * 2 + kb) * 3) * 64 + lane;
-                bh[ct] = b[0];
-                bm[ct] = b[64];
-                bl[ct] = b[128];
-            }
-#pragma unroll
-            for (int t = 0; t < NTW; ++t) {
-                // a wave with fewer real row tiles computes throw-away ones on row 0
-                const int row = 16 * (wave + W * t) + col;
-                const uint16_t* a = U + static_cast<int64_t>(row < LD ? row : 0) * KP32 + 32 * kb + 8 * kq;
-                // U's rank columns 16 KT .. KP32 are zero padding: the lanes that
-                // hold only padding skip the load (half the bytes of an odd KT's
-                // last block)
-                const bool live = 32 * kb + 8 * kq < KP16;
-                bf8 ah = {}, am = {}, al = {};
-                if (kb == 0 && t == 0) {  // requested before the barriers
-                    ah = u0[0];
-                    am = u0[1];
-                    al = u0[2];
-                } else if (live) {
-                    ah = ld8(a);
-                    am = ld8(a + static_cast<int64_t>(LD) * KP32);
-                    if constexpr (NPL == 3) al = ld8(a + 2 * static_cast<int64_t>(LD) * KP32);
-                }
-#pragma unroll
-                for (int ct = 0; ct < COLS; ++ct) {
-                    if (t == 0)
-                        fresh[ct] = mmaN<NPL>(ah, am, al, bh[ct], bm[ct], bl[ct], fresh[ct]);
-                    else
-                        st.acc[t][ct] = mmaN<NPL>(ah, am, al, bh[ct], bm[ct], bl[ct], st.acc[t][ct]);
-                }
-            }
+        for (int s = 0; s < NS; ++s) {
+            const int kb = s / NTW, t = s % NTW;
+            if (s + 2 < NS) load_urows<NTW, KP16, KP32, NPL>(ub[(s + 2) % 3], U, LD, wave, col, kq, s + 2);
+            if (t == 0) read_bfrags<COLS, NPL>(b, Bs, lane, kb);
+            if (t == 0)
+                mmaN_cols<NPL, COLS>(ub[s % 3], b, fresh);
+            else
+                mmaN_cols<NPL, COLS>(ub[s % 3], b, st.acc[t]);
+            __builtin_amdgcn_sched_barrier(0);
         }
         // (a target without a source here keeps its sum and compensation as
         // they are, whatever the other targets of the tile do)
@@ -812,6 +857,7 @@ inline unsigned blocks_of(int64_t n) { return static_cast<unsigned>((n + 255) / 
 
 template <int P, int COLS, int NPL>
 int m2l_mixed_launch(Context* c) {
+    if (c->LD != MixState<P, COLS>::LDP) return fail(c, HFMM_ERR_ARG, "mixed M2L: padded node count");
     int tiles = 0;
     const M2LLevels lv = make_levels(c, 16 * COLS, tiles);
     if (tiles == 0) return HFMM_OK;  // no M2L level in the working range
