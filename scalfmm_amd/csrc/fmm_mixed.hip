@@ -277,10 +277,10 @@ __device__ __forceinline__ void load_vfrag(bf8 (&a)[3], const uint16_t* VT, int 
 
 // GEMM1 on node block kb with the VT rows rolling one rank tile ahead: va[0]
 // holds rank tile 0 of this block on entry and rank tile 0 of block kb + 1 on
-// exit (when `more`).  With `more`, the FP32 multipole fragments of block
-// kb + 1 (xn) are split into bn alongside the MFMAs, column tile ct in the
-// region of rank tile ct % KT, so the conversions overlap the matrix work.
-template <int COLS, int KT, int KP16, int NPL>
+// exit (when `more`).  With `more` and SPLIT, the FP32 multipole fragments of
+// block kb + 1 (xn) are split into bn alongside the MFMAs, column tile ct in
+// the region of rank tile ct % KT, so the conversions overlap the matrix work.
+template <int COLS, int KT, int KP16, int NPL, bool SPLIT = true>
 __device__ __forceinline__ void g1_step(f4 (&c1)[KT][COLS], const bf8 (&b)[3][COLS], bf8 (&va)[2][3],
                                         const uint16_t* VT, int LD, int col, int kq, int kb, bool more,
                                         bf8 (&bn)[3][COLS], const f4 (&xn)[COLS][2]) {
@@ -292,7 +292,7 @@ __device__ __forceinline__ void g1_step(f4 (&c1)[KT][COLS], const bf8 (&b)[3][CO
         else if (more)
             load_vfrag<KP16>(va[cur ^ 1], VT, LD, 0, col, kq, kb + 1);
         mmaN_cols<NPL, COLS>(va[cur], b, c1[rt]);
-        if (more) {
+        if (SPLIT && more) {
 #pragma unroll
             for (int ct = 0; ct < COLS; ++ct)
                 if (ct % KT == rt) split_mcol<COLS>(bn, xn, ct);
@@ -626,6 +626,27 @@ struct WaveMixState {
     f4 comp[NC][CT];
 };
 
+// U rows of the wave-level GEMM2 step s = t KB32 + sb (row tile t, K block
+// sb).  With an odd KT the last block's rank tile 2 sb + 1 is zero padding,
+// stored (interleaved) in elements 4..7 of every lane: only the first half is
+// loaded.
+template <int KT, int KP32, int KB32, int NPL>
+__device__ __forceinline__ void load_urows_wave(bf8 (&a)[3], const uint16_t* U, int LD, int col, int kq, int s) {
+    const int t = s / KB32, sb = s % KB32;
+    const uint16_t* p = U + static_cast<int64_t>(16 * t + col) * KP32 + 32 * sb + 8 * kq;
+    constexpr bool HALF = (KT & 1) != 0;
+    a[2] = bf8{};
+    if (HALF && sb == KB32 - 1) {
+        a[0] = ld4z(p);
+        a[1] = ld4z(p + static_cast<int64_t>(LD) * KP32);
+        if constexpr (NPL == 3) a[2] = ld4z(p + 2 * static_cast<int64_t>(LD) * KP32);
+    } else {
+        a[0] = ld8(p);
+        a[1] = ld8(p + static_cast<int64_t>(LD) * KP32);
+        if constexpr (NPL == 3) a[2] = ld8(p + 2 * static_cast<int64_t>(LD) * KP32);
+    }
+}
+
 template <int P, int CT, int KT, int NPL>
 __device__ __forceinline__ void wave_segment(WaveMixState<P, CT>& st, int i0, int i1, const int16_t* s_slot,
                                              const int64_t* s_off, const int32_t* lslots, const int (&tgt)[CT],
@@ -661,24 +682,34 @@ __device__ __forceinline__ void wave_segment(WaveMixState<P, CT>& st, int i0, in
         for (int rt = 0; rt < KT; ++rt)
 #pragma unroll
             for (int ct = 0; ct < CT; ++ct) c1[rt][ct] = f4{0.f, 0.f, 0.f, 0.f};
-        // GEMM1: fragments of block kb + 1 are requested before block kb's MFMAs
+        // GEMM1: the VT rows are requested one rank tile ahead (va).  The
+        // FP32 multipole fragments of block kb are requested two blocks ahead
+        // (xr) and split during block kb - 1's MFMAs (b3); with two tiles of
+        // three parts, whose registers are the tightest, one block ahead and
+        // split after block kb - 1's MFMAs.
+        constexpr bool X2 = !(CT == 2 && NPL == 3);
+        f4 xr[2][CT][2];
         bf8 b3[2][3][CT];
+        bf8 va[2][3];
         load_mfrag<CT>(b3[0], Mrow, 0);
+        if constexpr (X2 && KB > 1) fetch_mfrag<CT>(xr[1], Mrow, 1);
+        if constexpr (X2 && KB > 2) fetch_mfrag<CT>(xr[0], Mrow, 2);
+        load_vfrag<KP16>(va[0], VT, LD, 0, col, kq, 0);
 #pragma unroll
         for (int kb = 0; kb < KB; ++kb) {
-            if (kb + 1 < KB) load_mfrag<CT>(b3[(kb + 1) & 1], Mrow, kb + 1);
-#pragma unroll
-            for (int rt = 0; rt < KT; ++rt) {
-                bf8 a[3];
-                load_vfrag<KP16>(a, VT, LD, rt, col, kq, kb);
-#pragma unroll
-                for (int ct = 0; ct < CT; ++ct)
-                    c1[rt][ct] = mmaN<NPL>(a[0], a[1], a[2], b3[kb & 1][0][ct], b3[kb & 1][1][ct],
-                                           b3[kb & 1][2][ct], c1[rt][ct]);
+            if constexpr (X2) {
+                g1_step<CT, KT, KP16, NPL>(c1, b3[kb & 1], va, VT, LD, col, kq, kb, kb + 1 < KB, b3[(kb + 1) & 1],
+                                           xr[(kb + 1) & 1]);
+                if (kb + 3 < KB) fetch_mfrag<CT>(xr[(kb + 1) & 1], Mrow, kb + 3);
+            } else {
+                if (kb + 1 < KB) fetch_mfrag<CT>(xr[0], Mrow, kb + 1);
+                g1_step<CT, KT, KP16, NPL, false>(c1, b3[kb & 1], va, VT, LD, col, kq, kb, kb + 1 < KB,
+                                                  b3[(kb + 1) & 1], xr[0]);
+                if (kb + 1 < KB) split_mfrag<CT>(b3[(kb + 1) & 1], xr[0]);
             }
         }
         // GEMM2's B fragments straight from the accumulators, split in three
-        bf8 bh[KB32][CT], bm[KB32][CT], bl[KB32][CT];
+        bf8 bb[KB32][3][CT];
 #pragma unroll
         for (int sb = 0; sb < KB32; ++sb)
 #pragma unroll
@@ -689,42 +720,31 @@ __device__ __forceinline__ void wave_segment(WaveMixState<P, CT>& st, int i0, in
                     const float v = rt < KT ? c1[rt < KT ? rt : 0][ct][e & 3] : 0.f;
                     __bf16 h, m, l;
                     split3(v, h, m, l);
-                    bh[sb][ct][e] = h;
-                    bm[sb][ct][e] = m;
-                    bl[sb][ct][e] = l;
+                    bb[sb][0][ct][e] = h;
+                    bb[sb][1][ct][e] = m;
+                    bb[sb][2][ct][e] = l;
                 }
         f4 fresh[NC][CT];
 #pragma unroll
         for (int t = 0; t < NC; ++t)
 #pragma unroll
             for (int ct = 0; ct < CT; ++ct) fresh[t][ct] = f4{0.f, 0.f, 0.f, 0.f};
+        // GEMM2 in steps s = t KB32 + sb, the U rows requested UD steps ahead
+        constexpr int NS = NT * KB32;
+        constexpr int UD = (CT == 2 && NPL == 3) ? 1 : 2;  // one for the tightest registers
+        bf8 ub[UD + 1][3];
 #pragma unroll
-        for (int t = 0; t < NT; ++t) {
+        for (int s = 0; s < UD && s < NS; ++s) load_urows_wave<KT, KP32, KB32, NPL>(ub[s], U, LD, col, kq, s);
 #pragma unroll
-            for (int sb = 0; sb < KB32; ++sb) {
-                const uint16_t* a = U + static_cast<int64_t>(16 * t + col) * KP32 + 32 * sb + 8 * kq;
-                // with an odd KT the last block's rank tile 2 sb + 1 is zero padding,
-                // stored (interleaved) in elements 4..7 of every lane: load only the
-                // first half
-                constexpr bool HALF = (KT & 1) != 0;
-                bf8 ah, am, al = {};
-                if (HALF && sb == KB32 - 1) {
-                    ah = ld4z(a);
-                    am = ld4z(a + static_cast<int64_t>(LD) * KP32);
-                    if constexpr (NPL == 3) al = ld4z(a + 2 * static_cast<int64_t>(LD) * KP32);
-                } else {
-                    ah = ld8(a);
-                    am = ld8(a + static_cast<int64_t>(LD) * KP32);
-                    if constexpr (NPL == 3) al = ld8(a + 2 * static_cast<int64_t>(LD) * KP32);
-                }
-#pragma unroll
-                for (int ct = 0; ct < CT; ++ct) {
-                    if (t < NC)
-                        fresh[t][ct] = mmaN<NPL>(ah, am, al, bh[sb][ct], bm[sb][ct], bl[sb][ct], fresh[t][ct]);
-                    else
-                        st.acc[t][ct] = mmaN<NPL>(ah, am, al, bh[sb][ct], bm[sb][ct], bl[sb][ct], st.acc[t][ct]);
-                }
-            }
+        for (int s = 0; s < NS; ++s) {
+            const int t = s / KB32, sb = s % KB32;
+            if (s + UD < NS)
+                load_urows_wave<KT, KP32, KB32, NPL>(ub[(s + UD) % (UD + 1)], U, LD, col, kq, s + UD);
+            if (t < NC)
+                mmaN_cols<NPL, CT>(ub[s % (UD + 1)], bb[sb], fresh[t < NC ? t : 0]);
+            else
+                mmaN_cols<NPL, CT>(ub[s % (UD + 1)], bb[sb], st.acc[t]);
+            __builtin_amdgcn_sched_barrier(0);
         }
 #pragma unroll
         for (int t = 0; t < NC; ++t)
