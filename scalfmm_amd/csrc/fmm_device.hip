@@ -899,12 +899,14 @@ __global__ void k_mfma_probe(double* out) {
 // Sorted -> input order.  The passes accumulate the potential and the field
 // grad(phi); the force is q grad(phi) (the reference's convention,
 // FP2P.hpp:80-101: F_t += q_t q_s grad_t(1/r)), the field is kept as well.
-__global__ __launch_bounds__(256) void k_scatter(int64_t n, const int32_t* __restrict__ perm,
+// One thread per input index o reads its sorted index i = iperm[o]: the seven
+// outputs are written contiguously and the five inputs gathered.
+__global__ __launch_bounds__(256) void k_scatter(int64_t n, const int32_t* __restrict__ iperm,
                                                 const double* __restrict__ sp, const double* __restrict__ rs,
                                                 double* __restrict__ r, double* __restrict__ field) {
-    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const int64_t o = perm[i];
+    const int64_t o = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (o >= n) return;
+    const int64_t i = iperm[o];
     const double q = sp[3 * n + i];
     r[o] = rs[i];
     for (int d = 1; d < 4; ++d) {
@@ -1166,7 +1168,7 @@ int zero_results(Context* c) { return zero_device(c, sorted_results(c), sizeof(d
 
 int scatter_results(Context* c, int rhs) {
     prof_begin(c, "scatter");
-    hipLaunchKernelGGL(k_scatter, dim3(blocks256(c->n)), dim3(256), 0, c->stream, c->n, c->d_perm.as<int32_t>(),
+    hipLaunchKernelGGL(k_scatter, dim3(blocks256(c->n)), dim3(256), 0, c->stream, c->n, c->d_iperm.as<int32_t>(),
                        c->d_sorted.as<double>(), sorted_results(c),
                        c->d_res.as<double>() + static_cast<size_t>(rhs) * 4 * c->n,
                        c->d_field.as<double>() + static_cast<size_t>(rhs) * 3 * c->n);

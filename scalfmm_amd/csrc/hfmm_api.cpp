@@ -209,7 +209,7 @@ static void destroy_side(Context* c) {
 static void free_all(Context* c) {
     DevBuf* bufs[] = {&c->d_Troots, &c->d_child, &c->d_pidx, &c->d_pvec, &c->d_rank, &c->d_opoff, &c->d_ops,
                       &c->d_slotpos, &c->d_slotord, &c->d_slotseg, &c->d_posoff, &c->d_opm, &c->d_opm2, &c->d_in, &c->d_keys, &c->d_keys2, &c->d_vals, &c->d_vals2, &c->d_hist,
-                      &c->d_scan_part, &c->d_qin, &c->d_flags, &c->d_scan, &c->d_sorted, &c->d_perm, &c->d_pleaf,
+                      &c->d_scan_part, &c->d_qin, &c->d_flags, &c->d_scan, &c->d_sorted, &c->d_perm, &c->d_iperm, &c->d_pleaf,
                       &c->d_count, &c->d_cellkey, &c->d_cellkey_tmp, &c->d_childstart, &c->d_leafstart,
                       &c->d_p2p, &c->d_p2pitems, &c->d_m2l, &c->d_order, &c->d_pcount, &c->d_dense, &c->d_M, &c->d_L, &c->d_Mb, &c->d_loc, &c->d_res_sorted, &c->d_qall,
                       &c->d_res, &c->d_field, &c->d_setoff, &c->d_keep, &c->d_kept, &c->d_gBt, &c->d_gU, &c->d_gC, &c->d_gCr, &c->d_Mt, &c->d_Lt, &c->d_uroots, &c->d_uK, &c->d_utw,

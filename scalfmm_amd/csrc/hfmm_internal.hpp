@@ -123,6 +123,7 @@ struct Context {
     DevBuf d_flags, d_scan;                   // int32 per element
     DevBuf d_sorted;                          // sx, sy, sz, sq (4*n doubles)
     DevBuf d_perm;                            // int32 sorted -> input index
+    DevBuf d_iperm;                           // int32 input -> sorted index
     DevBuf d_pleaf;                           // int32 leaf of each sorted particle
     DevBuf d_count;                           // small device counters (int64)
     int64_t ncell[kMaxLevels] = {0};

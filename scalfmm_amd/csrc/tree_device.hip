@@ -301,6 +301,14 @@ __global__ __launch_bounds__(kBlock) void k_morton(int64_t n, const double* __re
     vals[i] = static_cast<int32_t>(i);
 }
 
+// The inverse of the sort permutation: input index -> sorted index.
+__global__ __launch_bounds__(kBlock) void k_invert_perm(int64_t n, const int32_t* __restrict__ perm,
+                                                       int32_t* __restrict__ iperm) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    iperm[perm[i]] = static_cast<int32_t>(i);
+}
+
 __global__ __launch_bounds__(kBlock) void k_gather_particles(int64_t n, const int32_t* __restrict__ idx,
                                                             const double* __restrict__ x, const double* __restrict__ y,
                                                             const double* __restrict__ z, const double* __restrict__ q,
@@ -614,6 +622,11 @@ int build_tree_device(Context* c) {
     HFMM_TRY(ensure(c, c->d_sorted, sizeof(double) * (4 * n + kSortedPad)));
     HFMM_TRY(ensure(c, c->d_perm, sizeof(int32_t) * n));
     HFMM_HIP(c, hipMemcpyAsync(c->d_perm.ptr, vals, sizeof(int32_t) * n, hipMemcpyDeviceToDevice, c->stream));
+    // the results are written back in input order through the inverse
+    HFMM_TRY(ensure(c, c->d_iperm, sizeof(int32_t) * n));
+    hipLaunchKernelGGL(k_invert_perm, dim3(blocks_for(n)), dim3(kBlock), 0, c->stream, n, vals,
+                       c->d_iperm.as<int32_t>());
+    HFMM_TRY(check_launch(c, "invert_perm"));
     prof_begin(c, "gather");
     hipLaunchKernelGGL(k_gather_particles, dim3(blocks_for(n)), dim3(kBlock), 0, c->stream, n, vals, c->x, c->y, c->z,
                        c->q, c->d_sorted.as<double>());
