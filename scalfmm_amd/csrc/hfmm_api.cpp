@@ -213,7 +213,7 @@ static void free_all(Context* c) {
                       &c->d_count, &c->d_cellkey, &c->d_cellkey_tmp, &c->d_childstart, &c->d_leafstart,
                       &c->d_p2p, &c->d_p2pitems, &c->d_m2l, &c->d_order, &c->d_pcount, &c->d_dense, &c->d_M, &c->d_L, &c->d_Mb, &c->d_loc, &c->d_res_sorted, &c->d_qall,
                       &c->d_res, &c->d_field, &c->d_setoff, &c->d_keep, &c->d_kept, &c->d_gBt, &c->d_gU, &c->d_gC, &c->d_gCr, &c->d_Mt, &c->d_Lt, &c->d_uroots, &c->d_uK, &c->d_utw,
-                      &c->d_p2pitems_m, &c->d_csoff, &c->d_csbase, &c->d_cpart};
+                      &c->d_p2pitems_m, &c->d_itemkeys, &c->d_itemidx, &c->d_items_tmp, &c->d_csoff, &c->d_csbase, &c->d_cpart};
     for (DevBuf* b : bufs) release(*b);
 }
 

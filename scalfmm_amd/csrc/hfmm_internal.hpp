@@ -144,7 +144,8 @@ struct Context {
     // (each slot holds one float4 per particle of the leaf)
     bool p2p_mutual = false;                  // tables built and the partial buffer allocated
     bool p2p_mutual_on = true;                // hfmm_set_p2p_mutual (the ordered-pair kernel when off)
-    DevBuf d_p2pitems_m;                      // int2 (leaf, first target)
+    DevBuf d_p2pitems_m;                      // int2 (leaf, first target), costliest first
+    DevBuf d_itemkeys, d_itemidx, d_items_tmp; // sorting scratch of the mutual work items
     int64_t p2p_items_m_cap = 0;
     DevBuf d_csoff;                           // int32 [nleaf][13]
     DevBuf d_csbase;                          // int64 [nleaf + 1]

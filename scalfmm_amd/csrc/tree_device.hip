@@ -486,6 +486,46 @@ __global__ __launch_bounds__(kBlock) void k_p2p_items(int nleaf, const int32_t* 
     for (int q = 0; q < k; ++q) items[off[i] + q] = make_int2(i, lstart[i] + q * ITEM);
 }
 
+// Sort key of a mutual work item: the work items are claimed costliest first,
+// so the grid ends on small items instead of waiting for a large one claimed
+// late.  Cost = targets x sources the item visits: its own leaf, its small
+// neighbours and its big forward neighbours (the backward ones are skipped).
+// Keys ascend with falling cost; the padding past the item count sorts last.
+template <int ITEM>
+__global__ __launch_bounds__(kBlock) void k_p2p_item_cost(int64_t cap, const int32_t* __restrict__ nitems,
+                                                         const int2* __restrict__ items,
+                                                         const int32_t* __restrict__ lstart,
+                                                         const int32_t* __restrict__ slots, uint64_t* __restrict__ keys,
+                                                         int32_t* __restrict__ vals) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= cap) return;
+    uint64_t key = 0xFFFFFFFFull;
+    if (i < *nitems) {
+        const int2 it = items[i];
+        const int leaf = it.x;
+        int64_t src = lstart[leaf + 1] - lstart[leaf];
+        for (int s = 0; s < kNbrSlots; ++s) {
+            const int nb = slots[static_cast<int64_t>(leaf) * kNbrSlots + s];
+            if (nb < 0) continue;
+            const int sz = lstart[nb + 1] - lstart[nb];
+            if (sz <= kP2PBig || s < kNbrForward) src += sz;
+        }
+        const int64_t tc = min(static_cast<int64_t>(ITEM), static_cast<int64_t>(lstart[leaf + 1] - it.y));
+        const uint64_t cost = min(static_cast<uint64_t>(tc * src), 0xFFFFFFFEull);
+        key = 0xFFFFFFFEull - cost;
+    }
+    keys[i] = key;
+    vals[i] = static_cast<int32_t>(i);
+}
+
+__global__ __launch_bounds__(kBlock) void k_gather_items(int64_t cap, const int32_t* __restrict__ nitems,
+                                                        const int32_t* __restrict__ idx, const int2* __restrict__ in,
+                                                        int2* __restrict__ out) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= cap || i >= *nitems) return;
+    out[i] = in[idx[i]];
+}
+
 // Slot tables of the mutual mixed P2P.  A work item of big leaf A (tile t of
 // kP2PMutualItem targets) evaluates A against its big forward neighbours
 // (slots s < 13) once for both sides; the source-side sums for the particles
@@ -777,6 +817,31 @@ int build_p2p_mutual_tables(Context* c) {
                        static_cast<int>(nl), lstart, c->d_scan.as<int32_t>(), c->d_p2pitems_m.as<int2>());
     prof_end(c);
     HFMM_TRY(check_launch(c, "p2p_items"));
+    {
+        // costliest first (a stable sort: equal costs keep the leaf order); a
+        // work item's slots follow from its leaf and first target, not from its
+        // place in the list, so the order changes no sum
+        const int64_t cap = c->p2p_items_m_cap;
+        HFMM_TRY(ensure(c, c->d_itemkeys, sizeof(uint64_t) * 2 * static_cast<size_t>(cap)));
+        HFMM_TRY(ensure(c, c->d_itemidx, sizeof(int32_t) * 2 * static_cast<size_t>(cap)));
+        HFMM_TRY(ensure(c, c->d_items_tmp, sizeof(int2) * static_cast<size_t>(cap)));
+        uint64_t* ik = c->d_itemkeys.as<uint64_t>();
+        int32_t* iv = c->d_itemidx.as<int32_t>();
+        prof_begin(c, "p2p_items");
+        hipLaunchKernelGGL(k_p2p_item_cost<kP2PMutualItem>, dim3(blocks_for(cap)), dim3(kBlock), 0, c->stream, cap,
+                           p2p_mutual_item_count(c), c->d_p2pitems_m.as<int2>(), lstart, c->d_p2p.as<int32_t>(), ik,
+                           iv);
+        prof_end(c);
+        HFMM_TRY(check_launch(c, "p2p_items"));
+        HFMM_TRY(radix_sort(c, ik, iv, ik + cap, iv + cap, cap, 32));
+        HFMM_HIP(c, hipMemcpyAsync(c->d_items_tmp.ptr, c->d_p2pitems_m.ptr, sizeof(int2) * static_cast<size_t>(cap),
+                                   hipMemcpyDeviceToDevice, c->stream));
+        prof_begin(c, "p2p_items");
+        hipLaunchKernelGGL(k_gather_items, dim3(blocks_for(cap)), dim3(kBlock), 0, c->stream, cap,
+                           p2p_mutual_item_count(c), iv, c->d_items_tmp.as<int2>(), c->d_p2pitems_m.as<int2>());
+        prof_end(c);
+        HFMM_TRY(check_launch(c, "p2p_items"));
+    }
     HFMM_TRY(ensure(c, c->d_csoff, sizeof(int32_t) * static_cast<size_t>(nl) * kNbrForward));
     HFMM_TRY(ensure(c, c->d_csbase, sizeof(int64_t) * static_cast<size_t>(nl + 1)));
     int64_t* base = c->d_csbase.as<int64_t>();
