@@ -486,12 +486,13 @@ __global__ __launch_bounds__(kBlock) void k_p2p_items(int nleaf, const int32_t* 
     for (int q = 0; q < k; ++q) items[off[i] + q] = make_int2(i, lstart[i] + q * ITEM);
 }
 
-// Sort key of a mutual work item: the work items are claimed costliest first,
-// so the grid ends on small items instead of waiting for a large one claimed
-// late.  Cost = targets x sources the item visits: its own leaf, its small
+// Sort key of a big-leaf P2P work item: the items are taken costliest first,
+// so the grid ends on small items instead of waiting for a large one taken
+// late.  Cost = targets x sources the item visits: its own leaf and every
+// neighbour (ordered pairs), or for the mutual kernel its own leaf, its small
 // neighbours and its big forward neighbours (the backward ones are skipped).
 // Keys ascend with falling cost; the padding past the item count sorts last.
-template <int ITEM>
+template <int ITEM, bool MUTUAL>
 __global__ __launch_bounds__(kBlock) void k_p2p_item_cost(int64_t cap, const int32_t* __restrict__ nitems,
                                                          const int2* __restrict__ items,
                                                          const int32_t* __restrict__ lstart,
@@ -508,7 +509,7 @@ __global__ __launch_bounds__(kBlock) void k_p2p_item_cost(int64_t cap, const int
             const int nb = slots[static_cast<int64_t>(leaf) * kNbrSlots + s];
             if (nb < 0) continue;
             const int sz = lstart[nb + 1] - lstart[nb];
-            if (sz <= kP2PBig || s < kNbrForward) src += sz;
+            if (!MUTUAL || sz <= kP2PBig || s < kNbrForward) src += sz;
         }
         const int64_t tc = min(static_cast<int64_t>(ITEM), static_cast<int64_t>(lstart[leaf + 1] - it.y));
         const uint64_t cost = min(static_cast<uint64_t>(tc * src), 0xFFFFFFFEull);
@@ -524,6 +525,33 @@ __global__ __launch_bounds__(kBlock) void k_gather_items(int64_t cap, const int3
     const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= cap || i >= *nitems) return;
     out[i] = in[idx[i]];
+}
+
+// Sorts the work items (count on the device, at most cap) costliest first, a
+// stable sort: equal costs keep the leaf order.  A work item's results (and
+// the mutual kernel's slots) follow from its leaf and first target, not from
+// its place in the list, so the order changes no sum.
+template <int ITEM, bool MUTUAL>
+int sort_p2p_items(Context* c, int2* items, const int32_t* nitems, int64_t cap) {
+    if (cap <= 1) return HFMM_OK;
+    HFMM_TRY(ensure(c, c->d_itemkeys, sizeof(uint64_t) * 2 * static_cast<size_t>(cap)));
+    HFMM_TRY(ensure(c, c->d_itemidx, sizeof(int32_t) * 2 * static_cast<size_t>(cap)));
+    HFMM_TRY(ensure(c, c->d_items_tmp, sizeof(int2) * static_cast<size_t>(cap)));
+    uint64_t* ik = c->d_itemkeys.as<uint64_t>();
+    int32_t* iv = c->d_itemidx.as<int32_t>();
+    prof_begin(c, "p2p_items");
+    hipLaunchKernelGGL((k_p2p_item_cost<ITEM, MUTUAL>), dim3(blocks_for(cap)), dim3(kBlock), 0, c->stream, cap, nitems,
+                       items, c->d_leafstart.as<int32_t>(), c->d_p2p.as<int32_t>(), ik, iv);
+    prof_end(c);
+    HFMM_TRY(check_launch(c, "p2p_items"));
+    HFMM_TRY(radix_sort(c, ik, iv, ik + cap, iv + cap, cap, 32));
+    HFMM_HIP(c, hipMemcpyAsync(c->d_items_tmp.ptr, items, sizeof(int2) * static_cast<size_t>(cap),
+                               hipMemcpyDeviceToDevice, c->stream));
+    prof_begin(c, "p2p_items");
+    hipLaunchKernelGGL(k_gather_items, dim3(blocks_for(cap)), dim3(kBlock), 0, c->stream, cap, nitems, iv,
+                       c->d_items_tmp.as<int2>(), items);
+    prof_end(c);
+    return check_launch(c, "p2p_items");
 }
 
 // Slot tables of the mutual mixed P2P.  A work item of big leaf A (tile t of
@@ -763,6 +791,9 @@ int build_tree_device(Context* c) {
                            c->d_leafstart.as<int32_t>(), c->d_scan.as<int32_t>(), c->d_p2pitems.as<int2>());
         prof_end(c);
         HFMM_TRY(check_launch(c, "p2p_items"));
+        // the ordered big-leaf kernels take the items at a fixed stride: costliest
+        // first spreads the large ones over the waves
+        HFMM_TRY((sort_p2p_items<kP2PItem, false>(c, c->d_p2pitems.as<int2>(), p2p_item_count(c), c->p2p_items_cap)));
     }
 
     // 6. M2L slots and parity-sorted target order of levels top..h-1
@@ -817,31 +848,8 @@ int build_p2p_mutual_tables(Context* c) {
                        static_cast<int>(nl), lstart, c->d_scan.as<int32_t>(), c->d_p2pitems_m.as<int2>());
     prof_end(c);
     HFMM_TRY(check_launch(c, "p2p_items"));
-    {
-        // costliest first (a stable sort: equal costs keep the leaf order); a
-        // work item's slots follow from its leaf and first target, not from its
-        // place in the list, so the order changes no sum
-        const int64_t cap = c->p2p_items_m_cap;
-        HFMM_TRY(ensure(c, c->d_itemkeys, sizeof(uint64_t) * 2 * static_cast<size_t>(cap)));
-        HFMM_TRY(ensure(c, c->d_itemidx, sizeof(int32_t) * 2 * static_cast<size_t>(cap)));
-        HFMM_TRY(ensure(c, c->d_items_tmp, sizeof(int2) * static_cast<size_t>(cap)));
-        uint64_t* ik = c->d_itemkeys.as<uint64_t>();
-        int32_t* iv = c->d_itemidx.as<int32_t>();
-        prof_begin(c, "p2p_items");
-        hipLaunchKernelGGL(k_p2p_item_cost<kP2PMutualItem>, dim3(blocks_for(cap)), dim3(kBlock), 0, c->stream, cap,
-                           p2p_mutual_item_count(c), c->d_p2pitems_m.as<int2>(), lstart, c->d_p2p.as<int32_t>(), ik,
-                           iv);
-        prof_end(c);
-        HFMM_TRY(check_launch(c, "p2p_items"));
-        HFMM_TRY(radix_sort(c, ik, iv, ik + cap, iv + cap, cap, 32));
-        HFMM_HIP(c, hipMemcpyAsync(c->d_items_tmp.ptr, c->d_p2pitems_m.ptr, sizeof(int2) * static_cast<size_t>(cap),
-                                   hipMemcpyDeviceToDevice, c->stream));
-        prof_begin(c, "p2p_items");
-        hipLaunchKernelGGL(k_gather_items, dim3(blocks_for(cap)), dim3(kBlock), 0, c->stream, cap,
-                           p2p_mutual_item_count(c), iv, c->d_items_tmp.as<int2>(), c->d_p2pitems_m.as<int2>());
-        prof_end(c);
-        HFMM_TRY(check_launch(c, "p2p_items"));
-    }
+    HFMM_TRY((sort_p2p_items<kP2PMutualItem, true>(c, c->d_p2pitems_m.as<int2>(), p2p_mutual_item_count(c),
+                                                    c->p2p_items_m_cap)));
     HFMM_TRY(ensure(c, c->d_csoff, sizeof(int32_t) * static_cast<size_t>(nl) * kNbrForward));
     HFMM_TRY(ensure(c, c->d_csbase, sizeof(int64_t) * static_cast<size_t>(nl + 1)));
     int64_t* base = c->d_csbase.as<int64_t>();
