@@ -398,15 +398,13 @@ __device__ __forceinline__ void mix_segment(MixState<P, COLS>& st, int i0, int i
             for (int rt = 0; rt < KT; ++rt)
 #pragma unroll
                 for (int ct = 0; ct < COLS; ++ct) c1[rt][ct] = f4{0.f, 0.f, 0.f, 0.f};
-            // Software pipeline over (node block, rank tile): the multipole
-            // fragments of block kb + 1 and the VT rows of the next rank tile
-            // are requested before the MFMAs that precede their use, so the
-            // L2 latency overlaps the matrix work.
-            // The FP32 multipole fragments of block j are requested two blocks
-            // ahead (xr, a ring of two) and split during block j - 1's MFMAs
-            // (b3, a ring of two).  The block loop is unrolled: with a
-            // loop-carried c1 the compiler moves the accumulators between
-            // AGPRs and VGPRs every iteration.
+            // Software pipeline over (node block, rank tile), so the L2
+            // latency overlaps the matrix work: the VT rows of the next rank
+            // tile are requested before the current MFMAs (g1_step); the FP32
+            // multipole fragments of block j two blocks ahead (xr, a ring of
+            // two), split during block j - 1's MFMAs (b3, a ring of two).  The
+            // block loop is unrolled: with a loop-carried c1 the compiler moves
+            // the accumulators between AGPRs and VGPRs every iteration.
             f4 xr[2][COLS][2];
             bf8 b3[2][3][COLS];
             bf8 va[2][3];
